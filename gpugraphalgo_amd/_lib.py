@@ -1,0 +1,136 @@
+"""ctypes binding of libgga.so (the C-ABI declared in include/gga.h).
+
+The shared library is built in-tree (gpugraphalgo_amd/lib/libgga.so) by
+gpugraphalgo_amd.build.build().  Loading fails loudly when it is missing:
+there is no Python or CPU fallback for any traversal.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgga.so")
+CLI_PATH = os.path.join(_HERE, "bin", "gga_run")
+
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int32
+P = ctypes.POINTER
+vp = ctypes.c_void_p
+
+GGA_OK = 0
+ERRORS = {
+    -1: "GGA_EINVAL", -2: "GGA_EIO", -3: "GGA_EFORMAT", -4: "GGA_ERANGE", -5: "GGA_ENOMEM",
+    -6: "GGA_EHIP", -7: "GGA_ENOSPLIT", -8: "GGA_ETIMEDOUT", -9: "GGA_EBUDGET", -10: "GGA_ENODEV",
+}
+INF = 0xFFFFFFFF
+REF_INF = 1000000000
+
+MODE_TOPO, MODE_WORKLIST, MODE_PERSISTENT, MODE_AUTO = 0, 1, 2, 3
+MODES = {"topo": MODE_TOPO, "worklist": MODE_WORKLIST, "persistent": MODE_PERSISTENT,
+         "auto": MODE_AUTO}
+
+
+class Csr(ctypes.Structure):
+    _fields_ = [("n", u32), ("m", u32), ("row_ptr", P(u32)), ("dst", P(u32)), ("w", P(u32)),
+                ("has_weights", u32)]
+
+
+class SplitStats(ctypes.Structure):
+    _fields_ = [("nnode", u32), ("nedge", u32), ("ns_nnode", u32), ("ns_nedge", u32),
+                ("extranode", u32), ("maxdeg", u32), ("level", i32), ("bucket", u32), ("mda", u32),
+                ("max_chunk", u32), ("split_ms", ctypes.c_float)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("mode", i32), ("inf", u32), ("max_iters", u32), ("block", u32),
+                ("sync_every", u32), ("use_graph", u32), ("count", u32), ("grid", u32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", u32), ("reached", u32), ("time_ms", ctypes.c_float),
+                ("ktime_ms", ctypes.c_float), ("sktime_ms", ctypes.c_float), ("mode", i32),
+                ("edges_inspected", u64), ("chunks_expanded", u64), ("relaxations", u64),
+                ("m_reach", u64), ("launches", u64)]
+
+
+class VerifyResult(ctypes.Structure):
+    _fields_ = [("nerr", u64), ("untight", u64), ("src_ok", u32), ("reached", u32)]
+
+
+class ResultRow(ctypes.Structure):
+    _fields_ = [("exename", ctypes.c_char * 100), ("algo", ctypes.c_char * 10),
+                ("graphfile", ctypes.c_char * 200), ("bucket", u32), ("nnode", u32),
+                ("nedge", u32), ("ns_nnode", u32), ("ns_nedge", u32), ("extranode", u32),
+                ("maxdeg", u32), ("level", i32), ("mda", u32), ("iterations", u32),
+                ("runtime_ms", u32), ("time_ms", ctypes.c_float), ("ktime_ms", ctypes.c_float),
+                ("sktime_ms", ctypes.c_float)]
+
+
+# (name, restype, argtypes) for every symbol include/gga.h declares
+SIGNATURES = [
+    ("gga_strerror", ctypes.c_char_p, [ctypes.c_int]),
+    ("gga_last_error", ctypes.c_char_p, []),
+    ("gga_version", ctypes.c_char_p, []),
+    ("gga_device_count", ctypes.c_int, []),
+    ("gga_csr_read_gr", ctypes.c_int, [ctypes.c_char_p, P(Csr)]),
+    ("gga_csr_write_gr", ctypes.c_int, [ctypes.c_char_p, P(Csr)]),
+    ("gga_csr_from_row_end", ctypes.c_int, [u32, u32, vp, vp, vp, P(Csr)]),
+    ("gga_csr_free", None, [P(Csr)]),
+    ("gga_csr_maxdeg", u32, [P(Csr), P(u32)]),
+    ("gga_gen_rmat", ctypes.c_int, [u32, u32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    u64, u32, u32, ctypes.c_int, P(Csr)]),
+    ("gga_gen_grid", ctypes.c_int, [u32, u32, ctypes.c_double, u64, u32, u32, P(Csr)]),
+    ("gga_graph_from_csr", ctypes.c_int, [u32, u32, vp, vp, vp, ctypes.c_int, P(vp)]),
+    ("gga_graph_from_csr32", ctypes.c_int, [u32, u32, vp, vp, vp, ctypes.c_int, P(vp)]),
+    ("gga_graph_load_gr", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, P(vp)]),
+    ("gga_graph_free", ctypes.c_int, [vp]),
+    ("gga_graph_n", u32, [vp]),
+    ("gga_graph_m", u32, [vp]),
+    ("gga_graph_device_bytes", u64, [vp]),
+    ("gga_graph_split", ctypes.c_int, [vp, u32, i32, u32, P(SplitStats)]),
+    ("gga_graph_chunks", ctypes.c_int, [vp, vp, vp]),
+    ("gga_opts_init", None, [P(Opts)]),
+    ("gga_bfs", ctypes.c_int, [vp, u32, vp, P(Opts), P(Stats)]),
+    ("gga_sssp", ctypes.c_int, [vp, u32, vp, P(Opts), P(Stats)]),
+    ("gga_graph_copy_dist", ctypes.c_int, [vp, vp]),
+    ("gga_graph_dist_device", vp, [vp]),
+    ("gga_verify", ctypes.c_int, [u32, vp, vp, vp, u32, vp, u32, P(VerifyResult)]),
+    ("gga_graph_verify", ctypes.c_int, [vp, u32, ctypes.c_int, u32, P(VerifyResult)]),
+    ("gga_write_dist", ctypes.c_int, [ctypes.c_char_p, vp, u32]),
+    ("gga_write_result_row", ctypes.c_int, [ctypes.c_char_p, P(ResultRow)]),
+    ("gga_graph_name", None, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+]
+
+_LIB = None
+
+
+class GgaError(RuntimeError):
+    def __init__(self, rc: int, where: str):
+        self.rc = rc
+        detail = ""
+        if _LIB is not None:
+            detail = (_LIB.gga_last_error() or b"").decode(errors="replace")
+        super().__init__(f"{where}: {ERRORS.get(rc, rc)} ({detail})")
+
+
+def load() -> ctypes.CDLL:
+    """Loads libgga.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run gpugraphalgo_amd.build.build() "
+                              "(there is no fallback path)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def check(rc: int, where: str) -> None:
+    if rc != GGA_OK:
+        raise GgaError(rc, where)

@@ -1,0 +1,67 @@
+/*
+ * gga_dev.h -- layout shared by the C host layer and the HIP kernels.
+ *
+ * HBM layout of one resident graph (all uint32, 0-based):
+ *   row[n + 1]      CSR offsets                      (reference psrc/noutgoing, graph.h:55)
+ *   col[m], w[m]    destinations, weights (SoA, so BFS streams col only)
+ *   cstart[n + 1]   first chunk of each vertex; cstart[n] = C
+ *                   (= the reference's renumbered parent id v + presum[v], graph.h:453)
+ *   cbeg[C + 1]     first edge of each chunk; chunks of a vertex are contiguous
+ *                   and consecutive chunks cover consecutive edges, so chunk c
+ *                   spans [cbeg[c], cbeg[c + 1])          (reference psrc - 1)
+ *   cown[C]         owning original vertex              (reference parentNode)
+ * Per-query workspace:
+ *   dist[n]         result, indexed by ORIGINAL vertex (no pseudo-children)
+ *   mark[n]         last step that queued the vertex (worklist de-duplication)
+ *   q[2][C]         chunk worklists (a vertex enters a list at most once per step)
+ *   ctl[]           control words, one 128-byte line per hot counter
+ */
+#ifndef GGA_DEV_H
+#define GGA_DEV_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gga_dev_graph {
+    const uint32_t *row;
+    const uint32_t *col;
+    const uint32_t *w;
+    const uint32_t *cstart;
+    const uint32_t *cbeg;
+    const uint32_t *cown;
+    uint32_t n, m, C, mda;
+} gga_dev_graph;
+
+typedef struct gga_dev_work {
+    uint32_t *dist;
+    uint32_t *mark;
+    uint32_t *q0;
+    uint32_t *q1;
+    uint32_t *ctl;
+    uint32_t inf;
+    uint32_t qcap;   /* capacity of q0 / q1 in entries (>= C) */
+} gga_dev_work;
+
+/* ctl word offsets (uint32 index); hot atomics sit on their own 128-byte line */
+#define GGA_CTL_LINE      32u
+#define GGA_CTL_TAIL(s)   ((s) * GGA_CTL_LINE)          /* s = 0..2: list length of a ring slot */
+#define GGA_CTL_BC(s)     ((3u + (s)) * GGA_CTL_LINE)   /* batch counter of a ring slot */
+#define GGA_CTL_ITER(s)   (6u * GGA_CTL_LINE + (s))     /* step number of a ring slot */
+#define GGA_CTL_CHANGED   (7u * GGA_CTL_LINE)           /* topology-mode `changed` flag */
+#define GGA_CTL_STEPS     (7u * GGA_CTL_LINE + 1u)      /* steps that had a non-empty list */
+#define GGA_CTL_ERR       (7u * GGA_CTL_LINE + 2u)      /* error bits (see GGA_DEVERR_*) */
+#define GGA_CTL_CNT       (8u * GGA_CTL_LINE)           /* 3 x uint64: edges, chunks, relaxations */
+#define GGA_CTL_BAR       (9u * GGA_CTL_LINE)           /* grid barrier arrival counter */
+#define GGA_CTL_BARGEN    (10u * GGA_CTL_LINE)          /* grid barrier generation */
+#define GGA_CTL_RED       (11u * GGA_CTL_LINE)          /* reductions: 4 x uint64 scratch */
+#define GGA_CTL_WORDS     (12u * GGA_CTL_LINE)
+
+#define GGA_DEVERR_SPIN   1u   /* barrier spin limit hit */
+#define GGA_DEVERR_QFULL  2u   /* worklist overflow (cannot happen with de-duplication) */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,55 @@
+/*
+ * gga_kernels.h -- thin C-ABI between the C host layer and the HIP kernels.
+ * Every launcher enqueues on `s` and returns the hipError_t of the launch.
+ */
+#ifndef GGA_KERNELS_H
+#define GGA_KERNELS_H
+#include <stdint.h>
+#ifndef __HIP_PLATFORM_AMD__
+#define __HIP_PLATFORM_AMD__ 1
+#endif
+#include <hip/hip_runtime_api.h>
+
+#include "gga_dev.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- split pass (split.hip) ---- */
+/* key = (maxdeg << 32) | ~first_index, atomically maxed into *d_key (zero it first). */
+int ggak_degree_max(const uint32_t *row, uint32_t n, uint64_t *d_key, hipStream_t s);
+/* float32 bucket histogram of getMaxAllowedOutDegree (graph.h:359-385); buckets <= 4096. */
+int ggak_degree_hist(const uint32_t *row, uint32_t n, uint32_t maxdeg, uint32_t buckets,
+                     uint32_t *d_hist, hipStream_t s);
+/* Scratch words ggak_split_scan needs. */
+uint32_t ggak_split_tmp_words(uint32_t n);
+/* cstart[v] = exclusive prefix of max(1, ceil(deg / mda)); cstart[n] = C. */
+int ggak_split_scan(const uint32_t *row, uint32_t n, uint32_t mda, uint32_t *cstart,
+                    uint32_t *d_tmp, hipStream_t s);
+/* Chunk descriptors cbeg[C + 1], cown[C]; *d_maxchunk (zeroed) receives the largest chunk. */
+int ggak_split_fill(const uint32_t *row, const uint32_t *cstart, uint32_t n, uint32_t m,
+                    uint32_t *cbeg, uint32_t *cown, uint32_t *d_maxchunk, hipStream_t s);
+
+/* ---- traversal (relax.hip) ---- */
+int ggak_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t s);
+int ggak_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src, uint32_t epoch, hipStream_t s);
+int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int count, hipStream_t s);
+int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t phase, int bfs, int count,
+                 uint32_t grid, hipStream_t s);
+/* Occupancy-bound grid (workgroups) for the worklist / persistent kernels. */
+uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count);
+/* One cooperative launch running worklist steps until the list is empty or
+ * max_steps; grid must not exceed ggak_wl_grid(dev, 1, ...). */
+int ggak_persistent(gga_dev_graph g, gga_dev_work w, int bfs, int count, uint32_t grid,
+                    uint32_t max_steps, uint32_t spin_limit_ms, hipStream_t s);
+/* reached count -> ctl[RED] (u64), m_reach -> ctl[RED + 2] (u64). */
+int ggak_reach(gga_dev_graph g, gga_dev_work w, hipStream_t s);
+/* nerr -> ctl[RED] (u64), untight -> ctl[RED + 2] (u64); d_tight: n bytes scratch. */
+int ggak_verify(gga_dev_graph g, gga_dev_work w, uint32_t src, int bfs, uint8_t *d_tight,
+                hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,28 @@
+/* gga_host.h -- internal declarations of the C host layer. */
+#ifndef GGA_HOST_H
+#define GGA_HOST_H
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../../include/gga.h"
+
+#define GGA_ERRBUF 512
+
+/* thread-local detail for gga_last_error() */
+void gga_set_error(const char *fmt, ...);
+void gga_clear_error(void);
+
+/* 64-bit key radix sort (LSD, 16-bit digits over the low `bits` bits). */
+int gga_radix_sort_u64(uint64_t *keys, uint64_t count, unsigned bits);
+/* Builds CSR from sorted (src << 32 | dst) keys; keys must be sorted. */
+int gga_csr_from_sorted_keys(uint32_t n, const uint64_t *keys, uint64_t count, gga_csr *out);
+
+static inline uint64_t gga_splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+#endif

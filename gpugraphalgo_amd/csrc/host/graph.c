@@ -1,0 +1,681 @@
+/*
+ * graph.c -- device-resident graphs, the split driver and the traversal
+ * drivers of the C-ABI (include/gga.h).
+ *
+ * Replaces, from Programs/CUDA_src/Kepler_BFS_SSSP/:
+ *   Graph::allocOnDevice / cudaCopy / CudaTest   graph.h:523-553, 683-699
+ *   the split driver + MDA selection             bfs_sssp.cu:224-253, graph.h:338-522
+ *   distance init, fixpoint loop, timing         bfs_sssp.cu:302-321, 345-473
+ * The host never renumbers the graph: node splitting is the chunk table the
+ * split kernels build on the device (kernels/split.hip).
+ */
+#define _GNU_SOURCE
+#include <stdlib.h>
+#include <string.h>
+
+#include "../gga_kernels.h"
+#include "gga_host.h"
+
+#define GGA_VERSION "gga 0.1.0 (gfx950)"
+
+struct gga_graph {
+    int device;
+    hipStream_t stream;
+    uint32_t n, m;
+    uint32_t *d_row, *d_col, *d_w;
+    uint32_t maxdeg, maxdeg_idx;
+    /* split */
+    int split;
+    uint32_t C, mda;
+    uint32_t *d_cstart, *d_cbeg, *d_cown;
+    gga_split_stats sst;
+    /* query workspace */
+    uint32_t *d_dist, *d_mark, *d_q0, *d_q1, *d_ctl;
+    uint32_t qcap;
+    uint32_t *h_pin;          /* pinned: GGA_CTL_WORDS words */
+    uint32_t epoch;
+    int have_result;
+    uint32_t wl_grid[2][2];   /* [bfs][count] */
+    uint32_t p_grid[2][2];
+    hipEvent_t ev[4];
+    hipEvent_t *kev;          /* per-launch timing events */
+    uint32_t kev_cap;
+    uint64_t dev_bytes;
+};
+
+const char *gga_version(void) { return GGA_VERSION; }
+
+int gga_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            gga_set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); \
+            rc = (e_ == hipErrorOutOfMemory || e_ == hipErrorMemoryAllocation) ? GGA_ENOMEM \
+                                                                                : GGA_EHIP; \
+            goto fail;                                                                     \
+        }                                                                                  \
+    } while (0)
+
+#define K_TRY(expr)                                                                        \
+    do {                                                                                   \
+        int e_ = (expr);                                                                   \
+        if (e_ != 0) {                                                                     \
+            gga_set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,                       \
+                          hipGetErrorString((hipError_t)e_));                              \
+            rc = GGA_EHIP;                                                                 \
+            goto fail;                                                                     \
+        }                                                                                  \
+    } while (0)
+
+static int dmalloc(gga_graph *g, void **p, uint64_t bytes)
+{
+    *p = NULL;
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        gga_set_error("hipMalloc %llu bytes: %s", (unsigned long long)bytes, hipGetErrorString(e));
+        *p = NULL;
+        return GGA_ENOMEM;
+    }
+    g->dev_bytes += bytes;
+    return GGA_OK;
+}
+
+static void dfree(gga_graph *g, void **p, uint64_t bytes)
+{
+    if (*p) {
+        hipFree(*p);
+        if (bytes == 0) bytes = 4;
+        g->dev_bytes -= bytes;
+    }
+    *p = NULL;
+}
+
+static gga_dev_graph dev_graph(const gga_graph *g)
+{
+    gga_dev_graph d;
+    d.row = g->d_row;
+    d.col = g->d_col;
+    d.w = g->d_w;
+    d.cstart = g->d_cstart;
+    d.cbeg = g->d_cbeg;
+    d.cown = g->d_cown;
+    d.n = g->n;
+    d.m = g->m;
+    d.C = g->C;
+    d.mda = g->mda;
+    return d;
+}
+
+static gga_dev_work dev_work(const gga_graph *g, uint32_t inf)
+{
+    gga_dev_work w;
+    w.dist = g->d_dist;
+    w.mark = g->d_mark;
+    w.q0 = g->d_q0;
+    w.q1 = g->d_q1;
+    w.ctl = g->d_ctl;
+    w.inf = inf;
+    w.qcap = g->qcap;
+    return w;
+}
+
+uint32_t gga_graph_n(const gga_graph *g) { return g ? g->n : 0; }
+uint32_t gga_graph_m(const gga_graph *g) { return g ? g->m : 0; }
+uint64_t gga_graph_device_bytes(const gga_graph *g) { return g ? g->dev_bytes : 0; }
+
+static void free_split(gga_graph *g)
+{
+    dfree(g, (void **)&g->d_cstart, 4ull * ((uint64_t)g->n + 1));
+    dfree(g, (void **)&g->d_cbeg, 4ull * ((uint64_t)g->C + 1));
+    dfree(g, (void **)&g->d_cown, 4ull * g->C);
+    g->split = 0;
+}
+
+static void free_queues(gga_graph *g)
+{
+    dfree(g, (void **)&g->d_q0, 4ull * g->qcap);
+    dfree(g, (void **)&g->d_q1, 4ull * g->qcap);
+    g->qcap = 0;
+}
+
+int gga_graph_free(gga_graph *g)
+{
+    if (!g) return GGA_OK;
+    hipSetDevice(g->device);
+    if (g->stream) hipStreamSynchronize(g->stream);
+    dfree(g, (void **)&g->d_row, 4ull * ((uint64_t)g->n + 1));
+    dfree(g, (void **)&g->d_col, 4ull * g->m);
+    dfree(g, (void **)&g->d_w, 4ull * g->m);
+    free_split(g);
+    free_queues(g);
+    dfree(g, (void **)&g->d_dist, 4ull * g->n);
+    dfree(g, (void **)&g->d_mark, 4ull * g->n);
+    dfree(g, (void **)&g->d_ctl, 4ull * GGA_CTL_WORDS);
+    if (g->h_pin) hipHostFree(g->h_pin);
+    for (int i = 0; i < 4; i++)
+        if (g->ev[i]) hipEventDestroy(g->ev[i]);
+    for (uint32_t i = 0; i < g->kev_cap; i++)
+        if (g->kev[i]) hipEventDestroy(g->kev[i]);
+    free(g->kev);
+    if (g->stream) hipStreamDestroy(g->stream);
+    free(g);
+    return GGA_OK;
+}
+
+static int graph_create(uint32_t n, uint32_t m, const uint32_t *row_ptr, const uint32_t *dst,
+                        const uint32_t *w, int device, gga_graph **out)
+{
+    int rc = GGA_OK;
+    gga_clear_error();
+    if (!out) return GGA_EINVAL;
+    *out = NULL;
+    if (n == 0 || n == UINT32_MAX) return n ? GGA_ERANGE : GGA_EINVAL;
+    if ((uint64_t)n + (uint64_t)m >= UINT32_MAX) {
+        gga_set_error("n + m = %llu: chunk ids would exceed uint32", (unsigned long long)n + m);
+        return GGA_ERANGE;
+    }
+    int ndev = gga_device_count();
+    if (ndev <= 0) {
+        gga_set_error("no HIP device visible");
+        return GGA_ENODEV;
+    }
+    if (device < 0 || device >= ndev) return GGA_EINVAL;
+    gga_graph *g = (gga_graph *)calloc(1, sizeof *g);
+    if (!g) return GGA_ENOMEM;
+    g->device = device;
+    g->n = n;
+    g->m = m;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&g->ev[i]));
+    HIP_TRY(hipHostMalloc((void **)&g->h_pin, 4ull * GGA_CTL_WORDS, hipHostMallocDefault));
+    if ((rc = dmalloc(g, (void **)&g->d_row, 4ull * ((uint64_t)n + 1)))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_col, 4ull * m))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_w, 4ull * m))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_ctl, 4ull * GGA_CTL_WORDS))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_dist, 4ull * n))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_mark, 4ull * n))) goto fail;
+    HIP_TRY(hipMemcpyAsync(g->d_row, row_ptr, 4ull * ((uint64_t)n + 1), hipMemcpyHostToDevice, g->stream));
+    if (m) HIP_TRY(hipMemcpyAsync(g->d_col, dst, 4ull * m, hipMemcpyHostToDevice, g->stream));
+    if (m && w) HIP_TRY(hipMemcpyAsync(g->d_w, w, 4ull * m, hipMemcpyHostToDevice, g->stream));
+    else if (m) K_TRY(ggak_fill_u32(g->d_w, 1u, m, g->stream));
+    K_TRY(ggak_fill_u32(g->d_ctl, 0u, GGA_CTL_WORDS, g->stream));
+    K_TRY(ggak_fill_u32(g->d_mark, 0xFFFFFFFFu, n, g->stream));
+    /* max degree + first index on the device (graph.h:338-358) */
+    {
+        uint64_t *key = (uint64_t *)(g->d_ctl + GGA_CTL_RED);
+        K_TRY(ggak_degree_max(g->d_row, n, key, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->h_pin, key, 8, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        uint64_t k = (uint64_t)g->h_pin[0] | ((uint64_t)g->h_pin[1] << 32);
+        g->maxdeg = (uint32_t)(k >> 32);
+        g->maxdeg_idx = k ? 0xFFFFFFFFu - (uint32_t)k : 0u;
+        K_TRY(ggak_fill_u32(g->d_ctl, 0u, GGA_CTL_WORDS, g->stream));
+    }
+    /* validate destinations < n on the host side was the caller's job; check
+     * row_ptr shape here cheaply */
+    if (row_ptr[0] != 0 || row_ptr[n] != m) {
+        gga_set_error("row_ptr[0]=%u row_ptr[n]=%u m=%u", row_ptr[0], row_ptr[n], m);
+        rc = GGA_EFORMAT;
+        goto fail;
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    *out = g;
+    return GGA_OK;
+fail:
+    gga_graph_free(g);
+    return rc;
+}
+
+static int check_csr(uint32_t n, uint32_t m, const uint32_t *row_ptr, const uint32_t *dst)
+{
+    if (row_ptr[0] != 0 || row_ptr[n] != m) return GGA_EFORMAT;
+    for (uint32_t v = 0; v < n; v++)
+        if (row_ptr[v + 1] < row_ptr[v]) {
+            gga_set_error("row_ptr not monotone at %u", v);
+            return GGA_EFORMAT;
+        }
+    for (uint32_t e = 0; e < m; e++)
+        if (dst[e] >= n) {
+            gga_set_error("edge %u: dst %u >= n %u", e, dst[e], n);
+            return GGA_EFORMAT;
+        }
+    return GGA_OK;
+}
+
+int gga_graph_from_csr32(uint32_t n, uint32_t m, const uint32_t *row_ptr, const uint32_t *dst,
+                         const uint32_t *w, int device, gga_graph **out)
+{
+    if (!out || !row_ptr || (m && !dst)) return GGA_EINVAL;
+    if (n == 0) return GGA_EINVAL;
+    int rc = check_csr(n, m, row_ptr, dst);
+    if (rc) return rc;
+    return graph_create(n, m, row_ptr, dst, w, device, out);
+}
+
+int gga_graph_from_csr(uint32_t n, uint32_t m, const uint64_t *row_end, const uint32_t *dst,
+                       const uint32_t *w, int device, gga_graph **out)
+{
+    gga_csr c;
+    int rc = gga_csr_from_row_end(n, m, row_end, dst, w, &c);
+    if (rc) return rc;
+    rc = graph_create(c.n, c.m, c.row_ptr, c.dst, w ? c.w : NULL, device, out);
+    gga_csr_free(&c);
+    return rc;
+}
+
+int gga_graph_load_gr(const char *path, int device, gga_graph **out)
+{
+    gga_csr c;
+    int rc = gga_csr_read_gr(path, &c);
+    if (rc) return rc;
+    rc = graph_create(c.n, c.m, c.row_ptr, c.dst, c.has_weights ? c.w : NULL, device, out);
+    gga_csr_free(&c);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ split */
+static int ensure_queues(gga_graph *g, uint32_t cap)
+{
+    if (g->qcap >= cap && g->d_q0) return GGA_OK;
+    free_queues(g);
+    int rc = dmalloc(g, (void **)&g->d_q0, 4ull * cap);
+    if (!rc) rc = dmalloc(g, (void **)&g->d_q1, 4ull * cap);
+    if (rc) {
+        free_queues(g);
+        return rc;
+    }
+    g->qcap = cap;
+    return GGA_OK;
+}
+
+int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets, gga_split_stats *st)
+{
+    int rc = GGA_OK;
+    uint32_t *d_tmp = NULL;
+    uint32_t tmp_words = 0;
+    if (!g) return GGA_EINVAL;
+    gga_clear_error();
+    HIP_TRY(hipSetDevice(g->device));
+    uint32_t T;
+    uint32_t bucket_rec = 0;
+    int32_t level_rec = level;
+    if (mda > 0) {
+        T = mda;
+        level_rec = 0;
+    } else if (level > 0) {
+        T = (g->maxdeg + (uint32_t)level - 1u) / (uint32_t)level;   /* graph.h:390 */
+        if (T == 0) T = 1;                                           /* no edges at all */
+    } else if (level < 0) {
+        uint32_t b = buckets ? buckets : 10u;                        /* bfs_sssp.cu:166 */
+        if (b > 4096) return GGA_ERANGE;
+        bucket_rec = b;
+        uint32_t *hist = NULL;
+        rc = dmalloc(g, (void **)&hist, 4ull * b);
+        if (rc) return rc;
+        K_TRY(ggak_fill_u32(hist, 0u, b, g->stream));
+        K_TRY(ggak_degree_hist(g->d_row, g->n, g->maxdeg, b, hist, g->stream));
+        uint32_t *h = (uint32_t *)malloc(4ull * b);
+        if (!h) {
+            dfree(g, (void **)&hist, 4ull * b);
+            return GGA_ENOMEM;
+        }
+        HIP_TRY(hipMemcpyAsync(h, hist, 4ull * b, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        dfree(g, (void **)&hist, 4ull * b);
+        /* graph.h:375-384: most populated bucket, first on ties */
+        int max = -1, maxindex = -1;
+        for (uint32_t i = 0; i < b; i++)
+            if ((int)h[i] > max) {
+                max = (int)h[i];
+                maxindex = (int)i;
+            }
+        free(h);
+        maxindex++;
+        int maxdeg1 = (int)g->maxdeg + 1;
+        T = (uint32_t)((maxdeg1 / (int)b) * maxindex);
+        if (T == 0) {
+            if (g->maxdeg == 0) T = 1;
+            else {
+                gga_set_error("histogram threshold is 0 (maxdeg %u, %u buckets)", g->maxdeg, b);
+                return GGA_EINVAL;
+            }
+        }
+    } else {
+        /* L = 0: no node split (bfs_sssp.cu:170-171); one chunk per vertex */
+        T = g->maxdeg ? g->maxdeg : 1u;
+    }
+    free_split(g);
+    g->have_result = 0;
+    tmp_words = ggak_split_tmp_words(g->n);
+    if ((rc = dmalloc(g, (void **)&d_tmp, 4ull * tmp_words))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_cstart, 4ull * ((uint64_t)g->n + 1)))) goto fail;
+    HIP_TRY(hipEventRecord(g->ev[0], g->stream));
+    K_TRY(ggak_split_scan(g->d_row, g->n, T, g->d_cstart, d_tmp, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_cstart + g->n, 4, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->C = g->h_pin[0];
+    if ((rc = dmalloc(g, (void **)&g->d_cbeg, 4ull * ((uint64_t)g->C + 1)))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_cown, 4ull * g->C))) goto fail;
+    {
+        uint32_t *d_max = d_tmp;  /* scan scratch is free again */
+        K_TRY(ggak_fill_u32(d_max, 0u, 1, g->stream));
+        K_TRY(ggak_split_fill(g->d_row, g->d_cstart, g->n, g->m, g->d_cbeg, g->d_cown, d_max, g->stream));
+        HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+        HIP_TRY(hipMemcpyAsync(g->h_pin, d_max, 4, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    dfree(g, (void **)&d_tmp, 4ull * tmp_words);
+    g->mda = T;
+    g->split = 1;
+    memset(&g->sst, 0, sizeof g->sst);
+    g->sst.nnode = g->n;
+    g->sst.nedge = g->m;
+    g->sst.ns_nnode = g->C;
+    g->sst.ns_nedge = g->m;
+    g->sst.extranode = g->C - g->n;
+    g->sst.maxdeg = g->maxdeg;
+    g->sst.level = level_rec;
+    g->sst.bucket = bucket_rec;
+    g->sst.mda = T;
+    g->sst.max_chunk = g->h_pin[0];
+    HIP_TRY(hipEventElapsedTime(&g->sst.split_ms, g->ev[0], g->ev[1]));
+    if ((rc = ensure_queues(g, g->C))) goto fail;
+    if (st) *st = g->sst;
+    return GGA_OK;
+fail:
+    dfree(g, (void **)&d_tmp, 4ull * tmp_words);
+    if (rc != GGA_OK) free_split(g);
+    return rc;
+}
+
+int gga_graph_chunks(gga_graph *g, uint32_t *chunk_begin, uint32_t *owner)
+{
+    int rc = GGA_OK;
+    if (!g) return GGA_EINVAL;
+    if (!g->split) return GGA_ENOSPLIT;
+    HIP_TRY(hipSetDevice(g->device));
+    if (chunk_begin)
+        HIP_TRY(hipMemcpyAsync(chunk_begin, g->d_cbeg, 4ull * ((uint64_t)g->C + 1), hipMemcpyDeviceToHost,
+                               g->stream));
+    if (owner) HIP_TRY(hipMemcpyAsync(owner, g->d_cown, 4ull * g->C, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return GGA_OK;
+fail:
+    return rc;
+}
+
+/* -------------------------------------------------------------- traversal */
+void gga_opts_init(gga_opts *o)
+{
+    if (o) memset(o, 0, sizeof *o);
+}
+
+static int ensure_kev(gga_graph *g, uint32_t need)
+{
+    if (need <= g->kev_cap) return GGA_OK;
+    uint32_t cap = g->kev_cap ? g->kev_cap : 64;
+    while (cap < need) cap *= 2;
+    hipEvent_t *nk = (hipEvent_t *)realloc(g->kev, sizeof(hipEvent_t) * cap);
+    if (!nk) return GGA_ENOMEM;
+    g->kev = nk;
+    for (uint32_t i = g->kev_cap; i < cap; i++) {
+        if (hipEventCreate(&g->kev[i]) != hipSuccess) {
+            g->kev_cap = i;
+            return GGA_EHIP;
+        }
+    }
+    g->kev_cap = cap;
+    return GGA_OK;
+}
+
+static uint32_t wl_grid(gga_graph *g, int persistent, int bfs, int count)
+{
+    uint32_t *slot = persistent ? &g->p_grid[bfs][count] : &g->wl_grid[bfs][count];
+    if (*slot == 0) *slot = ggak_wl_grid(g->device, persistent, bfs, count);
+    return *slot;
+}
+
+static int choose_mode(const gga_graph *g)
+{
+    (void)g;
+    return GGA_MODE_WORKLIST;
+}
+
+static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gga_opts *user,
+                    gga_stats *st)
+{
+    int rc = GGA_OK;
+    gga_opts o;
+    gga_stats s;
+    memset(&s, 0, sizeof s);
+    if (user) o = *user;
+    else gga_opts_init(&o);
+    if (!g) return GGA_EINVAL;
+    gga_clear_error();
+    if (src >= g->n) {
+        gga_set_error("source %u >= n %u", src, g->n);
+        return GGA_EINVAL;
+    }
+    if (!g->split) {
+        rc = gga_graph_split(g, 0, 0, 0, NULL);   /* no split: one chunk per vertex */
+        if (rc) return rc;
+    }
+    const uint32_t inf = o.inf ? o.inf : GGA_INF;
+    const uint32_t max_iters = o.max_iters ? o.max_iters : (g->n + 1u);
+    int mode = o.mode;
+    if (mode == GGA_MODE_AUTO || mode < 0 || mode > GGA_MODE_AUTO) mode = choose_mode(g);
+    const int count = o.count ? 1 : 0;
+    HIP_TRY(hipSetDevice(g->device));
+    if ((rc = ensure_queues(g, g->C))) return rc;
+    /* step stamps keep growing across queries; refill before they wrap */
+    if (g->epoch > 0xF0000000u) {
+        K_TRY(ggak_fill_u32(g->d_mark, 0xFFFFFFFFu, g->n, g->stream));
+        g->epoch = 0;
+    }
+    gga_dev_graph dg = dev_graph(g);
+    gga_dev_work dw = dev_work(g, inf);
+    s.mode = mode;
+    float ktime = 0.f;
+
+    HIP_TRY(hipEventRecord(g->ev[0], g->stream));              /* bfs_sssp.cu:350 */
+    K_TRY(ggak_query_init(dg, dw, src, g->epoch, g->stream));
+    if (mode == GGA_MODE_TOPO) {
+        /* the reference's loop: changed = false; relax; read changed (bfs_sssp.cu:372-468) */
+        if ((rc = ensure_kev(g, 2))) goto fail;
+        uint32_t it = 0;
+        for (;;) {
+            it++;
+            g->h_pin[0] = 0;
+            HIP_TRY(hipMemcpyAsync(g->d_ctl + GGA_CTL_CHANGED, g->h_pin, 4, hipMemcpyHostToDevice, g->stream));
+            HIP_TRY(hipEventRecord(g->kev[0], g->stream));
+            K_TRY(ggak_topo_step(dg, dw, bfs, count, g->stream));
+            HIP_TRY(hipEventRecord(g->kev[1], g->stream));
+            HIP_TRY(hipMemcpyAsync(g->h_pin + 1, g->d_ctl + GGA_CTL_CHANGED, 4, hipMemcpyDeviceToHost, g->stream));
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, g->kev[0], g->kev[1]));
+            ktime += t;
+            s.launches++;
+            if (!g->h_pin[1]) break;
+            if (it >= max_iters) {
+                rc = GGA_ETIMEDOUT;
+                gga_set_error("topology mode: %u iterations without convergence", it);
+                goto fail;
+            }
+        }
+        s.iterations = it;
+    } else if (mode == GGA_MODE_WORKLIST) {
+        const uint32_t grid = o.grid ? o.grid : wl_grid(g, 0, bfs, count);
+        if (grid == 0) {
+            rc = GGA_EHIP;
+            gga_set_error("occupancy query failed");
+            goto fail;
+        }
+        uint32_t per = o.sync_every ? o.sync_every : 8u;
+        if ((rc = ensure_kev(g, 2 * per))) goto fail;
+        uint32_t step = 0;
+        for (;;) {
+            for (uint32_t k = 0; k < per; k++) {
+                HIP_TRY(hipEventRecord(g->kev[2 * k], g->stream));
+                K_TRY(ggak_wl_step(dg, dw, (step + k) % 6u, bfs, count, grid, g->stream));
+                HIP_TRY(hipEventRecord(g->kev[2 * k + 1], g->stream));
+            }
+            step += per;
+            HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_TAIL(step % 3u), 4, hipMemcpyDeviceToHost,
+                                   g->stream));
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            for (uint32_t k = 0; k < per; k++) {
+                float t = 0.f;
+                HIP_TRY(hipEventElapsedTime(&t, g->kev[2 * k], g->kev[2 * k + 1]));
+                ktime += t;
+            }
+            s.launches += per;
+            if (g->h_pin[0] == 0) break;
+            if (step >= max_iters) {
+                rc = GGA_ETIMEDOUT;
+                gga_set_error("worklist mode: %u steps without convergence", step);
+                goto fail;
+            }
+        }
+    } else { /* GGA_MODE_PERSISTENT */
+        const uint32_t maxg = wl_grid(g, 1, bfs, count);
+        uint32_t grid = o.grid ? o.grid : maxg;
+        if (grid > maxg) grid = maxg;
+        if (grid == 0) {
+            rc = GGA_EHIP;
+            gga_set_error("occupancy query failed");
+            goto fail;
+        }
+        if ((rc = ensure_kev(g, 2))) goto fail;
+        HIP_TRY(hipEventRecord(g->kev[0], g->stream));
+        K_TRY(ggak_persistent(dg, dw, bfs, count, grid, max_iters, 10000u, g->stream));
+        HIP_TRY(hipEventRecord(g->kev[1], g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipEventElapsedTime(&ktime, g->kev[0], g->kev[1]));
+        s.launches = 1;
+    }
+    HIP_TRY(hipEventRecord(g->ev[1], g->stream));              /* bfs_sssp.cu:469 */
+    K_TRY(ggak_reach(dg, dw, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(hipEventElapsedTime(&s.time_ms, g->ev[0], g->ev[1]));
+    {
+        const uint32_t *c = g->h_pin;
+        if (mode != GGA_MODE_TOPO) {
+            s.iterations = c[GGA_CTL_STEPS];
+            if (c[GGA_CTL_ERR] & GGA_DEVERR_QFULL) {
+                rc = GGA_ERANGE;
+                gga_set_error("worklist overflow (capacity %u)", g->qcap);
+                goto fail;
+            }
+            if (c[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
+                rc = GGA_ETIMEDOUT;
+                gga_set_error("persistent kernel: grid barrier spin limit");
+                goto fail;
+            }
+            if (mode == GGA_MODE_PERSISTENT) {
+                uint32_t left = c[GGA_CTL_TAIL(s.iterations % 3u)];
+                if (left) {
+                    rc = GGA_ETIMEDOUT;
+                    gga_set_error("persistent kernel: %u steps without convergence", s.iterations);
+                    goto fail;
+                }
+            }
+        }
+        const uint64_t *c64 = (const uint64_t *)(c + GGA_CTL_CNT);
+        s.edges_inspected = c64[0];
+        s.chunks_expanded = c64[1];
+        s.relaxations = c64[2];
+        const uint64_t *r64 = (const uint64_t *)(c + GGA_CTL_RED);
+        s.reached = (uint32_t)r64[0];
+        s.m_reach = r64[1];
+    }
+    s.ktime_ms = ktime;
+    s.sktime_ms = 0.f;
+    g->epoch += s.iterations + 2u;
+    g->have_result = 1;
+    if (out) HIP_TRY(hipMemcpy(out, g->d_dist, 4ull * g->n, hipMemcpyDeviceToHost));   /* bfs_sssp.cu:489 */
+    if (st) *st = s;
+    return GGA_OK;
+fail:
+    hipStreamSynchronize(g->stream);
+    g->epoch += max_iters < 0x10000000u ? max_iters + 2u : 0x10000000u;
+    g->have_result = 0;
+    if (st) *st = s;
+    return rc;
+}
+
+int gga_bfs(gga_graph *g, uint32_t src, uint32_t *out, const gga_opts *o, gga_stats *st)
+{
+    return traverse(g, src, 1, out, o, st);
+}
+
+int gga_sssp(gga_graph *g, uint32_t src, uint32_t *out, const gga_opts *o, gga_stats *st)
+{
+    return traverse(g, src, 0, out, o, st);
+}
+
+int gga_graph_copy_dist(gga_graph *g, uint32_t *out)
+{
+    int rc = GGA_OK;
+    if (!g || !out) return GGA_EINVAL;
+    if (!g->have_result) return GGA_EINVAL;
+    HIP_TRY(hipSetDevice(g->device));
+    HIP_TRY(hipMemcpy(out, g->d_dist, 4ull * g->n, hipMemcpyDeviceToHost));
+    return GGA_OK;
+fail:
+    return rc;
+}
+
+const uint32_t *gga_graph_dist_device(gga_graph *g)
+{
+    return (g && g->have_result) ? g->d_dist : NULL;
+}
+
+int gga_graph_verify(gga_graph *g, uint32_t src, int bfs, uint32_t inf, gga_verify_result *r)
+{
+    int rc = GGA_OK;
+    uint8_t *tight = NULL;
+    if (!g || !r) return GGA_EINVAL;
+    if (!g->have_result || !g->split) return GGA_EINVAL;
+    if (src >= g->n) return GGA_EINVAL;
+    if (inf == 0) inf = GGA_INF;
+    HIP_TRY(hipSetDevice(g->device));
+    if ((rc = dmalloc(g, (void **)&tight, g->n))) return rc;
+    HIP_TRY(hipMemsetAsync(tight, 0, g->n, g->stream));
+    K_TRY(ggak_fill_u32(g->d_ctl + GGA_CTL_RED, 0u, 8, g->stream));
+    gga_dev_graph dg = dev_graph(g);
+    gga_dev_work dw = dev_work(g, inf);
+    K_TRY(ggak_verify(dg, dw, src, bfs, tight, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_RED, 32, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    {
+        const uint64_t *r64 = (const uint64_t *)g->h_pin;
+        memset(r, 0, sizeof *r);
+        r->nerr = r64[0];
+        uint64_t bad = r64[1];
+        K_TRY(ggak_fill_u32(g->d_ctl + GGA_CTL_RED, 0u, 8, g->stream));
+        K_TRY(ggak_reach(dg, dw, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_RED, 32, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        uint32_t d0 = 1;
+        HIP_TRY(hipMemcpy(&d0, g->d_dist + src, 4, hipMemcpyDeviceToHost));
+        r->src_ok = d0 == 0;
+        r->untight = bad - (d0 != 0 ? 1u : 0u);
+        r->reached = (uint32_t)((const uint64_t *)g->h_pin)[0];
+    }
+    dfree(g, (void **)&tight, g->n);
+    return GGA_OK;
+fail:
+    dfree(g, (void **)&tight, g->n);
+    return rc;
+}

@@ -1,0 +1,495 @@
+// relax.hip -- BFS/SSSP edge relaxation over split chunks, for gfx950.
+//
+// Replaces M_my_drelax (cudaKernels.h:285-344) and its host fixpoint loop
+// (bfs_sssp.cu:372-468), my_drelax_no_atomic + my_updateDist
+// (cudaKernels.h:129-220) and the dynamic-parallelism child grids
+// (my_drelax_dynamicP / dynamicP_relax_coal, cudaKernels.h:347-377,438-521).
+//
+// Work unit: a wave takes 64 chunk ids (one per lane), reads each chunk's
+// bounds and its owner's distance, and walks the concatenation of the 64 edge
+// ranges 64 slots per round (wave_owner).  A chunk holds at most T edges, so a
+// hub vertex is spread over many chunks and many waves; this is the node
+// split.  Chunks of one vertex are adjacent in the worklist and in memory, so
+// col[]/w[] loads of a round are contiguous runs.
+//
+// Relax (all modes): alt = dist[u] + w (skipped when it would wrap or reach
+// inf); a plain read of dist[v] guards the atomicMin (cudaKernels.h:324-325).
+// Distances only decrease, so a stale read is never lower than the truth and
+// the guard cannot lose an update.  The owner's distance is read when the
+// chunk is expanded, so an update made earlier in the same step is used.
+//
+// Worklist: a vertex whose distance dropped is queued once per step (step
+// stamp in mark[]; BFS needs none because a level-synchronous atomicMin
+// succeeds once per vertex).  Queued vertices are staged in wave-private LDS
+// and flushed as chunk ids with one atomicAdd per 64 vertices.
+//
+// Bytes per unit of work (the roofline model): per edge inspected
+// col 4 + w 4 (SSSP) + dist[v] 4; per chunk expanded cbeg 8 + cown 4 +
+// dist[u] 4 (+ 4 for the worklist id); per successful relaxation 4 (+ 4 push).
+#include "../gga_kernels.h"
+#include "wave.hpp"
+
+using namespace gga;
+
+namespace {
+
+constexpr uint32_t BLOCK = 256;
+constexpr uint32_t WAVES = BLOCK / GGA_WAVE;
+constexpr uint32_t PBUF = 256;  // queued vertices staged per wave before a flush
+
+struct Counters {
+    uint64_t edges = 0, chunks = 0, relax = 0;
+};
+
+// Append the wave's staged vertices to the chunk worklist: every staged vertex
+// contributes its chunk range [cstart[v], cstart[v + 1]).
+__device__ __forceinline__ void flush_pushes(const gga_dev_graph &g, const uint32_t *pbuf,
+                                             uint32_t pcount, uint32_t *qout, uint32_t *tail,
+                                             uint32_t qcap, uint32_t *ctl) {
+    const uint32_t l = lane_id();
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i0 = 0; i0 < pcount; i0 += GGA_WAVE) {
+        const uint32_t idx = i0 + l;
+        uint32_t cs = 0, nch = 0;
+        if (idx < pcount) {
+            const uint32_t v = pbuf[idx];
+            cs = g.cstart[v];
+            nch = g.cstart[v + 1] - cs;
+        }
+        const uint32_t incl = wave_incl_scan(nch);
+        const uint32_t tot = wave_bcast(incl, GGA_WAVE - 1);
+        uint32_t base = 0;
+        if (l == 0) base = atomic_add_u32(tail, tot);
+        base = wave_bcast(base, 0);
+        const uint32_t excl = incl - nch;
+        for (uint32_t b = 0; b < tot; b += GGA_WAVE) {
+            const uint32_t k = b + l;
+            const uint32_t kk = k < tot ? k : tot - 1;
+            const uint32_t j = wave_owner(incl, kk);
+            const uint32_t id = wave_bcast(cs, j) + (kk - wave_bcast(excl, j));
+            if (k < tot) {
+                // a vertex is queued at most once per step, so base + k < C <= qcap;
+                // the guard only turns a broken invariant into an error bit
+                if ((uint64_t)base + k < qcap) qout[base + k] = id;
+                else __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_QFULL, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Expand one batch of up to 64 chunks (this lane's chunk c when valid).
+// PUSH: queue improved vertices (worklist); else raise `improved` (topology).
+template <bool BFS, bool COUNT, bool PUSH>
+__device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_dev_work &w,
+                                             uint32_t c, bool valid, uint32_t it, uint32_t *pbuf,
+                                             uint32_t &pcount, uint32_t *qout, uint32_t *tail,
+                                             bool &improved, Counters &cnt) {
+    const uint32_t l = lane_id();
+    uint32_t *__restrict__ dist = w.dist;
+    const uint32_t inf = w.inf;
+    uint32_t eb = 0, n = 0, du = inf;
+    if (valid) {
+        eb = g.cbeg[c];
+        const uint32_t ee = g.cbeg[c + 1];
+        du = dist[g.cown[c]];
+        n = du < inf ? ee - eb : 0u;
+    }
+    const uint32_t incl = wave_incl_scan(n);
+    const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+    if (COUNT) {
+        cnt.edges += total;
+        cnt.chunks += (uint32_t)__popcll(__ballot(n > 0));
+    }
+    const uint32_t excl = incl - n;
+    for (uint32_t base = 0; base < total; base += GGA_WAVE) {
+        const uint32_t k = base + l;
+        const uint32_t kk = k < total ? k : total - 1;
+        const uint32_t j = wave_owner(incl, kk);
+        const uint32_t e = wave_bcast(eb, j) + (kk - wave_bcast(excl, j));
+        const uint32_t ds = wave_bcast(du, j);
+        bool imp = false;
+        uint32_t v = 0;
+        if (k < total) {
+            v = g.col[e];
+            const uint32_t wt = BFS ? 1u : g.w[e];
+            const uint32_t alt = ds + wt;
+            if (alt >= ds && alt < inf) {
+                if (alt < dist[v]) {
+                    const uint32_t old = atomic_min_u32(&dist[v], alt);
+                    imp = alt < old;
+                }
+            }
+        }
+        if (COUNT) cnt.relax += (uint32_t)__popcll(__ballot(imp));
+        if (PUSH) {
+            bool want = imp;
+            if (!BFS && want) want = (w.mark[v] != it) && (atomic_xchg_u32(&w.mark[v], it) != it);
+            const uint64_t mask = __ballot(want);
+            if (mask) {
+                const uint32_t nw = (uint32_t)__popcll(mask);
+                if (pcount + nw > PBUF) {
+                    flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
+                    pcount = 0;
+                }
+                if (want) {
+                    const uint32_t pos = pcount + __builtin_amdgcn_mbcnt_hi(
+                                                      (uint32_t)(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                    pbuf[pos] = v;
+                }
+                pcount += nw;
+                __builtin_amdgcn_wave_barrier();
+            }
+        } else {
+            improved |= imp;
+        }
+    }
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void publish_counters(uint32_t *ctl, const Counters &c) {
+    if (COUNT && lane_id() == 0) {
+        uint64_t *cnt = reinterpret_cast<uint64_t *>(ctl + GGA_CTL_CNT);
+        if (c.edges) atomic_add_u64(&cnt[0], c.edges);
+        if (c.chunks) atomic_add_u64(&cnt[1], c.chunks);
+        if (c.relax) atomic_add_u64(&cnt[2], c.relax);
+    }
+}
+
+// ---------------------------------------------------------------- init
+__global__ __launch_bounds__(BLOCK) void k_fill(uint32_t *p, uint32_t value, uint64_t count) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// dist = inf except dist[src] = 0 (bfs_sssp.cu:302-318, with the source as an
+// argument instead of the hard-coded parent 0); worklist = chunks of src;
+// ring slot 0 = {len, step epoch}; counters zeroed.
+__global__ __launch_bounds__(BLOCK) void k_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src,
+                                                      uint32_t epoch) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        w.dist[v] = (v == src) ? 0u : w.inf;
+    if (blockIdx.x == 0) {
+        const uint32_t cs = g.cstart[src], nch = g.cstart[src + 1] - cs;
+        for (uint32_t i = threadIdx.x; i < nch; i += blockDim.x) w.q0[i] = cs + i;
+        if (threadIdx.x < 3) {
+            w.ctl[GGA_CTL_TAIL(threadIdx.x)] = threadIdx.x == 0 ? nch : 0u;
+            w.ctl[GGA_CTL_BC(threadIdx.x)] = 0;
+            w.ctl[GGA_CTL_ITER(threadIdx.x)] = epoch;
+        }
+        if (threadIdx.x == 0) {
+            w.ctl[GGA_CTL_CHANGED] = 0;
+            w.ctl[GGA_CTL_STEPS] = 0;
+            w.ctl[GGA_CTL_ERR] = 0;
+            w.ctl[GGA_CTL_BAR] = 0;
+        }
+        if (threadIdx.x < 6) w.ctl[GGA_CTL_CNT + threadIdx.x] = 0;
+        if (threadIdx.x < 8) w.ctl[GGA_CTL_RED + threadIdx.x] = 0;
+    }
+}
+
+// ------------------------------------------------------ topology step
+// Every chunk, every step: the reference's M_my_drelax schedule with chunks in
+// place of split nodes (chunks of unreached owners contribute no edges).
+template <bool BFS, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_topo_step(gga_dev_graph g, gga_dev_work w) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t c = ((uint64_t)lb * WAVES + threadIdx.x / GGA_WAVE) * GGA_WAVE + lane_id();
+    const bool valid = c < g.C;
+    bool improved = false;
+    uint32_t pcount = 0;
+    Counters cnt;
+    expand_batch<BFS, COUNT, false>(g, w, valid ? (uint32_t)c : 0u, valid, 0u, nullptr, pcount,
+                                    nullptr, nullptr, improved, cnt);
+    if (__ballot(improved) && lane_id() == 0) w.ctl[GGA_CTL_CHANGED] = 1u;  // *changed = true
+    publish_counters<COUNT>(w.ctl, cnt);
+}
+
+// ------------------------------------------------------ worklist step
+// One step = expand the list of ring slot (step % 3), fill slot (step + 1) % 3.
+template <bool BFS, bool COUNT>
+__device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_dev_work &w,
+                                              uint32_t step, uint32_t nq, uint32_t it,
+                                              uint32_t *pbuf, Counters &cnt) {
+    const uint32_t sin = step % 3u, sout = (step + 1u) % 3u;
+    const uint32_t *qin = (step & 1u) ? w.q1 : w.q0;
+    uint32_t *qout = (step & 1u) ? w.q0 : w.q1;
+    uint32_t *tail = &w.ctl[GGA_CTL_TAIL(sout)];
+    uint32_t *bc = &w.ctl[GGA_CTL_BC(sin)];
+    const uint32_t nbatch = (nq + GGA_WAVE - 1) / GGA_WAVE;
+    const uint32_t l = lane_id();
+    uint32_t pcount = 0;
+    bool dummy = false;
+    for (;;) {
+        uint32_t b = 0;
+        if (l == 0) b = atomic_add_u32(bc, 1u);
+        b = wave_bcast(b, 0);
+        if (b >= nbatch) break;
+        const uint32_t idx = b * GGA_WAVE + l;
+        const bool valid = idx < nq;
+        const uint32_t c = valid ? qin[idx] : 0u;
+        expand_batch<BFS, COUNT, true>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt);
+    }
+    if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
+}
+
+// Per-step bookkeeping done by one thread: next slot's step number, clear the
+// slot consumed one step ago (it becomes the output slot of the next step).
+__device__ __forceinline__ void step_bookkeeping(uint32_t *ctl, uint32_t step, uint32_t it,
+                                                 uint32_t nq) {
+    const uint32_t sout = (step + 1u) % 3u, sclr = (step + 2u) % 3u;
+    ctl[GGA_CTL_ITER(sout)] = it + 1u;
+    ctl[GGA_CTL_TAIL(sclr)] = 0u;
+    ctl[GGA_CTL_BC(sclr)] = 0u;
+    if (nq) ctl[GGA_CTL_STEPS] += 1u;
+}
+
+template <bool BFS, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+    __shared__ uint32_t s_pbuf[WAVES][PBUF];
+    const uint32_t sin = step % 3u;
+    const uint32_t tail_in = w.ctl[GGA_CTL_TAIL(sin)];
+    const uint32_t nq = tail_in < w.qcap ? tail_in : w.qcap;  // > qcap only with QFULL set
+    const uint32_t it = w.ctl[GGA_CTL_ITER(sin)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
+    if (nq == 0) return;
+    Counters cnt;
+    worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+    publish_counters<COUNT>(w.ctl, cnt);
+}
+
+// ------------------------------------------------------ persistent kernel
+// Grid barrier for a cooperative launch.  Every wait is bounded by
+// `spin_ticks` of the 100 MHz wall clock; on expiry the error bit is set and
+// every workgroup leaves the step loop at its next check.
+__device__ __forceinline__ void grid_barrier(uint32_t *ctl, uint32_t nblocks, uint32_t &gen,
+                                             uint64_t spin_ticks) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t arrived =
+            __hip_atomic_fetch_add(&ctl[GGA_CTL_BAR], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (arrived == nblocks) {
+            __hip_atomic_store(&ctl[GGA_CTL_BAR], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl[GGA_CTL_BARGEN], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(&ctl[GGA_CTL_BARGEN], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+                if (wall_clock64() - t0 > spin_ticks) {
+                    __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_SPIN, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        gen += 1u;
+    }
+    __syncthreads();
+    __threadfence();
+}
+
+template <bool BFS, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_work w,
+                                                      uint32_t max_steps, uint64_t spin_ticks) {
+    __shared__ uint32_t s_pbuf[WAVES][PBUF];
+    __shared__ uint32_t s_state[2];
+    uint32_t gen = 0;
+    if (threadIdx.x == 0)
+        gen = __hip_atomic_load(&w.ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Counters cnt;
+    for (uint32_t step = 0; step < max_steps; step++) {
+        const uint32_t sin = step % 3u;
+        if (threadIdx.x == 0) {
+            s_state[0] = __hip_atomic_load(&w.ctl[GGA_CTL_TAIL(sin)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_state[1] = __hip_atomic_load(&w.ctl[GGA_CTL_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        const uint32_t nq = s_state[0] < w.qcap ? s_state[0] : w.qcap, err = s_state[1];
+        __syncthreads();
+        if (nq == 0 || err) break;
+        const uint32_t it = w.ctl[GGA_CTL_ITER(sin)];
+        if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
+        worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+        grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
+    }
+    publish_counters<COUNT>(w.ctl, cnt);
+}
+
+// ------------------------------------------------------ reductions
+__global__ __launch_bounds__(BLOCK) void k_reach(gga_dev_graph g, gga_dev_work w) {
+    uint64_t reached = 0, mr = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        if (w.dist[v] < w.inf) {
+            reached += 1;
+            mr += g.row[v + 1] - g.row[v];
+        }
+    }
+    reached = wave_sum64(reached);
+    mr = wave_sum64(mr);
+    if (lane_id() == 0) {
+        uint64_t *red = reinterpret_cast<uint64_t *>(w.ctl + GGA_CTL_RED);
+        if (reached) atomic_add_u64(&red[0], reached);
+        if (mr) atomic_add_u64(&red[1], mr);
+    }
+}
+
+// verifysolution (myutils.h:1-18) plus tightness, over chunks.
+template <bool BFS>
+__global__ __launch_bounds__(BLOCK) void k_verify_edges(gga_dev_graph g, gga_dev_work w,
+                                                        uint8_t *tight) {
+    const uint32_t l = lane_id();
+    const uint64_t c = ((uint64_t)blockIdx.x * WAVES + threadIdx.x / GGA_WAVE) * GGA_WAVE + l;
+    uint32_t eb = 0, n = 0, du = w.inf;
+    if (c < g.C) {
+        eb = g.cbeg[c];
+        du = w.dist[g.cown[c]];
+        n = du < w.inf ? g.cbeg[c + 1] - eb : 0u;
+    }
+    const uint32_t incl = wave_incl_scan(n);
+    const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+    const uint32_t excl = incl - n;
+    uint64_t nerr = 0;
+    for (uint32_t base = 0; base < total; base += GGA_WAVE) {
+        const uint32_t k = base + l;
+        const uint32_t kk = k < total ? k : total - 1;
+        const uint32_t j = wave_owner(incl, kk);
+        const uint32_t e = wave_bcast(eb, j) + (kk - wave_bcast(excl, j));
+        const uint32_t ds = wave_bcast(du, j);
+        if (k < total) {
+            const uint32_t v = g.col[e];
+            const uint32_t wt = BFS ? 1u : g.w[e];
+            const uint32_t alt = ds + wt;
+            if (alt >= ds) {
+                const uint32_t dv = w.dist[v];
+                if (wt > 0 && alt < w.inf && alt < dv) nerr++;
+                if (alt == dv) tight[v] = 1;
+            }
+        }
+    }
+    nerr = wave_sum64(nerr);
+    if (l == 0 && nerr) atomic_add_u64(reinterpret_cast<uint64_t *>(w.ctl + GGA_CTL_RED), nerr);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_verify_vertices(gga_dev_graph g, gga_dev_work w,
+                                                           uint32_t src, const uint8_t *tight) {
+    uint64_t bad = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = w.dist[v];
+        if (v == src) bad += d != 0u;
+        else if (d < w.inf && !tight[v]) bad++;
+    }
+    bad = wave_sum64(bad);
+    if (lane_id() == 0 && bad) atomic_add_u64(reinterpret_cast<uint64_t *>(w.ctl + GGA_CTL_RED) + 1, bad);
+}
+
+inline int launch_status() { return (int)hipGetLastError(); }
+
+inline uint32_t grid_for(uint64_t items, uint32_t cap) {
+    uint64_t gsz = (items + BLOCK - 1) / BLOCK;
+    if (gsz == 0) gsz = 1;
+    return (uint32_t)(gsz < cap ? gsz : cap);
+}
+
+}  // namespace
+
+extern "C" int ggak_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t s) {
+    if (count == 0) return 0;
+    k_fill<<<grid_for(count, 8192), BLOCK, 0, s>>>(p, value, count);
+    return launch_status();
+}
+
+extern "C" int ggak_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src, uint32_t epoch,
+                               hipStream_t s) {
+    k_query_init<<<grid_for(g.n, 8192), BLOCK, 0, s>>>(g, w, src, epoch);
+    return launch_status();
+}
+
+extern "C" int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int count, hipStream_t s) {
+    const uint64_t waves = ((uint64_t)g.C + GGA_WAVE - 1) / GGA_WAVE;
+    const uint32_t grid = (uint32_t)((waves + WAVES - 1) / WAVES);
+    if (grid == 0) return 0;
+    if (bfs) {
+        if (count) k_topo_step<true, true><<<grid, BLOCK, 0, s>>>(g, w);
+        else k_topo_step<true, false><<<grid, BLOCK, 0, s>>>(g, w);
+    } else {
+        if (count) k_topo_step<false, true><<<grid, BLOCK, 0, s>>>(g, w);
+        else k_topo_step<false, false><<<grid, BLOCK, 0, s>>>(g, w);
+    }
+    return launch_status();
+}
+
+extern "C" int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, int bfs, int count,
+                            uint32_t grid, hipStream_t s) {
+    if (grid == 0) return (int)hipErrorInvalidValue;
+    if (bfs) {
+        if (count) k_wl_step<true, true><<<grid, BLOCK, 0, s>>>(g, w, step);
+        else k_wl_step<true, false><<<grid, BLOCK, 0, s>>>(g, w, step);
+    } else {
+        if (count) k_wl_step<false, true><<<grid, BLOCK, 0, s>>>(g, w, step);
+        else k_wl_step<false, false><<<grid, BLOCK, 0, s>>>(g, w, step);
+    }
+    return launch_status();
+}
+
+template <bool BFS, bool COUNT>
+static const void *persistent_fn() {
+    return reinterpret_cast<const void *>(&k_persistent<BFS, COUNT>);
+}
+
+static const void *pick_persistent(int bfs, int count) {
+    if (bfs) return count ? persistent_fn<true, true>() : persistent_fn<true, false>();
+    return count ? persistent_fn<false, true>() : persistent_fn<false, false>();
+}
+
+extern "C" uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    const void *fn;
+    if (persistent) fn = pick_persistent(bfs, count);
+    else if (bfs) fn = count ? reinterpret_cast<const void *>(&k_wl_step<true, true>)
+                             : reinterpret_cast<const void *>(&k_wl_step<true, false>);
+    else fn = count ? reinterpret_cast<const void *>(&k_wl_step<false, true>)
+                    : reinterpret_cast<const void *>(&k_wl_step<false, false>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, 0) != hipSuccess) return 0;
+    if (per <= 0) return 0;
+    return (uint32_t)cus * (uint32_t)per;
+}
+
+extern "C" int ggak_persistent(gga_dev_graph g, gga_dev_work w, int bfs, int count, uint32_t grid,
+                               uint32_t max_steps, uint32_t spin_limit_ms, hipStream_t s) {
+    if (grid == 0) return (int)hipErrorInvalidValue;
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;  // 100 MHz
+    uint64_t spin_ticks = (uint64_t)spin_limit_ms * (uint64_t)khz;
+    void *args[] = {&g, &w, &max_steps, &spin_ticks};
+    hipError_t e = hipLaunchCooperativeKernel(pick_persistent(bfs, count), dim3(grid), dim3(BLOCK), args,
+                                              0, s);
+    if (e != hipSuccess) return (int)e;
+    return launch_status();
+}
+
+extern "C" int ggak_reach(gga_dev_graph g, gga_dev_work w, hipStream_t s) {
+    k_reach<<<grid_for(g.n, 4096), BLOCK, 0, s>>>(g, w);
+    return launch_status();
+}
+
+extern "C" int ggak_verify(gga_dev_graph g, gga_dev_work w, uint32_t src, int bfs, uint8_t *d_tight,
+                           hipStream_t s) {
+    const uint64_t waves = ((uint64_t)g.C + GGA_WAVE - 1) / GGA_WAVE;
+    const uint32_t grid = (uint32_t)((waves + WAVES - 1) / WAVES);
+    if (grid) {
+        if (bfs) k_verify_edges<true><<<grid, BLOCK, 0, s>>>(g, w, d_tight);
+        else k_verify_edges<false><<<grid, BLOCK, 0, s>>>(g, w, d_tight);
+    }
+    k_verify_vertices<<<grid_for(g.n, 4096), BLOCK, 0, s>>>(g, w, src, d_tight);
+    return launch_status();
+}

@@ -1,0 +1,106 @@
+// wave.hpp -- wave64 primitives for gfx950 (CDNA4).
+//
+// The flattened-range primitive (wave_owner) is what every kernel here uses
+// to turn 64 ragged ranges held one per lane into 64 balanced slots per round:
+// lane l handles slot k = round * 64 + l, which belongs to the first lane j
+// whose inclusive prefix incl_j exceeds k.  Edge loads of consecutive slots
+// are consecutive addresses inside a chunk, so col/w reads coalesce.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GGA_WAVE 64u
+
+namespace gga {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Inclusive prefix sum over the 64 lanes (all lanes must be active).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (uint32_t o = 1; o < GGA_WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, GGA_WAVE);
+        if (l >= o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_bcast(uint32_t x, uint32_t src_lane) {
+    return __shfl(x, (int)src_lane, GGA_WAVE);
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (uint32_t o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, (int)o, GGA_WAVE);
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t x) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+#pragma unroll
+    for (uint32_t o = 32; o >= 1; o >>= 1) {
+        uint32_t l2 = __shfl_xor(lo, (int)o, GGA_WAVE);
+        uint32_t h2 = __shfl_xor(hi, (int)o, GGA_WAVE);
+        uint64_t s = ((uint64_t)hi << 32 | lo) + ((uint64_t)h2 << 32 | l2);
+        lo = (uint32_t)s;
+        hi = (uint32_t)(s >> 32);
+    }
+    return (uint64_t)hi << 32 | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_max64(uint64_t x) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+#pragma unroll
+    for (uint32_t o = 32; o >= 1; o >>= 1) {
+        uint32_t l2 = __shfl_xor(lo, (int)o, GGA_WAVE);
+        uint32_t h2 = __shfl_xor(hi, (int)o, GGA_WAVE);
+        uint64_t a = (uint64_t)hi << 32 | lo, b = (uint64_t)h2 << 32 | l2;
+        uint64_t m = a > b ? a : b;
+        lo = (uint32_t)m;
+        hi = (uint32_t)(m >> 32);
+    }
+    return (uint64_t)hi << 32 | lo;
+}
+
+// Number of lanes j with incl_j <= k, i.e. the owner lane of flattened slot k
+// (incl must be non-decreasing over lanes; for k < incl_63 the result is <= 63).
+__device__ __forceinline__ uint32_t wave_owner(uint32_t incl, uint32_t k) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1) {
+        uint32_t v = __shfl(incl, (int)(pos + step - 1), GGA_WAVE);
+        if (v <= k) pos += step;
+    }
+    return pos;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this maps the
+// hardware block id to a logical id so that each XCD (and its L2) works on
+// one contiguous range.  Bijective on [0, nblk).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblk) {
+    const uint32_t x = b & 7u, idx = b >> 3;
+    const uint32_t per = nblk >> 3, rem = nblk & 7u;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+
+__device__ __forceinline__ uint32_t atomic_min_u32(uint32_t *p, uint32_t v) {
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_add_u32(uint32_t *p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_xchg_u32(uint32_t *p, uint32_t v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_add_u64(uint64_t *p, uint64_t v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_max_u64(uint64_t *p, uint64_t v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t load_relaxed(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace gga

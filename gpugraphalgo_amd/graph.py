@@ -1,0 +1,250 @@
+"""Host-side mirror of the reference's graph program over the C-ABI.
+
+The reference drives one graph through (bfs_sssp.cu:103-539):
+    readFromGR -> NodeSplit(L, bucket) -> cudaCopy -> fixpoint loop -> Ary2File / writeResult
+Here the same steps are
+    read_gr / gen_rmat / gen_grid  ->  DeviceGraph(csr)  ->  .split(level=L, buckets=b)
+    ->  .bfs(src) / .sssp(src)  ->  write_dist / write_result_row
+all executed by libgga.so (C host + HIP kernels); this module only marshals
+numpy arrays through ctypes.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import INF, REF_INF, MODES, check, load
+
+_u32 = np.uint32
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class CsrGraph:
+    """Host CSR: row_ptr[n + 1], dst[m], w[m] (uint32, 0-based)."""
+
+    row_ptr: np.ndarray
+    dst: np.ndarray
+    w: np.ndarray
+    has_weights: bool = True
+
+    @property
+    def n(self) -> int:
+        return int(self.row_ptr.size - 1)
+
+    @property
+    def m(self) -> int:
+        return int(self.dst.size)
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.row_ptr.astype(np.int64)).astype(_u32)
+
+    def maxdeg(self) -> tuple[int, int]:
+        """(max out-degree, first vertex with it) -- getMaxdeg/getMaxdegIndex."""
+        d = self.degrees()
+        if d.size == 0:
+            return 0, 0
+        i = int(np.argmax(d))
+        return int(d[i]), i
+
+    def _as_c(self) -> _lib.Csr:
+        c = _lib.Csr()
+        c.n, c.m = self.n, self.m
+        c.row_ptr = self.row_ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        c.dst = self.dst.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        c.w = self.w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        c.has_weights = 1 if self.has_weights else 0
+        return c
+
+
+def _take_csr(c: _lib.Csr) -> CsrGraph:
+    """Copies a library-owned gga_csr into numpy and frees it."""
+    L = load()
+    try:
+        n, m = c.n, c.m
+        rp = np.ctypeslib.as_array(c.row_ptr, shape=(n + 1,)).copy()
+        dst = np.ctypeslib.as_array(c.dst, shape=(max(m, 1),))[:m].copy()
+        w = np.ctypeslib.as_array(c.w, shape=(max(m, 1),))[:m].copy()
+        return CsrGraph(rp, dst, w, bool(c.has_weights))
+    finally:
+        L.gga_csr_free(ctypes.byref(c))
+
+
+def csr_from_arrays(row_ptr, dst, w=None) -> CsrGraph:
+    rp = np.ascontiguousarray(row_ptr, dtype=_u32)
+    d = np.ascontiguousarray(dst, dtype=_u32)
+    if w is None:
+        return CsrGraph(rp, d, np.ones(d.size, dtype=_u32), False)
+    return CsrGraph(rp, d, np.ascontiguousarray(w, dtype=_u32), True)
+
+
+def read_gr(path: str) -> CsrGraph:
+    """Galois .gr v1 reader (Graph::readFromGR, graph.h:599-682)."""
+    c = _lib.Csr()
+    check(load().gga_csr_read_gr(os.fsencode(path), ctypes.byref(c)), f"read_gr({path})")
+    return _take_csr(c)
+
+
+def write_gr(path: str, g: CsrGraph) -> None:
+    c = g._as_c()
+    check(load().gga_csr_write_gr(os.fsencode(path), ctypes.byref(c)), f"write_gr({path})")
+
+
+def gen_rmat(scale: int, edge_factor: int, a=0.57, b=0.19, c=0.19, seed=1, wlo=1, whi=1,
+             dedup=True) -> CsrGraph:
+    out = _lib.Csr()
+    check(load().gga_gen_rmat(scale, edge_factor, a, b, c, seed, wlo, whi, 1 if dedup else 0,
+                              ctypes.byref(out)), "gen_rmat")
+    g = _take_csr(out)
+    g.has_weights = not (wlo == 1 and whi == 1)
+    return g
+
+
+def gen_grid(rows: int, cols: int, shortcut_frac=0.005, seed=11, wlo=1, whi=1000) -> CsrGraph:
+    out = _lib.Csr()
+    check(load().gga_gen_grid(rows, cols, shortcut_frac, seed, wlo, whi, ctypes.byref(out)),
+          "gen_grid")
+    return _take_csr(out)
+
+
+def verify(g: CsrGraph, src: int, dist: np.ndarray, bfs: bool, inf: int = INF) -> dict:
+    """Host fixpoint + tightness check (verifysolution, myutils.h:1-18, plus tightness)."""
+    r = _lib.VerifyResult()
+    dist = np.ascontiguousarray(dist, dtype=_u32)
+    check(load().gga_verify(g.n, _ptr(g.row_ptr), _ptr(g.dst), None if bfs else _ptr(g.w), src,
+                            _ptr(dist), inf, ctypes.byref(r)), "verify")
+    return {"nerr": r.nerr, "untight": r.untight, "src_ok": bool(r.src_ok), "reached": r.reached}
+
+
+def write_dist(path: str, dist: np.ndarray) -> None:
+    """Ary2File (myutils.h:51-62): one "%u\\n" per original vertex."""
+    dist = np.ascontiguousarray(dist, dtype=_u32)
+    check(load().gga_write_dist(os.fsencode(path), _ptr(dist), dist.size), "write_dist")
+
+
+def write_result_row(path: str, **fields) -> None:
+    """writeResult (myutils.h:86-113)."""
+    r = _lib.ResultRow()
+    for k, v in fields.items():
+        if isinstance(v, str):
+            v = v.encode()
+        setattr(r, k, v)
+    check(load().gga_write_result_row(os.fsencode(path), ctypes.byref(r)), "write_result_row")
+
+
+def device_count() -> int:
+    return int(load().gga_device_count())
+
+
+def _stats_dict(s: _lib.Stats) -> dict:
+    return {f: getattr(s, f) for f, _ in _lib.Stats._fields_}
+
+
+def _split_dict(s: _lib.SplitStats) -> dict:
+    return {f: getattr(s, f) for f, _ in _lib.SplitStats._fields_}
+
+
+class DeviceGraph:
+    """A CSR resident in HBM on one device (gga_graph)."""
+
+    def __init__(self, g: CsrGraph, device: int = 0):
+        L = load()
+        self._h = ctypes.c_void_p()
+        self.n, self.m = g.n, g.m
+        w = _ptr(g.w) if g.has_weights else None
+        check(L.gga_graph_from_csr32(g.n, g.m, _ptr(g.row_ptr), _ptr(g.dst), w, device,
+                                     ctypes.byref(self._h)), "gga_graph_from_csr32")
+        self.split_stats: dict | None = None
+
+    @classmethod
+    def load_gr(cls, path: str, device: int = 0) -> "DeviceGraph":
+        self = cls.__new__(cls)
+        L = load()
+        self._h = ctypes.c_void_p()
+        check(L.gga_graph_load_gr(os.fsencode(path), device, ctypes.byref(self._h)),
+              f"gga_graph_load_gr({path})")
+        self.n = int(L.gga_graph_n(self._h))
+        self.m = int(L.gga_graph_m(self._h))
+        self.split_stats = None
+        return self
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            load().gga_graph_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_bytes(self) -> int:
+        return int(load().gga_graph_device_bytes(self._h))
+
+    def split(self, mda: int = 0, level: int = 0, buckets: int = 0) -> dict:
+        """gga_graph_split: mda > 0 explicit T; level > 0 T = ceil(maxdeg / level);
+        level < 0 histogram heuristic; all zero = no split."""
+        st = _lib.SplitStats()
+        check(load().gga_graph_split(self._h, mda, level, buckets, ctypes.byref(st)), "split")
+        self.split_stats = _split_dict(st)
+        return self.split_stats
+
+    def chunks(self) -> tuple[np.ndarray, np.ndarray]:
+        C = self.split_stats["ns_nnode"]
+        cb = np.empty(C + 1, dtype=_u32)
+        own = np.empty(C, dtype=_u32)
+        check(load().gga_graph_chunks(self._h, _ptr(cb), _ptr(own)), "chunks")
+        return cb, own
+
+    def _run(self, fn, src: int, mode: str, inf: int, count: bool, keep_on_device: bool,
+             **kw) -> tuple[np.ndarray | None, dict]:
+        o = _lib.Opts()
+        o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+        o.inf = inf
+        o.count = 1 if count else 0
+        for k, v in kw.items():
+            setattr(o, k, v)
+        st = _lib.Stats()
+        out = None if keep_on_device else np.empty(self.n, dtype=_u32)
+        check(fn(self._h, src, _ptr(out), ctypes.byref(o), ctypes.byref(st)), fn.__name__)
+        return out, _stats_dict(st)
+
+    def bfs(self, src: int, mode: str = "auto", inf: int = INF, count: bool = False,
+            keep_on_device: bool = False, **kw):
+        return self._run(load().gga_bfs, src, mode, inf, count, keep_on_device, **kw)
+
+    def sssp(self, src: int, mode: str = "auto", inf: int = INF, count: bool = False,
+             keep_on_device: bool = False, **kw):
+        return self._run(load().gga_sssp, src, mode, inf, count, keep_on_device, **kw)
+
+    def copy_dist(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=_u32)
+        check(load().gga_graph_copy_dist(self._h, _ptr(out)), "copy_dist")
+        return out
+
+    def verify(self, src: int, bfs: bool, inf: int = INF) -> dict:
+        r = _lib.VerifyResult()
+        check(load().gga_graph_verify(self._h, src, 1 if bfs else 0, inf, ctypes.byref(r)),
+              "graph_verify")
+        return {"nerr": r.nerr, "untight": r.untight, "src_ok": bool(r.src_ok),
+                "reached": r.reached}
+
+
+__all__ = ["CsrGraph", "DeviceGraph", "csr_from_arrays", "read_gr", "write_gr", "gen_rmat",
+           "gen_grid", "verify", "write_dist", "write_result_row", "device_count", "INF",
+           "REF_INF"]

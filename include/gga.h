@@ -1,0 +1,230 @@
+/*
+ * gga.h -- C-ABI of the MI355X-native BFS/SSSP-with-node-splitting library.
+ *
+ * The reference (vivekkumark/GPUGraphAlgo, Programs/CUDA_src/Kepler_BFS_SSSP)
+ * has no library API: its boundary is the executable
+ *     na_{bfs,sssp}.exe <g.gr> [L] [bucket | outfile]     (bfs_sssp.cu:149-185)
+ * built from graph.h (CSR model, .gr reader, NodeSplit, serial CPU BFS/SSSP),
+ * cudaKernels.h (relax kernels) and myutils.h (verify + output writers).
+ * Every entry point below names the reference function or code region it
+ * replaces.  All arithmetic is unsigned 32-bit integer.  Plain pointers and
+ * sizes only; no C++ or torch types cross this boundary.
+ *
+ * Conventions
+ *   - Vertex ids and edge offsets are uint32 (the reference's `unsigned`,
+ *     graph.h:54-55).  Edge arrays are 0-based here (the reference keeps a
+ *     null slot 0 and 1-based psrc, graph.h:251-254).
+ *   - Distances: uint32; unreached vertices hold `inf` (GGA_INF by default,
+ *     GGA_REF_INF = the reference's MYINFINITY 1e9, bfs_sssp.cu:53).
+ *   - BFS is SSSP with every weight equal to 1 (cudaKernels.h:320, graph.h:163).
+ *   - Every function returns GGA_OK (0) or a negative GGA_E* code; nothing in
+ *     the library calls exit() (the reference's CudaTest does, graph.h:523-533).
+ *   - A gga_graph owns one HIP stream on its device.  Calls on one handle must
+ *     be serialised by the caller; distinct handles may run concurrently.
+ */
+#ifndef GGA_H
+#define GGA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+#define GGA_OK          0
+#define GGA_EINVAL     (-1)  /* bad argument: NULL pointer, source >= n, level == 0 ... */
+#define GGA_EIO        (-2)  /* open / stat / mmap / read / write failure */
+#define GGA_EFORMAT    (-3)  /* malformed .gr: version != 1, truncated, dst >= n, edge size not 0 or 4 */
+#define GGA_ERANGE     (-4)  /* counts beyond uint32 ids / offsets, or beyond a kernel limit */
+#define GGA_ENOMEM     (-5)  /* host or device allocation failed */
+#define GGA_EHIP       (-6)  /* HIP runtime error (message in gga_last_error()) */
+#define GGA_ENOSPLIT   (-7)  /* traversal requested before gga_graph_split */
+#define GGA_ETIMEDOUT  (-8)  /* iteration cap or persistent-kernel barrier spin limit reached */
+#define GGA_EBUDGET    (-9)  /* serving: request does not fit the device-memory budget */
+#define GGA_ENODEV    (-10)  /* no HIP device */
+
+#define GGA_INF       0xFFFFFFFFu
+#define GGA_REF_INF   1000000000u   /* MYINFINITY, bfs_sssp.cu:53 / graph.h:96 */
+
+/* Returns a static string for an error code. */
+const char *gga_strerror(int rc);
+/* Last HIP / IO error detail recorded by this thread ("" if none). */
+const char *gga_last_error(void);
+/* Library version string. */
+const char *gga_version(void);
+/* Number of visible HIP devices (0 on a host without GPUs; never fails). */
+int gga_device_count(void);
+
+/* ------------------------------------------------------- host CSR graphs */
+typedef struct gga_csr {
+    uint32_t n;           /* vertices */
+    uint32_t m;           /* directed edges */
+    uint32_t *row_ptr;    /* n + 1 offsets, row_ptr[0] = 0, row_ptr[n] = m */
+    uint32_t *dst;        /* m destinations */
+    uint32_t *w;          /* m weights (all 1 when the source carried none) */
+    uint32_t has_weights; /* 1 if w came from the input, 0 if synthesised as 1 */
+} gga_csr;
+
+/* Reads a Galois .gr v1 file (replaces Graph::readFromGR, graph.h:599-682):
+ * u64 version(=1), u64 sizeEdgeTy, u64 n, u64 m, u64 outIdx[n] (cumulative
+ * ends), u32 outs[m], u32 pad if m is odd, u32 edgeData[m].  sizeEdgeTy must be
+ * 0 (no weight section: weights are 1) or 4.  dst >= n is GGA_EFORMAT (the
+ * reference only prints a warning, graph.h:669-670). */
+int gga_csr_read_gr(const char *path, gga_csr *out);
+/* Writes the same layout, always with a 4-byte weight section. */
+int gga_csr_write_gr(const char *path, const gga_csr *g);
+/* Builds a host CSR from .gr-style cumulative row ends (outIdx, graph.h:637,660-664). */
+int gga_csr_from_row_end(uint32_t n, uint32_t m, const uint64_t *row_end,
+                         const uint32_t *dst, const uint32_t *w, gga_csr *out);
+void gga_csr_free(gga_csr *g);
+/* Maximum out-degree and its first vertex (graph.h:338-358, first index on ties). */
+uint32_t gga_csr_maxdeg(const gga_csr *g, uint32_t *first_index);
+
+/* Seeded R-MAT generator (the reference used GTgraph, which it does not ship).
+ * 2^scale vertices, edge_factor * 2^scale generated arcs, quadrant
+ * probabilities a, b, c, d = 1 - a - b - c; weights uniform in [wlo, whi].
+ * dedup != 0 removes duplicate arcs and self-loops. */
+int gga_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                 uint64_t seed, uint32_t wlo, uint32_t whi, int dedup, gga_csr *out);
+/* rows x cols 4-neighbour grid (both arc directions), plus
+ * round(shortcut_frac * undirected grid edges) random shortcut edges added in
+ * both directions, no duplicates; one weight per undirected edge, uniform in
+ * [wlo, whi]. */
+int gga_gen_grid(uint32_t rows, uint32_t cols, double shortcut_frac, uint64_t seed,
+                 uint32_t wlo, uint32_t whi, gga_csr *out);
+
+/* ----------------------------------------------- device-resident graphs */
+typedef struct gga_graph gga_graph;
+
+/* Uploads a CSR given as .gr-style cumulative row ends (the survey-level
+ * boundary: replaces Graph::cudaCopy + allocOnDevice, graph.h:534-553,683-699).
+ * w == NULL means all weights 1.  The graph stays resident until gga_graph_free. */
+int gga_graph_from_csr(uint32_t n, uint32_t m, const uint64_t *row_end,
+                       const uint32_t *dst, const uint32_t *w, int device,
+                       gga_graph **out);
+/* Same, from a 0-based int32 CSR row_ptr[n + 1]. */
+int gga_graph_from_csr32(uint32_t n, uint32_t m, const uint32_t *row_ptr,
+                         const uint32_t *dst, const uint32_t *w, int device,
+                         gga_graph **out);
+/* readFromGR + cudaCopy in one call (bfs_sssp.cu:188,323). */
+int gga_graph_load_gr(const char *path, int device, gga_graph **out);
+int gga_graph_free(gga_graph *g);
+uint32_t gga_graph_n(const gga_graph *g);
+uint32_t gga_graph_m(const gga_graph *g);
+/* Device bytes held by the graph (CSR + split + per-query workspace). */
+uint64_t gga_graph_device_bytes(const gga_graph *g);
+
+typedef struct gga_split_stats {
+    uint32_t nnode;      /* myresult.nnode */
+    uint32_t nedge;      /* myresult.nedge */
+    uint32_t ns_nnode;   /* chunks = nodes after NodeSplit (graph.h:386-414) */
+    uint32_t ns_nedge;   /* = nedge (splitting never adds edges) */
+    uint32_t extranode;  /* ns_nnode - nnode */
+    uint32_t maxdeg;     /* getMaxdeg, graph.h:338-347 */
+    int32_t  level;      /* NodeSplitLevel L; -1 = histogram heuristic; 0 = explicit mda */
+    uint32_t bucket;     /* histogram buckets when level < 0, else 0 (bfs_sssp.cu:177-180) */
+    uint32_t mda;        /* MAX_EDGES_ALLOWED (the split threshold T) */
+    uint32_t max_chunk;  /* largest chunk in edges (<= mda) */
+    float    split_ms;   /* device time of the split pass */
+} gga_split_stats;
+
+/* Node splitting as a GPU prefix-sum pass (replaces Graph::NodeSplit,
+ * graph.h:415-522, getSizeafterLoadBalance 386-414 and
+ * getMaxAllowedOutDegree 359-385).  Threshold selection, in this order:
+ *   mda > 0            : T = mda
+ *   level > 0          : T = ceil(maxdeg / level)            (graph.h:390)
+ *   level < 0          : T = histogram heuristic, `buckets` buckets (graph.h:359-385)
+ * Every vertex of out-degree d > 0 becomes ceil(d / T) contiguous edge chunks
+ * (first ones ceil(d / nchunks) edges, the last the remainder: INT_SPLIT,
+ * graph.h:461-476); a vertex of degree 0 is one empty chunk.  Chunk ids equal
+ * the reference's renumbered node ids.  Re-splitting replaces the old split. */
+int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets,
+                    gga_split_stats *st);
+/* Copies the chunk layout to the host: chunk_begin[ns_nnode + 1] (0-based edge
+ * offsets; the reference's psrc - 1) and owner[ns_nnode] (the reference's
+ * parentNode).  Either pointer may be NULL. */
+int gga_graph_chunks(gga_graph *g, uint32_t *chunk_begin, uint32_t *owner);
+
+/* ------------------------------------------------------------ traversal */
+#define GGA_MODE_TOPO        0  /* every chunk every iteration; host fixpoint loop (M_my_drelax + bfs_sssp.cu:372-468) */
+#define GGA_MODE_WORKLIST    1  /* chunk worklist, wave64 ballot push; host-launched steps, optional hipGraph */
+#define GGA_MODE_PERSISTENT  2  /* chunk worklist in one cooperative launch with a grid barrier per step */
+#define GGA_MODE_AUTO        3  /* library choice from the degree distribution */
+
+typedef struct gga_opts {
+    int32_t  mode;        /* GGA_MODE_* (default GGA_MODE_AUTO) */
+    uint32_t inf;         /* unreached value: GGA_INF (0 -> default) or GGA_REF_INF */
+    uint32_t max_iters;   /* 0 -> n + 1 (the serial bound, graph.h:122) */
+    uint32_t block;       /* workgroup size, 0 -> default (256) */
+    uint32_t sync_every;  /* WORKLIST: steps launched per host convergence check (0 -> 8) */
+    uint32_t use_graph;   /* WORKLIST: replay the sync_every steps as one captured hipGraph */
+    uint32_t count;       /* 1 -> collect edges_inspected / chunks_expanded / relaxations */
+    uint32_t grid;        /* PERSISTENT / WORKLIST grid in workgroups, 0 -> occupancy bound */
+} gga_opts;
+/* Fills defaults (all zero = defaults). */
+void gga_opts_init(gga_opts *o);
+
+typedef struct gga_stats {
+    uint32_t iterations;      /* relax steps run (reference `iterations`) */
+    uint32_t reached;         /* vertices with dist < inf */
+    float    time_ms;         /* device time, dist init -> convergence (reference `time`, bfs_sssp.cu:347-469) */
+    float    ktime_ms;        /* relax-kernel device time (reference `ktime`) */
+    float    sktime_ms;       /* child-sync time: always 0 (no child copies; reference `sktime`) */
+    int32_t  mode;            /* mode actually used */
+    uint64_t edges_inspected; /* I_e  (opts.count) */
+    uint64_t chunks_expanded; /* I_v  (opts.count) */
+    uint64_t relaxations;     /* R: successful atomicMin (opts.count) */
+    uint64_t m_reach;         /* sum of out-degrees of reached vertices */
+    uint64_t launches;        /* relax-kernel launches */
+} gga_stats;
+
+/* BFS levels from src (replaces the na_bfs path: dist init bfs_sssp.cu:302-321,
+ * fixpoint loop 372-468, M_my_drelax cudaKernels.h:285-344).  out: n uint32 on
+ * the host, or NULL to keep the result in the graph's device buffer. */
+int gga_bfs(gga_graph *g, uint32_t src, uint32_t *out, const gga_opts *o, gga_stats *st);
+/* SSSP distances from src with the stored uint32 weights (the na_sssp path). */
+int gga_sssp(gga_graph *g, uint32_t src, uint32_t *out, const gga_opts *o, gga_stats *st);
+/* Copies the last result (n uint32) to the host (bfs_sssp.cu:489). */
+int gga_graph_copy_dist(gga_graph *g, uint32_t *out);
+/* Device pointer of the last result (n uint32), valid until the next query. */
+const uint32_t *gga_graph_dist_device(gga_graph *g);
+
+/* --------------------------------------------------- checks and outputs */
+typedef struct gga_verify_result {
+    uint64_t nerr;        /* edges with w > 0 and dist[u] + w < dist[v] (verifysolution, myutils.h:1-18) */
+    uint64_t untight;     /* reached v != src with no u: dist[u] + w == dist[v] */
+    uint32_t src_ok;      /* dist[src] == 0 */
+    uint32_t reached;     /* vertices with dist < inf */
+} gga_verify_result;
+/* Host-side fixpoint + tightness check of dist against the CSR.  nerr == 0,
+ * untight == 0 and src_ok == 1 together fix the shortest-path distances when
+ * every weight is >= 1.  w == NULL means BFS (weights 1). */
+int gga_verify(uint32_t n, const uint32_t *row_ptr, const uint32_t *dst, const uint32_t *w,
+               uint32_t src, const uint32_t *dist, uint32_t inf, gga_verify_result *r);
+/* Same check on the device against the graph's last result (for sizes where
+ * a host copy is wasteful). */
+int gga_graph_verify(gga_graph *g, uint32_t src, int bfs, uint32_t inf, gga_verify_result *r);
+
+/* Distance file: one "%u\n" per original vertex (Ary2File, myutils.h:51-62). */
+int gga_write_dist(const char *path, const uint32_t *dist, uint32_t n);
+
+typedef struct gga_result_row {
+    char exename[100];
+    char algo[10];
+    char graphfile[200];
+    uint32_t bucket, nnode, nedge, ns_nnode, ns_nedge, extranode, maxdeg;
+    int32_t  level;
+    uint32_t mda, iterations, runtime_ms;
+    float time_ms, ktime_ms, sktime_ms;
+} gga_result_row;
+/* Appends the 17-field CSV row of writeResult (myutils.h:86-113) to path. */
+int gga_write_result_row(const char *path, const gga_result_row *r);
+/* Graph name = last path component (getGraphname, myutils.h:76-84). */
+void gga_graph_name(const char *path, char *out, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GGA_H */

@@ -1,0 +1,184 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle, bit-exact.
+
+Every traversal mode x split threshold x golden vector; GPU chunk tables
+against the oracle's NodeSplit restatement; the split known answers of the
+reference's published runs on synthetic graphs with the published degree
+sequences.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gpugraphalgo_amd as gga
+from gpugraphalgo_amd import _lib
+import oracle
+from conftest import golden_graph
+
+pytestmark = pytest.mark.gpu
+
+MODES = ["topo", "worklist", "persistent"]
+INF = gga.INF
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def test_golden_vectors_all_modes(gpu, golden):
+    index, arrays, _ = golden
+    graphs = sorted({e["graph"] for e in index})
+    for name in graphs:
+        rp, dst, w = golden_graph(arrays, name)
+        announce(f"golden {name} n={rp.size - 1} m={dst.size}")
+        g = gga.csr_from_arrays(rp, dst, w)
+        with gga.DeviceGraph(g) as dg:
+            maxdeg = int(np.diff(rp.astype(np.int64)).max()) if dst.size else 0
+            for mda in sorted({0, 1, 2, 3, 64, max(maxdeg, 1)}):
+                if mda:
+                    dg.split(mda=mda)
+                else:
+                    dg.split()
+                for e in (e for e in index if e["graph"] == name):
+                    for mode in MODES:
+                        announce(f"  {e['key']} mda={mda} mode={mode}")
+                        run = dg.bfs if e["algo"] == "bfs" else dg.sssp
+                        d, st = run(e["src"], mode=mode, inf=e["inf"], count=True)
+                        np.testing.assert_array_equal(d, arrays[e["key"]],
+                                                      err_msg=f"{e['key']} mda={mda} {mode}")
+                        assert st["reached"] == int((arrays[e["key"]] != e["inf"]).sum())
+
+
+def test_rmat14_bfs_config_all_modes(gpu, golden):
+    _, arrays, meta = golden
+    g = gga.gen_rmat(14, 8, seed=1, wlo=1, whi=1)
+    want = arrays[meta["rmat14"]["key"]]
+    with gga.DeviceGraph(g) as dg:
+        for level in (0, 1, 4, 16, -1):
+            st = dg.split(level=level) if level else dg.split()
+            for mode in MODES:
+                announce(f"rmat14 bfs level={level} mode={mode}")
+                d, _ = dg.bfs(0, mode=mode)
+                np.testing.assert_array_equal(d, want)
+            assert st["ns_nnode"] == oracle.split_count(g.degrees(), st["mda"])
+
+
+@pytest.mark.parametrize("T", [1, 3, 32, 64, 128, 256, 1024])
+def test_split_chunks_equal_oracle_node_split(gpu, T):
+    rng = np.random.default_rng(T)
+    g = gga.gen_rmat(13, 16, seed=100 + T, wlo=1, whi=100)
+    announce(f"split chunks T={T} n={g.n} m={g.m}")
+    with gga.DeviceGraph(g) as dg:
+        st = dg.split(mda=T)
+        cb, own = dg.chunks()
+        ocb, oown, _, _ = oracle.node_split(g.row_ptr, T)
+        np.testing.assert_array_equal(cb, ocb)
+        np.testing.assert_array_equal(own, oown)
+        assert st["ns_nnode"] == own.size
+        assert st["extranode"] == own.size - g.n
+        assert st["maxdeg"] == g.maxdeg()[0]
+        assert st["max_chunk"] == int(np.diff(ocb.astype(np.int64)).max())
+    del rng
+
+
+def synth_from_dd(dd, seed):
+    """A graph with exactly the published degree sequence (destinations random)."""
+    dd = np.asarray(dd, dtype=np.int64)
+    deg = np.repeat(dd[:, 0], dd[:, 1]).astype(np.uint32)
+    rng = np.random.default_rng(seed)
+    rng.shuffle(deg)
+    rp = np.concatenate([[0], np.cumsum(deg, dtype=np.int64)]).astype(np.uint32)
+    dst = rng.integers(0, deg.size, int(rp[-1]), dtype=np.int64).astype(np.uint32)
+    return gga.csr_from_arrays(rp, dst, None)
+
+
+@pytest.mark.parametrize("graph", ["rmat20", "webs"])
+def test_split_kat_published_rows_on_gpu(gpu, split_kat, graph):
+    """Every distinct (L, bucket) of the published node-split runs on this graph:
+    the GPU split reproduces MAX_EDGES_ALLOWED and ns_nnode (result.vi)."""
+    g = synth_from_dd(split_kat["dd"][graph], 1)
+    rows = [r for r in split_kat["rows"] if r["graph"] == graph]
+    seen = {}
+    for r in rows:
+        seen[(r["level"], r["bucket"])] = (r["mda"], r["ns_nnode"], r["maxdeg"])
+    with gga.DeviceGraph(g) as dg:
+        for (level, bucket), (mda, ns, maxdeg) in sorted(seen.items()):
+            announce(f"kat {graph} L={level} bucket={bucket}")
+            st = dg.split(level=level, buckets=bucket if level < 0 else 0)
+            assert (st["mda"], st["ns_nnode"], st["maxdeg"]) == (mda, ns, maxdeg), (level, bucket)
+
+
+@pytest.mark.parametrize("buckets", [2, 5, 10, 15, 37])
+def test_histogram_threshold_equals_oracle(gpu, split_kat, buckets):
+    for graph in ("lj", "webB"):
+        g = synth_from_dd(split_kat["dd"][graph], 2)
+        announce(f"histogram {graph} buckets={buckets}")
+        with gga.DeviceGraph(g) as dg:
+            st = dg.split(level=-1, buckets=buckets)
+            assert st["mda"] == oracle.mda_histogram(g.degrees(), buckets)
+            assert st["ns_nnode"] == oracle.split_count(g.degrees(), st["mda"])
+
+
+@pytest.mark.parametrize("T", [32, 64, 128, 256, 1024])
+def test_rmat16_sssp_threshold_sweep(gpu, T):
+    g = gga.gen_rmat(16, 16, seed=7, wlo=1, whi=100)
+    src = g.maxdeg()[1]
+    want = oracle.dijkstra(g.row_ptr, g.dst, g.w, src)
+    serial, _, _ = oracle.serial_sp(g.row_ptr, g.dst, g.w, src)
+    np.testing.assert_array_equal(serial, want)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=T)
+        for mode in MODES:
+            announce(f"rmat16 sssp T={T} mode={mode}")
+            d, st = dg.sssp(src, mode=mode, count=True)
+            np.testing.assert_array_equal(d, want)
+            assert st["m_reach"] == oracle.m_reach(g.row_ptr, want)
+            assert st["edges_inspected"] >= st["m_reach"]
+
+
+def test_grid_bfs_sssp(gpu):
+    g = gga.gen_grid(200, 300, 0.005, seed=11, wlo=1, whi=1000)
+    want_s = oracle.dijkstra(g.row_ptr, g.dst, g.w, 0)
+    want_b = oracle.bfs_queue(g.row_ptr, g.dst, 0)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=4)
+        for mode in MODES:
+            announce(f"grid200x300 mode={mode}")
+            d, _ = dg.sssp(0, mode=mode)
+            np.testing.assert_array_equal(d, want_s)
+            d, _ = dg.bfs(0, mode=mode)
+            np.testing.assert_array_equal(d, want_b)
+
+
+def test_result_same_alone_and_after_other_queries(gpu):
+    g = gga.gen_rmat(14, 16, seed=3, wlo=1, whi=100)
+    rng = np.random.default_rng(0)
+    srcs = rng.integers(0, g.n, 12)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=64)
+        alone = {}
+        for s in srcs[:4]:
+            alone[int(s)] = dg.sssp(int(s), mode="worklist")[0]
+        for i, s in enumerate(srcs):
+            announce(f"interleaved query {i} src={s}")
+            dg.bfs(int(s), mode=MODES[i % 3])
+            dg.sssp(int(srcs[-1 - i]), mode=MODES[(i + 1) % 3])
+        for s, want in alone.items():
+            for mode in MODES:
+                np.testing.assert_array_equal(dg.sssp(s, mode=mode)[0], want)
+
+
+def test_device_verify_and_errors(gpu):
+    g = gga.gen_rmat(12, 8, seed=9, wlo=1, whi=100)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=16)
+        d, _ = dg.sssp(0)
+        r = dg.verify(0, bfs=False)
+        assert r["nerr"] == 0 and r["untight"] == 0 and r["src_ok"]
+        assert r["reached"] == int((d != INF).sum())
+        np.testing.assert_array_equal(dg.copy_dist(), d)
+        with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+            dg.sssp(g.n)
+        with pytest.raises(_lib.GgaError, match="GGA_ERANGE"):
+            dg.split(level=-1, buckets=1 << 20)  # more buckets than the kernel holds
