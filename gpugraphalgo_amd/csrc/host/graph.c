@@ -557,7 +557,8 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
         }
         if ((rc = ensure_kev(g, 2))) goto fail;
         HIP_TRY(hipEventRecord(g->kev[0], g->stream));
-        K_TRY(ggak_persistent(dg, dw, bfs, count, grid, max_iters, 10000u, g->stream));
+        /* every barrier wait is bounded (5 s); a step never comes near that */
+        K_TRY(ggak_persistent(dg, dw, bfs, count, grid, max_iters, 5000u, g->stream));
         HIP_TRY(hipEventRecord(g->kev[1], g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
         HIP_TRY(hipEventElapsedTime(&ktime, g->kev[0], g->kev[1]));

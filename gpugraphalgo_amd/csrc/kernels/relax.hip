@@ -459,6 +459,11 @@ extern "C" uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count)
                     : reinterpret_cast<const void *>(&k_wl_step<false, false>);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, 0) != hipSuccess) return 0;
     if (per <= 0) return 0;
+    // A grid barrier needs every workgroup resident at once.  On gfx950 the
+    // occupancy query (7-8 per CU for these kernels) overstates what the
+    // hardware co-schedules (6 per CU measured for the persistent kernel), and
+    // the cooperative launch does not reject the excess; 4 per CU keeps a margin.
+    if (persistent && per > 4) per = 4;
     return (uint32_t)cus * (uint32_t)per;
 }
 
