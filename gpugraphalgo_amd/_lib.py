@@ -45,12 +45,14 @@ class SplitStats(ctypes.Structure):
 
 class Opts(ctypes.Structure):
     _fields_ = [("mode", i32), ("inf", u32), ("max_iters", u32), ("block", u32),
-                ("sync_every", u32), ("use_graph", u32), ("count", u32), ("grid", u32)]
+                ("sync_every", u32), ("use_graph", u32), ("count", u32), ("grid", u32),
+                ("dense_min", u32)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", u32), ("reached", u32), ("time_ms", ctypes.c_float),
-                ("ktime_ms", ctypes.c_float), ("sktime_ms", ctypes.c_float), ("mode", i32),
+                ("ktime_ms", ctypes.c_float), ("sktime_ms", ctypes.c_float),
+                ("ctime_ms", ctypes.c_float), ("mode", i32),
                 ("edges_inspected", u64), ("chunks_expanded", u64), ("relaxations", u64),
                 ("m_reach", u64), ("launches", u64)]
 

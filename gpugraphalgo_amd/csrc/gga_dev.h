@@ -40,9 +40,21 @@ typedef struct gga_dev_work {
     uint32_t *q0;
     uint32_t *q1;
     uint32_t *ctl;
+    uint64_t *cnt;       /* GGA_CNT_SLOTS x 4 work counters, striped over many addresses */
+    uint32_t *tiles;     /* list compaction: per-tile counts, then offsets (GGA_TILES(n) + 1) */
     uint32_t inf;
-    uint32_t qcap;   /* capacity of q0 / q1 in entries (>= C) */
+    uint32_t qcap;       /* capacity of q0 / q1 in entries (>= C) */
+    uint32_t dense_min;  /* a step whose list holds >= dense_min chunks is a flag step */
+    uint32_t pad;
 } gga_dev_work;
+
+/* Device-scope atomics are performed past the per-XCD L2 and same-address
+ * ones serialize (about 16 ns each on MI355X), so nothing hot shares one
+ * counter: work counters are striped over GGA_CNT_SLOTS addresses. */
+#define GGA_CNT_SLOTS 1024u
+/* list compaction works on tiles of GGA_TILE vertices */
+#define GGA_TILE 2048u
+#define GGA_TILES(n) (((n) + GGA_TILE - 1u) / GGA_TILE)
 
 /* ctl word offsets (uint32 index); hot atomics sit on their own 128-byte line */
 #define GGA_CTL_LINE      32u
@@ -53,10 +65,12 @@ typedef struct gga_dev_work {
 #define GGA_CTL_STEPS     (7u * GGA_CTL_LINE + 1u)      /* steps that had a non-empty list */
 #define GGA_CTL_ERR       (7u * GGA_CTL_LINE + 2u)      /* error bits (see GGA_DEVERR_*) */
 #define GGA_CTL_CNT       (8u * GGA_CTL_LINE)           /* 3 x uint64: edges, chunks, relaxations */
-#define GGA_CTL_BAR       (9u * GGA_CTL_LINE)           /* grid barrier arrival counter */
+#define GGA_CTL_BAR       (9u * GGA_CTL_LINE)           /* grid barrier: arrivals of group leaders */
 #define GGA_CTL_BARGEN    (10u * GGA_CTL_LINE)          /* grid barrier generation */
 #define GGA_CTL_RED       (11u * GGA_CTL_LINE)          /* reductions: 4 x uint64 scratch */
-#define GGA_CTL_WORDS     (12u * GGA_CTL_LINE)
+#define GGA_BAR_GROUPS    32u                           /* grid barrier: first-level groups */
+#define GGA_CTL_BARG(i)   ((12u + (i)) * GGA_CTL_LINE)  /* grid barrier: arrivals within group i */
+#define GGA_CTL_WORDS     ((12u + GGA_BAR_GROUPS) * GGA_CTL_LINE)
 
 #define GGA_DEVERR_SPIN   1u   /* barrier spin limit hit */
 #define GGA_DEVERR_QFULL  2u   /* worklist overflow (cannot happen with de-duplication) */

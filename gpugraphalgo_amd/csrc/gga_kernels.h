@@ -37,6 +37,9 @@ int ggak_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src, uint32_t epoc
 int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int count, hipStream_t s);
 int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t phase, int bfs, int count,
                  uint32_t grid, hipStream_t s);
+/* After ggak_wl_step(phase): builds the next list from step stamps when that
+ * step was a flag step (its list held >= w.dense_min chunks); else a no-op. */
+int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t phase, hipStream_t s);
 /* Occupancy-bound grid (workgroups) for the worklist / persistent kernels. */
 uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count);
 /* One cooperative launch running worklist steps until the list is empty or

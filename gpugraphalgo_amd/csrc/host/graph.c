@@ -30,7 +30,8 @@ struct gga_graph {
     uint32_t *d_cstart, *d_cbeg, *d_cown;
     gga_split_stats sst;
     /* query workspace */
-    uint32_t *d_dist, *d_mark, *d_q0, *d_q1, *d_ctl;
+    uint32_t *d_dist, *d_mark, *d_q0, *d_q1, *d_ctl, *d_tiles;
+    uint64_t *d_cnt;
     uint32_t qcap;
     uint32_t *h_pin;          /* pinned: GGA_CTL_WORDS words */
     uint32_t epoch;
@@ -122,8 +123,12 @@ static gga_dev_work dev_work(const gga_graph *g, uint32_t inf)
     w.q0 = g->d_q0;
     w.q1 = g->d_q1;
     w.ctl = g->d_ctl;
+    w.cnt = g->d_cnt;
+    w.tiles = g->d_tiles;
     w.inf = inf;
     w.qcap = g->qcap;
+    w.dense_min = 0xFFFFFFFFu;
+    w.pad = 0;
     return w;
 }
 
@@ -159,6 +164,8 @@ int gga_graph_free(gga_graph *g)
     dfree(g, (void **)&g->d_dist, 4ull * g->n);
     dfree(g, (void **)&g->d_mark, 4ull * g->n);
     dfree(g, (void **)&g->d_ctl, 4ull * GGA_CTL_WORDS);
+    dfree(g, (void **)&g->d_cnt, 32ull * GGA_CNT_SLOTS);
+    dfree(g, (void **)&g->d_tiles, 4ull * (GGA_TILES((uint64_t)g->n) + 1));
     if (g->h_pin) hipHostFree(g->h_pin);
     for (int i = 0; i < 4; i++)
         if (g->ev[i]) hipEventDestroy(g->ev[i]);
@@ -201,6 +208,8 @@ static int graph_create(uint32_t n, uint32_t m, const uint32_t *row_ptr, const u
     if ((rc = dmalloc(g, (void **)&g->d_col, 4ull * m))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_w, 4ull * m))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_ctl, 4ull * GGA_CTL_WORDS))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_cnt, 32ull * GGA_CNT_SLOTS))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_tiles, 4ull * (GGA_TILES((uint64_t)n) + 1)))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_dist, 4ull * n))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_mark, 4ull * n))) goto fail;
     HIP_TRY(hipMemcpyAsync(g->d_row, row_ptr, 4ull * ((uint64_t)n + 1), hipMemcpyHostToDevice, g->stream));
@@ -483,6 +492,7 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
     }
     gga_dev_graph dg = dev_graph(g);
     gga_dev_work dw = dev_work(g, inf);
+    dw.dense_min = o.dense_min ? o.dense_min : (g->n / 64u > 4096u ? g->n / 64u : 4096u);
     s.mode = mode;
     float ktime = 0.f;
 
@@ -521,22 +531,28 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
             goto fail;
         }
         uint32_t per = o.sync_every ? o.sync_every : 8u;
-        if ((rc = ensure_kev(g, 2 * per))) goto fail;
+        if ((rc = ensure_kev(g, 3 * per))) goto fail;
         uint32_t step = 0;
         for (;;) {
+            /* `per` steps back to back: every kernel reads its list length on
+             * the device, so steps after convergence are empty no-ops */
             for (uint32_t k = 0; k < per; k++) {
-                HIP_TRY(hipEventRecord(g->kev[2 * k], g->stream));
+                HIP_TRY(hipEventRecord(g->kev[3 * k], g->stream));
                 K_TRY(ggak_wl_step(dg, dw, (step + k) % 6u, bfs, count, grid, g->stream));
-                HIP_TRY(hipEventRecord(g->kev[2 * k + 1], g->stream));
+                HIP_TRY(hipEventRecord(g->kev[3 * k + 1], g->stream));
+                K_TRY(ggak_compact(dg, dw, (step + k) % 6u, g->stream));
+                HIP_TRY(hipEventRecord(g->kev[3 * k + 2], g->stream));
             }
             step += per;
             HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_TAIL(step % 3u), 4, hipMemcpyDeviceToHost,
                                    g->stream));
             HIP_TRY(hipStreamSynchronize(g->stream));
             for (uint32_t k = 0; k < per; k++) {
-                float t = 0.f;
-                HIP_TRY(hipEventElapsedTime(&t, g->kev[2 * k], g->kev[2 * k + 1]));
+                float t = 0.f, tc = 0.f;
+                HIP_TRY(hipEventElapsedTime(&t, g->kev[3 * k], g->kev[3 * k + 1]));
+                HIP_TRY(hipEventElapsedTime(&tc, g->kev[3 * k + 1], g->kev[3 * k + 2]));
                 ktime += t;
+                s.ctime_ms += tc;
             }
             s.launches += per;
             if (g->h_pin[0] == 0) break;

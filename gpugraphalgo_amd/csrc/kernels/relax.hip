@@ -5,12 +5,15 @@
 // (cudaKernels.h:129-220) and the dynamic-parallelism child grids
 // (my_drelax_dynamicP / dynamicP_relax_coal, cudaKernels.h:347-377,438-521).
 //
-// Work unit: a wave takes 64 chunk ids (one per lane), reads each chunk's
-// bounds and its owner's distance, and walks the concatenation of the 64 edge
-// ranges 64 slots per round (wave_owner).  A chunk holds at most T edges, so a
-// hub vertex is spread over many chunks and many waves; this is the node
-// split.  Chunks of one vertex are adjacent in the worklist and in memory, so
-// col[]/w[] loads of a round are contiguous runs.
+// Work unit: a wave takes a batch of up to 64 chunk ids (one per lane), reads
+// each chunk's bounds and its owner's distance, and walks the concatenation of
+// the batch's edge ranges 64 * UNROLL slots per round (wave_owner).  A chunk
+// holds at most T edges, so a hub vertex is spread over many chunks and many
+// waves; this is the node split.  Chunks of one vertex are adjacent in the
+// worklist and in memory, so col[]/w[] loads of a round are contiguous runs.
+// Batches are dealt round-robin to the waves of the grid (no shared counter);
+// the batch size shrinks with the frontier so that a small frontier still
+// spreads over every wave.
 //
 // Relax (all modes): alt = dist[u] + w (skipped when it would wrap or reach
 // inf); a plain read of dist[v] guards the atomicMin (cudaKernels.h:324-325).
@@ -18,14 +21,24 @@
 // the guard cannot lose an update.  The owner's distance is read when the
 // chunk is expanded, so an update made earlier in the same step is used.
 //
-// Worklist: a vertex whose distance dropped is queued once per step (step
-// stamp in mark[]; BFS needs none because a level-synchronous atomicMin
-// succeeds once per vertex).  Queued vertices are staged in wave-private LDS
-// and flushed as chunk ids with one atomicAdd per 64 vertices.
+// Next frontier, chosen per step from the size of the current one:
+//   queue step (small frontier): atomicMin with return; a vertex whose
+//     distance dropped is queued once per step (step stamp in mark[], taken
+//     with an exchange; BFS needs none since a level-synchronous atomicMin
+//     succeeds once per vertex).  Queued vertices are staged in wave-private
+//     LDS and flushed as chunk ids with one atomicAdd per 64 vertices.
+//   flag step (large frontier): fire-and-forget atomicMin and a plain store of
+//     the step stamp into mark[v]; then a three-pass compaction (count per
+//     tile, scan of tile counts, write) rebuilds the list from mark[] with no
+//     shared counter at all.
+// Device-scope atomics on MI355X execute past the per-XCD L2 and atomics on
+// one address serialize (~16 ns each), so no hot path funnels through a single
+// counter: batch assignment is static, work counters are striped, list
+// allocation in flag steps is a scan, and the grid barrier is two-level.
 //
 // Bytes per unit of work (the roofline model): per edge inspected
-// col 4 + w 4 (SSSP) + dist[v] 4; per chunk expanded cbeg 8 + cown 4 +
-// dist[u] 4 (+ 4 for the worklist id); per successful relaxation 4 (+ 4 push).
+// col 4 + w 4 (SSSP) + dist[v] 4; per chunk expanded cbeg 4 + cown 4 +
+// dist[u] 4 (+ 4 for the worklist id); per relaxation 4 + 4 (queue / stamp).
 #include "../gga_kernels.h"
 #include "wave.hpp"
 
@@ -35,11 +48,28 @@ namespace {
 
 constexpr uint32_t BLOCK = 256;
 constexpr uint32_t WAVES = BLOCK / GGA_WAVE;
-constexpr uint32_t PBUF = 256;  // queued vertices staged per wave before a flush
+constexpr uint32_t PBUF = 256;          // queued vertices staged per wave before a flush
+constexpr uint32_t CITEMS = GGA_TILE / BLOCK;  // compaction: vertices per thread per tile
+constexpr uint32_t CSMALL = 4;          // compaction: a thread writes at most this many ids itself
+#ifndef GGA_UNROLL
+#define GGA_UNROLL 4
+#endif
+constexpr uint32_t UNROLL = GGA_UNROLL; // edge slots per lane per round in expand_batch
+
+enum { OUT_CHANGED = 0, OUT_QUEUE = 1, OUT_FLAG = 2 };
 
 struct Counters {
     uint64_t edges = 0, chunks = 0, relax = 0;
 };
+
+__device__ __forceinline__ void queue_store(uint32_t *qout, uint32_t pos, uint32_t id, uint32_t qcap,
+                                            uint32_t *ctl) {
+    // a vertex is queued at most once per step, so pos < C <= qcap; the guard
+    // only turns a broken invariant into an error bit
+    if (pos < qcap) qout[pos] = id;
+    else __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_QFULL, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Append the wave's staged vertices to the chunk worklist: every staged vertex
 // contributes its chunk range [cstart[v], cstart[v + 1]).
@@ -67,21 +97,14 @@ __device__ __forceinline__ void flush_pushes(const gga_dev_graph &g, const uint3
             const uint32_t kk = k < tot ? k : tot - 1;
             const uint32_t j = wave_owner(incl, kk);
             const uint32_t id = wave_bcast(cs, j) + (kk - wave_bcast(excl, j));
-            if (k < tot) {
-                // a vertex is queued at most once per step, so base + k < C <= qcap;
-                // the guard only turns a broken invariant into an error bit
-                if ((uint64_t)base + k < qcap) qout[base + k] = id;
-                else __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_QFULL, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (k < tot) queue_store(qout, base + k, id, qcap, ctl);
         }
     }
     __builtin_amdgcn_wave_barrier();
 }
 
-// Expand one batch of up to 64 chunks (this lane's chunk c when valid).
-// PUSH: queue improved vertices (worklist); else raise `improved` (topology).
-template <bool BFS, bool COUNT, bool PUSH>
+// Expand one batch of chunks (this lane's chunk c when valid).
+template <bool BFS, bool COUNT, int OUT>
 __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_dev_work &w,
                                              uint32_t c, bool valid, uint32_t it, uint32_t *pbuf,
                                              uint32_t &pcount, uint32_t *qout, uint32_t *tail,
@@ -91,9 +114,9 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
     const uint32_t inf = w.inf;
     uint32_t eb = 0, n = 0, du = inf;
     if (valid) {
-        eb = g.cbeg[c];
-        const uint32_t ee = g.cbeg[c + 1];
-        du = dist[g.cown[c]];
+        eb = stream_load(&g.cbeg[c]);
+        const uint32_t ee = stream_load(&g.cbeg[c + 1]);
+        du = dist[stream_load(&g.cown[c])];
         n = du < inf ? ee - eb : 0u;
     }
     const uint32_t incl = wave_incl_scan(n);
@@ -103,59 +126,98 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
         cnt.chunks += (uint32_t)__popcll(__ballot(n > 0));
     }
     const uint32_t excl = incl - n;
-    for (uint32_t base = 0; base < total; base += GGA_WAVE) {
-        const uint32_t k = base + l;
-        const uint32_t kk = k < total ? k : total - 1;
-        const uint32_t j = wave_owner(incl, kk);
-        const uint32_t e = wave_bcast(eb, j) + (kk - wave_bcast(excl, j));
-        const uint32_t ds = wave_bcast(du, j);
-        bool imp = false;
-        uint32_t v = 0;
-        if (k < total) {
-            v = g.col[e];
-            const uint32_t wt = BFS ? 1u : g.w[e];
-            const uint32_t alt = ds + wt;
-            if (alt >= ds && alt < inf) {
-                if (alt < dist[v]) {
-                    const uint32_t old = atomic_min_u32(&dist[v], alt);
+    // UNROLL slots per lane per round.  Slots past the end are clamped to the
+    // last slot, so every load below has an in-bounds address and all UNROLL
+    // gathers of a lane are issued before the first result is used.
+    for (uint32_t base = 0; base < total; base += GGA_WAVE * UNROLL) {
+        uint32_t e[UNROLL], ds[UNROLL], v[UNROLL], wt[UNROLL], cur[UNROLL];
+        bool act[UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; u++) {
+            const uint32_t k = base + u * GGA_WAVE + l;
+            act[u] = k < total;
+            const uint32_t kk = act[u] ? k : total - 1;
+            const uint32_t j = wave_owner(incl, kk);
+            e[u] = wave_bcast(eb, j) + (kk - wave_bcast(excl, j));
+            ds[u] = wave_bcast(du, j);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; u++) {
+            v[u] = stream_load(&g.col[e[u]]);
+            wt[u] = BFS ? 1u : stream_load(&g.w[e[u]]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; u++) cur[u] = dist[v[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; u++) {
+            bool imp = false;
+            const uint32_t alt = ds[u] + wt[u];
+            if (act[u] && alt >= ds[u] && alt < inf && alt < cur[u]) {
+                if (OUT == OUT_FLAG) {
+                    (void)atomic_min_u32(&dist[v[u]], alt);  // result unused: no-return atomic
+                    w.mark[v[u]] = it;
+                    imp = true;
+                } else {
+                    const uint32_t old = atomic_min_u32(&dist[v[u]], alt);
                     imp = alt < old;
                 }
             }
-        }
-        if (COUNT) cnt.relax += (uint32_t)__popcll(__ballot(imp));
-        if (PUSH) {
-            bool want = imp;
-            if (!BFS && want) want = (w.mark[v] != it) && (atomic_xchg_u32(&w.mark[v], it) != it);
-            const uint64_t mask = __ballot(want);
-            if (mask) {
-                const uint32_t nw = (uint32_t)__popcll(mask);
-                if (pcount + nw > PBUF) {
-                    flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
-                    pcount = 0;
+            if (COUNT) cnt.relax += (uint32_t)__popcll(__ballot(imp));
+            if (OUT == OUT_QUEUE) {
+                bool want = imp;
+                if (!BFS && want)
+                    want = (w.mark[v[u]] != it) && (atomic_xchg_u32(&w.mark[v[u]], it) != it);
+                const uint64_t mask = __ballot(want);
+                if (mask) {
+                    const uint32_t nw = (uint32_t)__popcll(mask);
+                    if (pcount + nw > PBUF) {
+                        flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
+                        pcount = 0;
+                    }
+                    if (want) {
+                        const uint32_t pos = pcount + __builtin_amdgcn_mbcnt_hi(
+                                                          (uint32_t)(mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                        pbuf[pos] = v[u];
+                    }
+                    pcount += nw;
+                    __builtin_amdgcn_wave_barrier();
                 }
-                if (want) {
-                    const uint32_t pos = pcount + __builtin_amdgcn_mbcnt_hi(
-                                                      (uint32_t)(mask >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                    pbuf[pos] = v;
-                }
-                pcount += nw;
-                __builtin_amdgcn_wave_barrier();
+            } else {
+                improved |= imp;
             }
-        } else {
-            improved |= imp;
         }
     }
 }
 
+// Work counters: lane 0 of each wave adds to one of GGA_CNT_SLOTS striped slots.
 template <bool COUNT>
-__device__ __forceinline__ void publish_counters(uint32_t *ctl, const Counters &c) {
+__device__ __forceinline__ void publish_counters(const gga_dev_work &w, const Counters &c) {
     if (COUNT && lane_id() == 0) {
-        uint64_t *cnt = reinterpret_cast<uint64_t *>(ctl + GGA_CTL_CNT);
-        if (c.edges) atomic_add_u64(&cnt[0], c.edges);
-        if (c.chunks) atomic_add_u64(&cnt[1], c.chunks);
-        if (c.relax) atomic_add_u64(&cnt[2], c.relax);
+        const uint32_t slot = (blockIdx.x * WAVES + threadIdx.x / GGA_WAVE) % GGA_CNT_SLOTS;
+        uint64_t *s = w.cnt + 4u * slot;
+        if (c.edges) atomic_add_u64(&s[0], c.edges);
+        if (c.chunks) atomic_add_u64(&s[1], c.chunks);
+        if (c.relax) atomic_add_u64(&s[2], c.relax);
     }
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+// s_wave needs WAVES entries.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t &excl, uint32_t *s_wave) {
+    const uint32_t wv = threadIdx.x / GGA_WAVE;
+    const uint32_t incl = wave_incl_scan(x);
+    if (lane_id() == GGA_WAVE - 1) s_wave[wv] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (uint32_t i = 0; i < WAVES; i++) {
+        const uint32_t t = s_wave[i];
+        off += i < wv ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    excl = off + incl - x;
+    return tot;
 }
 
 // ---------------------------------------------------------------- init
@@ -167,12 +229,13 @@ __global__ __launch_bounds__(BLOCK) void k_fill(uint32_t *p, uint32_t value, uin
 
 // dist = inf except dist[src] = 0 (bfs_sssp.cu:302-318, with the source as an
 // argument instead of the hard-coded parent 0); worklist = chunks of src;
-// ring slot 0 = {len, step epoch}; counters zeroed.
+// ring slot 0 = {len, step epoch}; counters and barrier zeroed.
 __global__ __launch_bounds__(BLOCK) void k_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src,
                                                       uint32_t epoch) {
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-         v += (uint64_t)gridDim.x * blockDim.x)
-        w.dist[v] = (v == src) ? 0u : w.inf;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nth = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = tid; v < g.n; v += nth) w.dist[v] = (v == src) ? 0u : w.inf;
+    for (uint64_t i = tid; i < 4ull * GGA_CNT_SLOTS; i += nth) w.cnt[i] = 0;
     if (blockIdx.x == 0) {
         const uint32_t cs = g.cstart[src], nch = g.cstart[src + 1] - cs;
         for (uint32_t i = threadIdx.x; i < nch; i += blockDim.x) w.q0[i] = cs + i;
@@ -181,6 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_query_init(gga_dev_graph g, gga_dev_w
             w.ctl[GGA_CTL_BC(threadIdx.x)] = 0;
             w.ctl[GGA_CTL_ITER(threadIdx.x)] = epoch;
         }
+        if (threadIdx.x < GGA_BAR_GROUPS) w.ctl[GGA_CTL_BARG(threadIdx.x)] = 0;
         if (threadIdx.x == 0) {
             w.ctl[GGA_CTL_CHANGED] = 0;
             w.ctl[GGA_CTL_STEPS] = 0;
@@ -203,38 +267,129 @@ __global__ __launch_bounds__(BLOCK) void k_topo_step(gga_dev_graph g, gga_dev_wo
     bool improved = false;
     uint32_t pcount = 0;
     Counters cnt;
-    expand_batch<BFS, COUNT, false>(g, w, valid ? (uint32_t)c : 0u, valid, 0u, nullptr, pcount,
-                                    nullptr, nullptr, improved, cnt);
+    expand_batch<BFS, COUNT, OUT_CHANGED>(g, w, valid ? (uint32_t)c : 0u, valid, 0u, nullptr, pcount,
+                                          nullptr, nullptr, improved, cnt);
     if (__ballot(improved) && lane_id() == 0) w.ctl[GGA_CTL_CHANGED] = 1u;  // *changed = true
-    publish_counters<COUNT>(w.ctl, cnt);
+    publish_counters<COUNT>(w, cnt);
 }
 
 // ------------------------------------------------------ worklist step
+__device__ __forceinline__ bool flag_step(const gga_dev_work &w, uint32_t nq) {
+    return nq >= w.dense_min;
+}
+
 // One step = expand the list of ring slot (step % 3), fill slot (step + 1) % 3.
 template <bool BFS, bool COUNT>
 __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_dev_work &w,
                                               uint32_t step, uint32_t nq, uint32_t it,
                                               uint32_t *pbuf, Counters &cnt) {
-    const uint32_t sin = step % 3u, sout = (step + 1u) % 3u;
+    const uint32_t sout = (step + 1u) % 3u;
     const uint32_t *qin = (step & 1u) ? w.q1 : w.q0;
     uint32_t *qout = (step & 1u) ? w.q0 : w.q1;
     uint32_t *tail = &w.ctl[GGA_CTL_TAIL(sout)];
-    uint32_t *bc = &w.ctl[GGA_CTL_BC(sin)];
-    const uint32_t nbatch = (nq + GGA_WAVE - 1) / GGA_WAVE;
+    // batches of bsz chunks dealt round-robin: at least ~8 batches per wave
+    // when the list is long enough, at most 64 chunks per batch
+    const uint32_t waves = gridDim.x * WAVES;
+    const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
+    uint32_t bsz = nq / (8u * waves);
+    bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
+    const uint32_t nbatch = (nq + bsz - 1) / bsz;
+    const bool flags = flag_step(w, nq);
     const uint32_t l = lane_id();
     uint32_t pcount = 0;
     bool dummy = false;
-    for (;;) {
-        uint32_t b = 0;
-        if (l == 0) b = atomic_add_u32(bc, 1u);
-        b = wave_bcast(b, 0);
-        if (b >= nbatch) break;
-        const uint32_t idx = b * GGA_WAVE + l;
-        const bool valid = idx < nq;
-        const uint32_t c = valid ? qin[idx] : 0u;
-        expand_batch<BFS, COUNT, true>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt);
+    for (uint32_t b = wid; b < nbatch; b += waves) {
+        const uint32_t idx = b * bsz + l;
+        const bool valid = l < bsz && idx < nq;
+        const uint32_t c = valid ? stream_load(&qin[idx]) : 0u;
+        if (flags)
+            expand_batch<BFS, COUNT, OUT_FLAG>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt);
+        else
+            expand_batch<BFS, COUNT, OUT_QUEUE>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt);
     }
     if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
+}
+
+// ---- list compaction after a flag step: chunks of every vertex stamped `it`
+// Per-thread view of one tile: CITEMS vertices v0 + i * BLOCK + tid (coalesced).
+__device__ __forceinline__ uint32_t tile_items(const gga_dev_graph &g, const gga_dev_work &w,
+                                               uint32_t t, uint32_t it, uint32_t *cs, uint32_t *nch) {
+    const uint64_t v0 = (uint64_t)t * GGA_TILE;
+    uint32_t mine = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        const uint64_t v = v0 + (uint64_t)i * BLOCK + threadIdx.x;
+        cs[i] = 0;
+        nch[i] = 0;
+        if (v < g.n && w.mark[v] == it) {
+            cs[i] = g.cstart[v];
+            nch[i] = g.cstart[v + 1] - cs[i];
+        }
+        mine += nch[i];
+    }
+    return mine;
+}
+
+// pass 1: tiles[t] = number of chunk ids tile t contributes
+__device__ __forceinline__ void compact_count(const gga_dev_graph &g, const gga_dev_work &w, uint32_t it,
+                                              uint32_t tile0, uint32_t tstride, uint32_t *s_wave) {
+    const uint32_t ntiles = GGA_TILES(g.n);
+    for (uint32_t t = tile0; t < ntiles; t += tstride) {
+        uint32_t cs[CITEMS], nch[CITEMS], excl;
+        const uint32_t tot = block_excl_scan(tile_items(g, w, t, it, cs, nch), excl, s_wave);
+        if (threadIdx.x == 0) w.tiles[t] = tot;
+    }
+}
+
+// pass 2 (one workgroup): exclusive scan of tiles[]; the total is the new list length
+__device__ __forceinline__ void compact_scan(const gga_dev_graph &g, const gga_dev_work &w,
+                                             uint32_t step, uint32_t *s_wave) {
+    const uint32_t ntiles = GGA_TILES(g.n);
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t x = i < ntiles ? w.tiles[i] : 0u;
+        uint32_t excl;
+        const uint32_t tot = block_excl_scan(x, excl, s_wave);
+        if (i < ntiles) w.tiles[i] = carry + excl;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) w.ctl[GGA_CTL_TAIL((step + 1u) % 3u)] = carry;
+}
+
+// pass 3: write.  A vertex with few chunks is written by its thread; a hub's
+// hundreds of chunk ids are written by the whole wave, 64 per store round.
+__device__ __forceinline__ void compact_write(const gga_dev_graph &g, const gga_dev_work &w,
+                                              uint32_t step, uint32_t it, uint32_t tile0,
+                                              uint32_t tstride, uint32_t *s_wave) {
+    uint32_t *qout = (step & 1u) ? w.q0 : w.q1;
+    const uint32_t ntiles = GGA_TILES(g.n);
+    for (uint32_t t = tile0; t < ntiles; t += tstride) {
+        uint32_t cs[CITEMS], nch[CITEMS], excl;
+        const uint32_t mine = tile_items(g, w, t, it, cs, nch);
+        block_excl_scan(mine, excl, s_wave);
+        uint32_t pos = w.tiles[t] + excl;
+        uint32_t bpos[CITEMS];
+#pragma unroll
+        for (uint32_t i = 0; i < CITEMS; i++) {
+            bpos[i] = pos;
+            if (nch[i] <= CSMALL)
+                for (uint32_t k = 0; k < nch[i]; k++) queue_store(qout, pos + k, cs[i] + k, w.qcap, w.ctl);
+            pos += nch[i];
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < CITEMS; i++) {
+            uint64_t big = __ballot(nch[i] > CSMALL);
+            while (big) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(big);
+                big &= big - 1;
+                const uint32_t p = wave_bcast(bpos[i], j), c0 = wave_bcast(cs[i], j);
+                const uint32_t cnt = wave_bcast(nch[i], j);
+                for (uint32_t k = lane_id(); k < cnt; k += GGA_WAVE)
+                    queue_store(qout, p + k, c0 + k, w.qcap, w.ctl);
+            }
+        }
+    }
 }
 
 // Per-step bookkeeping done by one thread: next slot's step number, clear the
@@ -244,36 +399,75 @@ __device__ __forceinline__ void step_bookkeeping(uint32_t *ctl, uint32_t step, u
     const uint32_t sout = (step + 1u) % 3u, sclr = (step + 2u) % 3u;
     ctl[GGA_CTL_ITER(sout)] = it + 1u;
     ctl[GGA_CTL_TAIL(sclr)] = 0u;
-    ctl[GGA_CTL_BC(sclr)] = 0u;
     if (nq) ctl[GGA_CTL_STEPS] += 1u;
+}
+
+__device__ __forceinline__ uint32_t step_len(const gga_dev_work &w, uint32_t step) {
+    const uint32_t t = w.ctl[GGA_CTL_TAIL(step % 3u)];
+    return t < w.qcap ? t : w.qcap;  // > qcap only with QFULL set
 }
 
 template <bool BFS, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
-    const uint32_t sin = step % 3u;
-    const uint32_t tail_in = w.ctl[GGA_CTL_TAIL(sin)];
-    const uint32_t nq = tail_in < w.qcap ? tail_in : w.qcap;  // > qcap only with QFULL set
-    const uint32_t it = w.ctl[GGA_CTL_ITER(sin)];
+    const uint32_t nq = step_len(w, step);
+    const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
     if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
     if (nq == 0) return;
     Counters cnt;
     worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
-    publish_counters<COUNT>(w.ctl, cnt);
+    publish_counters<COUNT>(w, cnt);
+}
+
+// The three compaction passes run after k_wl_step(step); each is a no-op
+// unless that step was a flag step.
+__global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+    __shared__ uint32_t s_wave[WAVES];
+    const uint32_t nq = step_len(w, step);
+    if (nq == 0 || !flag_step(w, nq)) return;
+    compact_count(g, w, w.ctl[GGA_CTL_ITER(step % 3u)], blockIdx.x, gridDim.x, s_wave);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_compact_scan(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+    __shared__ uint32_t s_wave[WAVES];
+    const uint32_t nq = step_len(w, step);
+    if (nq == 0 || !flag_step(w, nq)) return;
+    compact_scan(g, w, step, s_wave);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+    __shared__ uint32_t s_wave[WAVES];
+    const uint32_t nq = step_len(w, step);
+    if (nq == 0 || !flag_step(w, nq)) return;
+    compact_write(g, w, step, w.ctl[GGA_CTL_ITER(step % 3u)], blockIdx.x, gridDim.x, s_wave);
 }
 
 // ------------------------------------------------------ persistent kernel
-// Grid barrier for a cooperative launch.  Every wait is bounded by
-// `spin_ticks` of the 100 MHz wall clock; on expiry the error bit is set and
-// every workgroup leaves the step loop at its next check.
+// Two-level grid barrier for a cooperative launch: workgroup b arrives at the
+// counter of group b % GGA_BAR_GROUPS; the last arrival of a group arrives at
+// the top counter; the last group leader releases everyone by bumping the
+// generation.  Each counter sees at most ~nblocks / 32 + 32 atomics per
+// barrier instead of nblocks.  Every wait is bounded by `spin_ticks` of the
+// 100 MHz wall clock; on expiry the error bit is set and every workgroup
+// leaves the step loop at its next check.
 __device__ __forceinline__ void grid_barrier(uint32_t *ctl, uint32_t nblocks, uint32_t &gen,
                                              uint64_t spin_ticks) {
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t arrived =
-            __hip_atomic_fetch_add(&ctl[GGA_CTL_BAR], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        if (arrived == nblocks) {
+        const uint32_t ngroups = nblocks < GGA_BAR_GROUPS ? nblocks : GGA_BAR_GROUPS;
+        const uint32_t grp = blockIdx.x % GGA_BAR_GROUPS;
+        const uint32_t gsize = (nblocks - grp + GGA_BAR_GROUPS - 1u) / GGA_BAR_GROUPS;
+        bool release = false;
+        const uint32_t arrived = __hip_atomic_fetch_add(&ctl[GGA_CTL_BARG(grp)], 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (arrived == gsize) {
+            __hip_atomic_store(&ctl[GGA_CTL_BARG(grp)], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t top = __hip_atomic_fetch_add(&ctl[GGA_CTL_BAR], 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            release = top == ngroups;
+        }
+        if (release) {
             __hip_atomic_store(&ctl[GGA_CTL_BAR], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctl[GGA_CTL_BARGEN], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
@@ -298,6 +492,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
                                                       uint32_t max_steps, uint64_t spin_ticks) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
     __shared__ uint32_t s_state[2];
+    __shared__ uint32_t s_wave[WAVES];
     uint32_t gen = 0;
     if (threadIdx.x == 0)
         gen = __hip_atomic_load(&w.ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -315,13 +510,37 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
         const uint32_t it = w.ctl[GGA_CTL_ITER(sin)];
         if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
         worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+        if (flag_step(w, nq)) {
+            grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
+            compact_count(g, w, it, blockIdx.x, gridDim.x, s_wave);
+            grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
+            if (blockIdx.x == 0) compact_scan(g, w, step, s_wave);
+            grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
+            compact_write(g, w, step, it, blockIdx.x, gridDim.x, s_wave);
+        }
         grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
     }
-    publish_counters<COUNT>(w.ctl, cnt);
+    publish_counters<COUNT>(w, cnt);
 }
 
 // ------------------------------------------------------ reductions
+// Block-wide sum of one uint64 per thread (all threads get the total).
+__device__ __forceinline__ uint64_t block_sum64(uint64_t x, uint64_t *s_red) {
+    x = wave_sum64(x);
+    if (lane_id() == 0) s_red[threadIdx.x / GGA_WAVE] = x;
+    __syncthreads();
+    uint64_t t = 0;
+    for (uint32_t i = 0; i < WAVES; i++) t += s_red[i];
+    __syncthreads();
+    return t;
+}
+
+// Pass 1 (REACH_BLOCKS workgroups): per-workgroup reached / m_reach partials
+// into cnt slots [0, REACH_BLOCKS) word 3 and [REACH_BLOCKS, 2 REACH_BLOCKS)
+// word 3 -- plain stores, no shared counter.
+constexpr uint32_t REACH_BLOCKS = GGA_CNT_SLOTS / 2;
 __global__ __launch_bounds__(BLOCK) void k_reach(gga_dev_graph g, gga_dev_work w) {
+    __shared__ uint64_t s_red[WAVES];
     uint64_t reached = 0, mr = 0;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
          v += (uint64_t)gridDim.x * blockDim.x) {
@@ -330,12 +549,31 @@ __global__ __launch_bounds__(BLOCK) void k_reach(gga_dev_graph g, gga_dev_work w
             mr += g.row[v + 1] - g.row[v];
         }
     }
-    reached = wave_sum64(reached);
-    mr = wave_sum64(mr);
-    if (lane_id() == 0) {
-        uint64_t *red = reinterpret_cast<uint64_t *>(w.ctl + GGA_CTL_RED);
-        if (reached) atomic_add_u64(&red[0], reached);
-        if (mr) atomic_add_u64(&red[1], mr);
+    reached = block_sum64(reached, s_red);
+    mr = block_sum64(mr, s_red);
+    if (threadIdx.x == 0) {
+        w.cnt[4u * blockIdx.x + 3u] = reached;
+        w.cnt[4u * (REACH_BLOCKS + blockIdx.x) + 3u] = mr;
+    }
+}
+
+// Pass 2 (one workgroup): fold the partials into ctl[RED] and the striped work
+// counters into ctl[CNT].
+__global__ __launch_bounds__(BLOCK) void k_reach_fold(gga_dev_work w) {
+    __shared__ uint64_t s_red[WAVES];
+    uint64_t s[5] = {0, 0, 0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < GGA_CNT_SLOTS; i += blockDim.x) {
+        for (uint32_t k = 0; k < 3; k++) s[k] += w.cnt[4u * i + k];
+        s[i < REACH_BLOCKS ? 3 : 4] += w.cnt[4u * i + 3u];
+    }
+    uint64_t *cnt = reinterpret_cast<uint64_t *>(w.ctl + GGA_CTL_CNT);
+    uint64_t *red = reinterpret_cast<uint64_t *>(w.ctl + GGA_CTL_RED);
+    for (uint32_t k = 0; k < 5; k++) {
+        const uint64_t t = block_sum64(s[k], s_red);
+        if (threadIdx.x == 0) {
+            if (k < 3) cnt[k] = t;
+            else red[k - 3] = t;
+        }
     }
 }
 
@@ -438,6 +676,15 @@ extern "C" int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, int 
     return launch_status();
 }
 
+extern "C" int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t step, hipStream_t s) {
+    const uint32_t ntiles = GGA_TILES(g.n);
+    const uint32_t grid = ntiles < 2048u ? (ntiles ? ntiles : 1u) : 2048u;
+    k_compact_count<<<grid, BLOCK, 0, s>>>(g, w, step);
+    k_compact_scan<<<1, BLOCK, 0, s>>>(g, w, step);
+    k_compact_write<<<grid, BLOCK, 0, s>>>(g, w, step);
+    return launch_status();
+}
+
 template <bool BFS, bool COUNT>
 static const void *persistent_fn() {
     return reinterpret_cast<const void *>(&k_persistent<BFS, COUNT>);
@@ -483,7 +730,9 @@ extern "C" int ggak_persistent(gga_dev_graph g, gga_dev_work w, int bfs, int cou
 }
 
 extern "C" int ggak_reach(gga_dev_graph g, gga_dev_work w, hipStream_t s) {
-    k_reach<<<grid_for(g.n, 4096), BLOCK, 0, s>>>(g, w);
+    // REACH_BLOCKS partial slots are always all written (idle workgroups store 0)
+    k_reach<<<REACH_BLOCKS, BLOCK, 0, s>>>(g, w);
+    k_reach_fold<<<1, BLOCK, 0, s>>>(w);
     return launch_status();
 }
 

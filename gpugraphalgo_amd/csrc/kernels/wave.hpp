@@ -103,4 +103,10 @@ __device__ __forceinline__ uint32_t load_relaxed(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Read-once streams (edge arrays, chunk descriptors, worklists): non-temporal
+// loads so that they do not evict the randomly gathered dist[] lines from L2.
+__device__ __forceinline__ uint32_t stream_load(const uint32_t *p) {
+    return __builtin_nontemporal_load(p);
+}
+
 }  // namespace gga

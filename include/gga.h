@@ -162,6 +162,9 @@ typedef struct gga_opts {
     uint32_t use_graph;   /* WORKLIST: replay the sync_every steps as one captured hipGraph */
     uint32_t count;       /* 1 -> collect edges_inspected / chunks_expanded / relaxations */
     uint32_t grid;        /* PERSISTENT / WORKLIST grid in workgroups, 0 -> occupancy bound */
+    uint32_t dense_min;   /* WORKLIST / PERSISTENT: a step whose list holds >= dense_min chunks
+                             records the next frontier as step stamps + compaction instead of
+                             queue pushes; 0 -> max(4096, n / 64); 0xFFFFFFFF -> never */
 } gga_opts;
 /* Fills defaults (all zero = defaults). */
 void gga_opts_init(gga_opts *o);
@@ -172,6 +175,7 @@ typedef struct gga_stats {
     float    time_ms;         /* device time, dist init -> convergence (reference `time`, bfs_sssp.cu:347-469) */
     float    ktime_ms;        /* relax-kernel device time (reference `ktime`) */
     float    sktime_ms;       /* child-sync time: always 0 (no child copies; reference `sktime`) */
+    float    ctime_ms;        /* WORKLIST: list-compaction kernel time (after flag steps) */
     int32_t  mode;            /* mode actually used */
     uint64_t edges_inspected; /* I_e  (opts.count) */
     uint64_t chunks_expanded; /* I_v  (opts.count) */

@@ -19,6 +19,8 @@ pytestmark = pytest.mark.gpu
 
 MODES = ["topo", "worklist", "persistent"]
 INF = gga.INF
+# frontier representation: default switch, every step a flag step, every step a queue step
+DENSE = {"auto": 0, "flags": 1, "queue": 0xFFFFFFFF}
 
 
 def announce(msg):
@@ -41,12 +43,13 @@ def test_golden_vectors_all_modes(gpu, golden):
                 else:
                     dg.split()
                 for e in (e for e in index if e["graph"] == name):
-                    for mode in MODES:
-                        announce(f"  {e['key']} mda={mda} mode={mode}")
+                    for mode, rep in [("topo", "auto")] + [(m, r) for m in MODES[1:] for r in DENSE]:
+                        announce(f"  {e['key']} mda={mda} mode={mode} frontier={rep}")
                         run = dg.bfs if e["algo"] == "bfs" else dg.sssp
-                        d, st = run(e["src"], mode=mode, inf=e["inf"], count=True)
-                        np.testing.assert_array_equal(d, arrays[e["key"]],
-                                                      err_msg=f"{e['key']} mda={mda} {mode}")
+                        d, st = run(e["src"], mode=mode, inf=e["inf"], count=True,
+                                    dense_min=DENSE[rep])
+                        np.testing.assert_array_equal(
+                            d, arrays[e["key"]], err_msg=f"{e['key']} mda={mda} {mode} {rep}")
                         assert st["reached"] == int((arrays[e["key"]] != e["inf"]).sum())
 
 
@@ -125,16 +128,20 @@ def test_rmat16_sssp_threshold_sweep(gpu, T):
     g = gga.gen_rmat(16, 16, seed=7, wlo=1, whi=100)
     src = g.maxdeg()[1]
     want = oracle.dijkstra(g.row_ptr, g.dst, g.w, src)
+    want_bfs = oracle.bfs_queue(g.row_ptr, g.dst, src)
     serial, _, _ = oracle.serial_sp(g.row_ptr, g.dst, g.w, src)
     np.testing.assert_array_equal(serial, want)
     with gga.DeviceGraph(g) as dg:
         dg.split(mda=T)
         for mode in MODES:
-            announce(f"rmat16 sssp T={T} mode={mode}")
-            d, st = dg.sssp(src, mode=mode, count=True)
-            np.testing.assert_array_equal(d, want)
-            assert st["m_reach"] == oracle.m_reach(g.row_ptr, want)
-            assert st["edges_inspected"] >= st["m_reach"]
+            for rep in (DENSE if mode != "topo" else ["auto"]):
+                announce(f"rmat16 sssp T={T} mode={mode} frontier={rep}")
+                d, st = dg.sssp(src, mode=mode, count=True, dense_min=DENSE[rep])
+                np.testing.assert_array_equal(d, want)
+                assert st["m_reach"] == oracle.m_reach(g.row_ptr, want)
+                assert st["edges_inspected"] >= st["m_reach"]
+                d, st = dg.bfs(src, mode=mode, dense_min=DENSE[rep])
+                np.testing.assert_array_equal(d, want_bfs)
 
 
 def test_grid_bfs_sssp(gpu):
@@ -144,11 +151,12 @@ def test_grid_bfs_sssp(gpu):
     with gga.DeviceGraph(g) as dg:
         dg.split(mda=4)
         for mode in MODES:
-            announce(f"grid200x300 mode={mode}")
-            d, _ = dg.sssp(0, mode=mode)
-            np.testing.assert_array_equal(d, want_s)
-            d, _ = dg.bfs(0, mode=mode)
-            np.testing.assert_array_equal(d, want_b)
+            for rep in (DENSE if mode != "topo" else ["auto"]):
+                announce(f"grid200x300 mode={mode} frontier={rep}")
+                d, _ = dg.sssp(0, mode=mode, dense_min=DENSE[rep])
+                np.testing.assert_array_equal(d, want_s)
+                d, _ = dg.bfs(0, mode=mode, dense_min=DENSE[rep])
+                np.testing.assert_array_equal(d, want_b)
 
 
 def test_result_same_alone_and_after_other_queries(gpu):
