@@ -4,8 +4,9 @@
 Workload (the single-GPU configuration the metric is quoted on): R-MAT
 2^22 vertices, 16 * 2^22 generated arcs (a, b, c, d = .57, .19, .19, .05,
 seed 7, duplicates and self-loops removed), uint32 weights uniform in
-[1, 100], source = the highest-degree vertex (lowest id on ties).  The split
-threshold T is swept over {32, 64, 128, 256, 1024}; the best T gives `value`.
+[1, 100], source = the highest-degree vertex (lowest id on ties).  Split
+threshold T = 64 by default (the best of {32, 64, 128, 256, 1024} on MI355X;
+--sweep times all five and reports the best).
 
 A step is one full SSSP query (distance init -> convergence) with the graph
 already resident in HBM and the result left in HBM.  value = GTEPS =
@@ -19,7 +20,7 @@ reference's serial SSSP (graph.h:115-148), one thread, full run on the same
 graph and source.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode auto|worklist|persistent|topo]
-                  [--mda T] [--no-cpu] [--quick]
+                  [--mda T | --sweep] [--no-cpu] [--quick]
 """
 from __future__ import annotations
 
@@ -36,6 +37,8 @@ if ROOT not in sys.path:
 METRIC = "GTEPS + % HBM roofline (1 GPU); served BFS/SSSP queries/s with p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 SWEEP = (32, 64, 128, 256, 1024)
+DEFAULT_T = 64  # best of SWEEP on MI355X (profiles/sweep_rmat22_sssp.txt)
+TRAFFIC = os.path.join(ROOT, "profiles", "k_wl_step_rmat22_sssp_traffic.txt")
 
 
 def log(msg: str) -> None:
@@ -108,7 +111,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="auto")
-    ap.add_argument("--mda", type=int, default=0, help="one threshold instead of the sweep")
+    ap.add_argument("--mda", type=int, default=DEFAULT_T, help="split threshold T")
+    ap.add_argument("--sweep", action="store_true",
+                    help="time every T in {32, 64, 128, 256, 1024} and report the best")
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=7)
@@ -140,7 +145,7 @@ def main() -> int:
     g = gga.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed, wlo=1, whi=100)
     maxdeg, src = g.maxdeg()
     log(f"n={g.n} m={g.m} maxdeg={maxdeg} src={src}")
-    sweep = (args.mda,) if args.mda else ((64,) if args.quick else SWEEP)
+    sweep = SWEEP if args.sweep else (args.mda,)
     results = {}
     with gga.DeviceGraph(g) as dg:
         for T in sweep:
@@ -164,6 +169,10 @@ def main() -> int:
         assert np.array_equal(d_cpu, d_gpu), "GPU distances differ from the CPU reference"
         log(f"cpu: {cpu['value']:.4f} GTEPS; GPU result bit-exact")
 
+    traffic = None
+    if os.path.exists(TRAFFIC) and r["mode"] == 1:
+        from gpugraphalgo_amd.profiling import read_summary
+        traffic = float(read_summary(TRAFFIC)["traffic_bytes_per_dispatch"])
     mode_name = {0: "topo", 1: "worklist", 2: "persistent"}.get(r["mode"], str(r["mode"]))
     kname = {0: "k_topo_step", 1: "k_wl_step", 2: "k_persistent"}.get(r["mode"], "?")
     line = {
@@ -195,7 +204,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": round(r["bytes_per_query"] / max(r["launches"], 1)),
             "launches_per_step": r["launches"],
             "avg_launch_ms": round(r["ktime_ms"] / max(r["launches"], 1), 5),

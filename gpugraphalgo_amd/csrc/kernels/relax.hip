@@ -450,9 +450,13 @@ __global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_de
 // barrier instead of nblocks.  Every wait is bounded by `spin_ticks` of the
 // 100 MHz wall clock; on expiry the error bit is set and every workgroup
 // leaves the step loop at its next check.
+// Agent-scope fences cost an L2 write-back / invalidate on gfx950, so only
+// thread 0 fences: its acq_rel arrival publishes the workgroup's writes (the
+// workgroup barrier orders them first), it polls with relaxed coherent loads
+// (an acquire load per poll would invalidate L2 every iteration), and one
+// acquire fence after the release covers the whole workgroup (shared L1/L2).
 __device__ __forceinline__ void grid_barrier(uint32_t *ctl, uint32_t nblocks, uint32_t &gen,
                                              uint64_t spin_ticks) {
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t ngroups = nblocks < GGA_BAR_GROUPS ? nblocks : GGA_BAR_GROUPS;
@@ -472,19 +476,19 @@ __device__ __forceinline__ void grid_barrier(uint32_t *ctl, uint32_t nblocks, ui
             __hip_atomic_store(&ctl[GGA_CTL_BARGEN], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(&ctl[GGA_CTL_BARGEN], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(1);
+            while (__hip_atomic_load(&ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(2);
                 if (wall_clock64() - t0 > spin_ticks) {
                     __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_SPIN, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         gen += 1u;
     }
     __syncthreads();
-    __threadfence();
 }
 
 template <bool BFS, bool COUNT>
@@ -709,8 +713,10 @@ extern "C" uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count)
     // A grid barrier needs every workgroup resident at once.  On gfx950 the
     // occupancy query (7-8 per CU for these kernels) overstates what the
     // hardware co-schedules (6 per CU measured for the persistent kernel), and
-    // the cooperative launch does not reject the excess; 4 per CU keeps a margin.
-    if (persistent && per > 4) per = 4;
+    // the cooperative launch does not reject the excess.  Barrier cost grows
+    // with the number of workgroups (8 us at 256, 25 us at 1024 measured), so
+    // the persistent grid is one workgroup per CU.
+    if (persistent && per > 1) per = 1;
     return (uint32_t)cus * (uint32_t)per;
 }
 
