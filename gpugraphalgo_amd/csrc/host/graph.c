@@ -365,6 +365,7 @@ int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets,
     free_split(g);
     g->have_result = 0;
     tmp_words = ggak_split_tmp_words(g->n);
+    if (tmp_words < GGA_MAXCHUNK_SLOTS) tmp_words = GGA_MAXCHUNK_SLOTS;
     if ((rc = dmalloc(g, (void **)&d_tmp, 4ull * tmp_words))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_cstart, 4ull * ((uint64_t)g->n + 1)))) goto fail;
     HIP_TRY(hipEventRecord(g->ev[0], g->stream));
@@ -376,11 +377,13 @@ int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets,
     if ((rc = dmalloc(g, (void **)&g->d_cown, 4ull * g->C))) goto fail;
     {
         uint32_t *d_max = d_tmp;  /* scan scratch is free again */
-        K_TRY(ggak_fill_u32(d_max, 0u, 1, g->stream));
+        K_TRY(ggak_fill_u32(d_max, 0u, GGA_MAXCHUNK_SLOTS, g->stream));
         K_TRY(ggak_split_fill(g->d_row, g->d_cstart, g->n, g->m, g->d_cbeg, g->d_cown, d_max, g->stream));
         HIP_TRY(hipEventRecord(g->ev[1], g->stream));
-        HIP_TRY(hipMemcpyAsync(g->h_pin, d_max, 4, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->h_pin, d_max, 4ull * GGA_MAXCHUNK_SLOTS, hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
+        for (uint32_t i = 1; i < GGA_MAXCHUNK_SLOTS; i++)
+            if (g->h_pin[i] > g->h_pin[0]) g->h_pin[0] = g->h_pin[i];
     }
     dfree(g, (void **)&d_tmp, 4ull * tmp_words);
     g->mda = T;

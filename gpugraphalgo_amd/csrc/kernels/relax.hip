@@ -311,8 +311,17 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
 }
 
 // ---- list compaction after a flag step: chunks of every vertex stamped `it`
+// go to one output list.
+struct ListOut {
+    const uint32_t *cstart;  // first chunk of each vertex (n + 1)
+    uint32_t *q;             // output list
+    uint32_t qcap;           // its capacity
+    uint32_t *tiles;         // per-tile counts, then offsets (GGA_TILES(n) entries)
+    uint32_t *tail;          // list length
+};
+
 // Per-thread view of one tile: CITEMS vertices v0 + i * BLOCK + tid (coalesced).
-__device__ __forceinline__ uint32_t tile_items(const gga_dev_graph &g, const gga_dev_work &w,
+__device__ __forceinline__ uint32_t tile_items(const ListOut &o, const gga_dev_work &w, uint32_t n,
                                                uint32_t t, uint32_t it, uint32_t *cs, uint32_t *nch) {
     const uint64_t v0 = (uint64_t)t * GGA_TILE;
     uint32_t mine = 0;
@@ -321,75 +330,76 @@ __device__ __forceinline__ uint32_t tile_items(const gga_dev_graph &g, const gga
         const uint64_t v = v0 + (uint64_t)i * BLOCK + threadIdx.x;
         cs[i] = 0;
         nch[i] = 0;
-        if (v < g.n && w.mark[v] == it) {
-            cs[i] = g.cstart[v];
-            nch[i] = g.cstart[v + 1] - cs[i];
+        if (v < n && w.mark[v] == it) {
+            cs[i] = o.cstart[v];
+            nch[i] = o.cstart[v + 1] - cs[i];
         }
         mine += nch[i];
     }
     return mine;
 }
 
-// pass 1: tiles[t] = number of chunk ids tile t contributes
-__device__ __forceinline__ void compact_count(const gga_dev_graph &g, const gga_dev_work &w, uint32_t it,
-                                              uint32_t tile0, uint32_t tstride, uint32_t *s_wave) {
-    const uint32_t ntiles = GGA_TILES(g.n);
-    for (uint32_t t = tile0; t < ntiles; t += tstride) {
-        uint32_t cs[CITEMS], nch[CITEMS], excl;
-        const uint32_t tot = block_excl_scan(tile_items(g, w, t, it, cs, nch), excl, s_wave);
-        if (threadIdx.x == 0) w.tiles[t] = tot;
-    }
+// pass 1, one tile: tiles[t] = number of chunk ids tile t contributes
+__device__ __forceinline__ void compact_count_tile(const ListOut &o, const gga_dev_work &w, uint32_t n,
+                                                   uint32_t t, uint32_t it, uint32_t *s_wave) {
+    uint32_t cs[CITEMS], nch[CITEMS], excl;
+    const uint32_t tot = block_excl_scan(tile_items(o, w, n, t, it, cs, nch), excl, s_wave);
+    if (threadIdx.x == 0) o.tiles[t] = tot;
 }
 
 // pass 2 (one workgroup): exclusive scan of tiles[]; the total is the new list length
-__device__ __forceinline__ void compact_scan(const gga_dev_graph &g, const gga_dev_work &w,
-                                             uint32_t step, uint32_t *s_wave) {
-    const uint32_t ntiles = GGA_TILES(g.n);
+__device__ __forceinline__ void compact_scan(const ListOut &o, uint32_t n, uint32_t *s_wave) {
+    const uint32_t ntiles = GGA_TILES(n);
     uint32_t carry = 0;
     for (uint32_t base = 0; base < ntiles; base += BLOCK) {
         const uint32_t i = base + threadIdx.x;
-        const uint32_t x = i < ntiles ? w.tiles[i] : 0u;
+        const uint32_t x = i < ntiles ? o.tiles[i] : 0u;
         uint32_t excl;
         const uint32_t tot = block_excl_scan(x, excl, s_wave);
-        if (i < ntiles) w.tiles[i] = carry + excl;
+        if (i < ntiles) o.tiles[i] = carry + excl;
         carry += tot;
     }
-    if (threadIdx.x == 0) w.ctl[GGA_CTL_TAIL((step + 1u) % 3u)] = carry;
+    if (threadIdx.x == 0) *o.tail = carry;
 }
 
-// pass 3: write.  A vertex with few chunks is written by its thread; a hub's
-// hundreds of chunk ids are written by the whole wave, 64 per store round.
-__device__ __forceinline__ void compact_write(const gga_dev_graph &g, const gga_dev_work &w,
-                                              uint32_t step, uint32_t it, uint32_t tile0,
-                                              uint32_t tstride, uint32_t *s_wave) {
-    uint32_t *qout = (step & 1u) ? w.q0 : w.q1;
-    const uint32_t ntiles = GGA_TILES(g.n);
-    for (uint32_t t = tile0; t < ntiles; t += tstride) {
-        uint32_t cs[CITEMS], nch[CITEMS], excl;
-        const uint32_t mine = tile_items(g, w, t, it, cs, nch);
-        block_excl_scan(mine, excl, s_wave);
-        uint32_t pos = w.tiles[t] + excl;
-        uint32_t bpos[CITEMS];
+// pass 3, one tile: write.  A vertex with few chunks is written by its thread;
+// a hub's hundreds of chunk ids are written by the whole wave, 64 per round.
+__device__ __forceinline__ void compact_write_tile(const ListOut &o, const gga_dev_work &w, uint32_t n,
+                                                   uint32_t t, uint32_t it, uint32_t *s_wave) {
+    uint32_t cs[CITEMS], nch[CITEMS], excl;
+    const uint32_t mine = tile_items(o, w, n, t, it, cs, nch);
+    block_excl_scan(mine, excl, s_wave);
+    uint32_t pos = o.tiles[t] + excl;
+    uint32_t bpos[CITEMS];
 #pragma unroll
-        for (uint32_t i = 0; i < CITEMS; i++) {
-            bpos[i] = pos;
-            if (nch[i] <= CSMALL)
-                for (uint32_t k = 0; k < nch[i]; k++) queue_store(qout, pos + k, cs[i] + k, w.qcap, w.ctl);
-            pos += nch[i];
-        }
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        bpos[i] = pos;
+        if (nch[i] <= CSMALL)
+            for (uint32_t k = 0; k < nch[i]; k++) queue_store(o.q, pos + k, cs[i] + k, o.qcap, w.ctl);
+        pos += nch[i];
+    }
 #pragma unroll
-        for (uint32_t i = 0; i < CITEMS; i++) {
-            uint64_t big = __ballot(nch[i] > CSMALL);
-            while (big) {
-                const uint32_t j = (uint32_t)__builtin_ctzll(big);
-                big &= big - 1;
-                const uint32_t p = wave_bcast(bpos[i], j), c0 = wave_bcast(cs[i], j);
-                const uint32_t cnt = wave_bcast(nch[i], j);
-                for (uint32_t k = lane_id(); k < cnt; k += GGA_WAVE)
-                    queue_store(qout, p + k, c0 + k, w.qcap, w.ctl);
-            }
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        uint64_t big = __ballot(nch[i] > CSMALL);
+        while (big) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(big);
+            big &= big - 1;
+            const uint32_t p = wave_bcast(bpos[i], j), c0 = wave_bcast(cs[i], j);
+            const uint32_t cnt = wave_bcast(nch[i], j);
+            for (uint32_t k = lane_id(); k < cnt; k += GGA_WAVE) queue_store(o.q, p + k, c0 + k, o.qcap, w.ctl);
         }
     }
+}
+
+// The single list of the plain layout for step `step`.
+__device__ __forceinline__ ListOut list_out(const gga_dev_graph &g, const gga_dev_work &w, uint32_t step) {
+    ListOut o;
+    o.cstart = g.cstart;
+    o.q = (step & 1u) ? w.q0 : w.q1;
+    o.qcap = w.qcap;
+    o.tiles = w.tiles;
+    o.tail = &w.ctl[GGA_CTL_TAIL((step + 1u) % 3u)];
+    return o;
 }
 
 // Per-step bookkeeping done by one thread: next slot's step number, clear the
@@ -425,21 +435,25 @@ __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_de
     __shared__ uint32_t s_wave[WAVES];
     const uint32_t nq = step_len(w, step);
     if (nq == 0 || !flag_step(w, nq)) return;
-    compact_count(g, w, w.ctl[GGA_CTL_ITER(step % 3u)], blockIdx.x, gridDim.x, s_wave);
+    const ListOut o = list_out(g, w, step);
+    const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, w, g.n, t, it, s_wave);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_compact_scan(gga_dev_graph g, gga_dev_work w, uint32_t step) {
     __shared__ uint32_t s_wave[WAVES];
     const uint32_t nq = step_len(w, step);
     if (nq == 0 || !flag_step(w, nq)) return;
-    compact_scan(g, w, step, s_wave);
+    compact_scan(list_out(g, w, step), g.n, s_wave);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_dev_work w, uint32_t step) {
     __shared__ uint32_t s_wave[WAVES];
     const uint32_t nq = step_len(w, step);
     if (nq == 0 || !flag_step(w, nq)) return;
-    compact_write(g, w, step, w.ctl[GGA_CTL_ITER(step % 3u)], blockIdx.x, gridDim.x, s_wave);
+    const ListOut o = list_out(g, w, step);
+    const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, w, g.n, t, it, s_wave);
 }
 
 // ------------------------------------------------------ persistent kernel
@@ -515,12 +529,15 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
         if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
         worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
         if (flag_step(w, nq)) {
+            const ListOut o = list_out(g, w, step);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
-            compact_count(g, w, it, blockIdx.x, gridDim.x, s_wave);
+            for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
+                compact_count_tile(o, w, g.n, t, it, s_wave);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
-            if (blockIdx.x == 0) compact_scan(g, w, step, s_wave);
+            if (blockIdx.x == 0) compact_scan(o, g.n, s_wave);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
-            compact_write(g, w, step, it, blockIdx.x, gridDim.x, s_wave);
+            for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
+                compact_write_tile(o, w, g.n, t, it, s_wave);
         }
         grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
     }
@@ -667,17 +684,29 @@ extern "C" int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int coun
     return launch_status();
 }
 
+// Kernel variants by (bfs, count).
+template <bool BFS, bool COUNT>
+static const void *wl_fn() {
+    return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT>);
+}
+template <bool BFS, bool COUNT>
+static const void *persistent_fn() {
+    return reinterpret_cast<const void *>(&k_persistent<BFS, COUNT>);
+}
+static const void *pick_kernel(int persistent, int bfs, int count) {
+    static const void *const wl[4] = {wl_fn<false, false>(), wl_fn<false, true>(), wl_fn<true, false>(),
+                                      wl_fn<true, true>()};
+    static const void *const pk[4] = {persistent_fn<false, false>(), persistent_fn<false, true>(),
+                                      persistent_fn<true, false>(), persistent_fn<true, true>()};
+    const int i = (bfs ? 2 : 0) | (count ? 1 : 0);
+    return persistent ? pk[i] : wl[i];
+}
+
 extern "C" int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, int bfs, int count,
                             uint32_t grid, hipStream_t s) {
     if (grid == 0) return (int)hipErrorInvalidValue;
-    if (bfs) {
-        if (count) k_wl_step<true, true><<<grid, BLOCK, 0, s>>>(g, w, step);
-        else k_wl_step<true, false><<<grid, BLOCK, 0, s>>>(g, w, step);
-    } else {
-        if (count) k_wl_step<false, true><<<grid, BLOCK, 0, s>>>(g, w, step);
-        else k_wl_step<false, false><<<grid, BLOCK, 0, s>>>(g, w, step);
-    }
-    return launch_status();
+    void *args[] = {&g, &w, &step};
+    return (int)hipLaunchKernel(pick_kernel(0, bfs, count), dim3(grid), dim3(BLOCK), args, 0, s);
 }
 
 extern "C" int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t step, hipStream_t s) {
@@ -689,25 +718,10 @@ extern "C" int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t step, hipS
     return launch_status();
 }
 
-template <bool BFS, bool COUNT>
-static const void *persistent_fn() {
-    return reinterpret_cast<const void *>(&k_persistent<BFS, COUNT>);
-}
-
-static const void *pick_persistent(int bfs, int count) {
-    if (bfs) return count ? persistent_fn<true, true>() : persistent_fn<true, false>();
-    return count ? persistent_fn<false, true>() : persistent_fn<false, false>();
-}
-
 extern "C" uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count) {
     int cus = 0, per = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    const void *fn;
-    if (persistent) fn = pick_persistent(bfs, count);
-    else if (bfs) fn = count ? reinterpret_cast<const void *>(&k_wl_step<true, true>)
-                             : reinterpret_cast<const void *>(&k_wl_step<true, false>);
-    else fn = count ? reinterpret_cast<const void *>(&k_wl_step<false, true>)
-                    : reinterpret_cast<const void *>(&k_wl_step<false, false>);
+    const void *fn = pick_kernel(persistent, bfs, count);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, 0) != hipSuccess) return 0;
     if (per <= 0) return 0;
     // A grid barrier needs every workgroup resident at once.  On gfx950 the
@@ -729,8 +743,8 @@ extern "C" int ggak_persistent(gga_dev_graph g, gga_dev_work w, int bfs, int cou
         khz = 100000;  // 100 MHz
     uint64_t spin_ticks = (uint64_t)spin_limit_ms * (uint64_t)khz;
     void *args[] = {&g, &w, &max_steps, &spin_ticks};
-    hipError_t e = hipLaunchCooperativeKernel(pick_persistent(bfs, count), dim3(grid), dim3(BLOCK), args,
-                                              0, s);
+    hipError_t e = hipLaunchCooperativeKernel(pick_kernel(1, bfs, count), dim3(grid), dim3(BLOCK),
+                                              args, 0, s);
     if (e != hipSuccess) return (int)e;
     return launch_status();
 }

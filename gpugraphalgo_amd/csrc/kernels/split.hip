@@ -142,6 +142,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_split_scan_down(const uint32_t
 // Chunk descriptors: each wave covers 64 consecutive vertices and writes their
 // chunks (a contiguous id range) with all 64 lanes, so a hub's thousands of
 // chunks cost its wave a few coalesced rounds rather than one lane a loop.
+// owner_mod != 0: item i is vertex i % owner_mod (the part-major table of the
+// XCD-partitioned layout, where items are (part, vertex) pairs).
 __global__ __launch_bounds__(256) void k_split_fill(const uint32_t *__restrict__ row,
                                                     const uint32_t *__restrict__ cstart, uint32_t n,
                                                     uint32_t m, uint32_t *__restrict__ cbeg,
@@ -171,9 +173,17 @@ __global__ __launch_bounds__(256) void k_split_fill(const uint32_t *__restrict__
             cown[first + k] = (uint32_t)vbase + j;
         }
     }
-    uint32_t big = local;  // largest chunk of this vertex is the first one
-    uint64_t mx = wave_max64(big);
-    if (l == 0 && mx) atomicMax(maxchunk, (uint32_t)mx);
+    // largest chunk of a vertex is its first one; reduce per workgroup and
+    // stripe over 64 slots (one hot counter would serialize ~16 ns per atomic)
+    __shared__ uint32_t s_max[256 / GGA_WAVE];
+    const uint64_t mx = wave_max64(local);
+    if (l == 0) s_max[threadIdx.x / GGA_WAVE] = (uint32_t)mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = 0;
+        for (uint32_t i = 0; i < blockDim.x / GGA_WAVE; i++) b = s_max[i] > b ? s_max[i] : b;
+        if (b) atomicMax(&maxchunk[blockIdx.x & (GGA_MAXCHUNK_SLOTS - 1)], b);
+    }
     if (v == (uint64_t)n - 1) cbeg[cs + nch] = m;
 }
 
