@@ -103,6 +103,14 @@ SIGNATURES = [
     ("gga_write_dist", ctypes.c_int, [ctypes.c_char_p, vp, u32]),
     ("gga_write_result_row", ctypes.c_int, [ctypes.c_char_p, P(ResultRow)]),
     ("gga_graph_name", None, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+    ("gga_ms_create", ctypes.c_int, [vp, u32, P(vp)]),
+    ("gga_ms_free", ctypes.c_int, [vp]),
+    ("gga_ms_start", ctypes.c_int, [vp, u32, vp, u32]),
+    ("gga_ms_run", ctypes.c_int, [vp, u32, P(Opts), P(u32)]),
+    ("gga_ms_done", u32, [vp]),
+    ("gga_ms_fetch", ctypes.c_int, [vp, u32, vp, ctypes.c_int]),
+    ("gga_ms_stats", ctypes.c_int, [vp, P(Stats)]),
+    ("gga_multi_sp", ctypes.c_int, [vp, vp, u32, u32, vp, P(Opts), P(Stats)]),
 ]
 
 _LIB = None

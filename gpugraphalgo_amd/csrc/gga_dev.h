@@ -48,6 +48,34 @@ typedef struct gga_dev_work {
     uint32_t pad;
 } gga_dev_work;
 
+/* Multi-source engine: GGA_MS_LANES independent traversals share every step.
+ *   D[n * 32]   distances, vertex-major: the 32 lanes of a vertex are one
+ *               128-byte line, so relaxing an edge for every lane touches one
+ *               line of dist instead of 32
+ *   M[2][n]     lane masks: M[s & 1][v] = lanes whose distance at v dropped in
+ *               step s - 1 (step s expands exactly those lanes from v)
+ *   q0, q1      chunk lists of the vertices with a non-zero mask
+ *   tiles[2 * GGA_TILES(n)]  compaction counts / offsets, then per-tile lane ORs */
+#ifndef GGA_MS_LANES
+#define GGA_MS_LANES 32   /* same value as include/gga.h */
+#endif
+typedef struct gga_dev_ms {
+    uint32_t *D;
+    uint32_t *M0;
+    uint32_t *M1;
+    uint32_t *q0;
+    uint32_t *q1;
+    uint32_t *ctl;       /* GGA_CTL_WORDS words, same layout as gga_dev_work.ctl */
+    uint32_t *tiles;
+    uint64_t *cnt;       /* GGA_CNT_SLOTS x 4 work counters */
+    uint32_t inf;
+    uint32_t qcap;
+    uint32_t unit_lanes; /* lanes that relax with weight 1 (BFS levels) */
+    uint32_t pad;
+} gga_dev_ms;
+typedef struct gga_ms_sources { uint32_t v[GGA_MS_LANES]; } gga_ms_sources;
+typedef struct gga_ms_outputs { uint32_t *p[GGA_MS_LANES]; } gga_ms_outputs;
+
 /* Device-scope atomics are performed past the per-XCD L2 and same-address
  * ones serialize (about 16 ns each on MI355X), so nothing hot shares one
  * counter: work counters are striped over GGA_CNT_SLOTS addresses. */
@@ -71,6 +99,9 @@ typedef struct gga_dev_work {
 #define GGA_BAR_GROUPS    32u                           /* grid barrier: first-level groups */
 #define GGA_CTL_BARG(i)   ((12u + (i)) * GGA_CTL_LINE)  /* grid barrier: arrivals within group i */
 #define GGA_CTL_WORDS     ((12u + GGA_BAR_GROUPS) * GGA_CTL_LINE)
+
+/* multi-source engine (kernels/ms.hip): lane mask of a ring slot's list */
+#define GGA_CTL_LANES(s)  (7u * GGA_CTL_LINE + 8u + (s))
 
 #define GGA_DEVERR_SPIN   1u   /* barrier spin limit hit */
 #define GGA_DEVERR_QFULL  2u   /* worklist overflow (cannot happen with de-duplication) */

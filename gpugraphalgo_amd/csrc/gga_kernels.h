@@ -54,6 +54,27 @@ int ggak_reach(gga_dev_graph g, gga_dev_work w, hipStream_t s);
 int ggak_verify(gga_dev_graph g, gga_dev_work w, uint32_t src, int bfs, uint8_t *d_tight,
                 hipStream_t s);
 
+/* ---- multi-source engine (ms.hip), 32 lanes; `step` is the engine's step number */
+int ggak_ms_clear(gga_dev_ms ms, hipStream_t s);
+/* Restart `lanes` at src.v[lane] and list them for step `step`. */
+int ggak_ms_start(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, gga_ms_sources src, uint32_t step,
+                  hipStream_t s);
+/* One step: relax + list rebuild; ctl[TAIL((step+1)%3)] / ctl[LANES((step+1)%3)] give
+ * the next list length and its live lanes. */
+int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int count, uint32_t grid, hipStream_t s);
+uint32_t ggak_ms_grid(int device, int persistent, int count);
+/* Steps step0, step0 + 1, ... in one cooperative launch until the list is empty
+ * or max_steps ran; ctl[STEPS] grows by the number of steps run. */
+int ggak_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0, uint32_t max_steps, int count,
+                       uint32_t grid, uint32_t spin_limit_ms, hipStream_t s);
+/* out.p[lane][v] = D[v][lane] for the lanes in `lanes` (device pointers). */
+int ggak_ms_harvest(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, gga_ms_outputs out, hipStream_t s);
+/* Reached (vertex, lane) pairs of `lanes` and their out-degree sum -> cnt word 3
+ * partials; ggak_ms_fold then leaves them in ctl[RED], ctl[RED + 2] (uint64). */
+int ggak_ms_reach(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, hipStream_t s);
+/* Striped work counters -> ctl[CNT] (3 x uint64), reach partials -> ctl[RED]. */
+int ggak_ms_fold(gga_dev_ms ms, hipStream_t s);
+
 #ifdef __cplusplus
 }
 #endif

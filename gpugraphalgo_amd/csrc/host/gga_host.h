@@ -6,7 +6,22 @@
 
 #include "../../../include/gga.h"
 
+#include "../gga_kernels.h"
+
 #define GGA_ERRBUF 512
+
+/* What the engines need of a resident graph (graph.c); splits it with the
+ * default threshold first if it has no chunk table. */
+typedef struct gga_graph_view_t {
+    gga_dev_graph dg;
+    hipStream_t stream;
+    int device;
+    uint32_t split_gen;
+    uint32_t maxdeg;
+} gga_graph_view_t;
+int gga_graph_view(gga_graph *g, gga_graph_view_t *v);
+/* The graph's cached engine for gga_multi_sp (freed with the graph or a re-split). */
+gga_ms **gga_graph_ms_slot(gga_graph *g);
 
 /* thread-local detail for gga_last_error() */
 void gga_set_error(const char *fmt, ...);

@@ -38,6 +38,8 @@ struct gga_graph {
     int have_result;
     uint32_t wl_grid[2][2];   /* [bfs][count] */
     uint32_t p_grid[2][2];
+    uint32_t split_gen;       /* bumped by every split; engines built on an older table are stale */
+    gga_ms *ms_cache;         /* engine of gga_multi_sp */
     hipEvent_t ev[4];
     hipEvent_t *kev;          /* per-launch timing events */
     uint32_t kev_cap;
@@ -156,6 +158,7 @@ int gga_graph_free(gga_graph *g)
     if (!g) return GGA_OK;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
+    if (g->ms_cache) gga_ms_free(g->ms_cache);
     dfree(g, (void **)&g->d_row, 4ull * ((uint64_t)g->n + 1));
     dfree(g, (void **)&g->d_col, 4ull * g->m);
     dfree(g, (void **)&g->d_w, 4ull * g->m);
@@ -363,6 +366,11 @@ int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets,
         T = g->maxdeg ? g->maxdeg : 1u;
     }
     free_split(g);
+    g->split_gen++;
+    if (g->ms_cache) {
+        gga_ms_free(g->ms_cache);
+        g->ms_cache = NULL;
+    }
     g->have_result = 0;
     tmp_words = ggak_split_tmp_words(g->n);
     if (tmp_words < GGA_MAXCHUNK_SLOTS) tmp_words = GGA_MAXCHUNK_SLOTS;
@@ -424,6 +432,23 @@ int gga_graph_chunks(gga_graph *g, uint32_t *chunk_begin, uint32_t *owner)
 fail:
     return rc;
 }
+
+int gga_graph_view(gga_graph *g, gga_graph_view_t *v)
+{
+    if (!g || !v) return GGA_EINVAL;
+    if (!g->split) {
+        int rc = gga_graph_split(g, 0, 0, 0, NULL);
+        if (rc) return rc;
+    }
+    v->dg = dev_graph(g);
+    v->stream = g->stream;
+    v->device = g->device;
+    v->split_gen = g->split_gen;
+    v->maxdeg = g->maxdeg;
+    return GGA_OK;
+}
+
+gga_ms **gga_graph_ms_slot(gga_graph *g) { return &g->ms_cache; }
 
 /* -------------------------------------------------------------- traversal */
 void gga_opts_init(gga_opts *o)

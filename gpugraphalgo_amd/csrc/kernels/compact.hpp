@@ -1,0 +1,111 @@
+// compact.hpp -- list building after a flag step.
+//
+// A flag step records the next frontier as a per-vertex predicate (a step
+// stamp, a lane mask); three passes then rebuild the chunk list from it with
+// no shared counter: count the chunk ids each tile of GGA_TILE vertices
+// contributes, scan the tile counts (one workgroup), write.  Lists are in
+// vertex order, so chunks of one vertex stay adjacent.
+#pragma once
+#include "../gga_dev.h"
+#include "sync.hpp"
+#include "wave.hpp"
+
+namespace gga {
+
+constexpr uint32_t CBLOCK = 256;                 // workgroup size of the compaction passes
+constexpr uint32_t CITEMS = GGA_TILE / CBLOCK;   // vertices per thread per tile
+constexpr uint32_t CSMALL = 4;                   // a thread writes at most this many ids itself
+
+__device__ __forceinline__ void queue_store(uint32_t *qout, uint32_t pos, uint32_t id, uint32_t qcap,
+                                            uint32_t *ctl) {
+    // a vertex is listed at most once per step, so pos < C <= qcap; the guard
+    // only turns a broken invariant into an error bit
+    if (pos < qcap) qout[pos] = id;
+    else __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_QFULL, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct ListOut {
+    const uint32_t *cstart;  // first chunk of each vertex (n + 1)
+    uint32_t *q;             // output list
+    uint32_t qcap;           // its capacity
+    uint32_t *tiles;         // per-tile counts, then offsets (GGA_TILES(n) entries)
+    uint32_t *tail;          // list length
+    uint32_t *ctl;           // error bits
+};
+
+// Per-thread view of one tile: CITEMS vertices v0 + i * CBLOCK + tid (coalesced).
+template <class Marked>
+__device__ __forceinline__ uint32_t tile_items(const ListOut &o, uint32_t n, uint32_t t, const Marked &marked,
+                                               uint32_t *cs, uint32_t *nch) {
+    const uint64_t v0 = (uint64_t)t * GGA_TILE;
+    uint32_t mine = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        const uint64_t v = v0 + (uint64_t)i * CBLOCK + threadIdx.x;
+        cs[i] = 0;
+        nch[i] = 0;
+        if (v < n && marked((uint32_t)v)) {
+            cs[i] = o.cstart[v];
+            nch[i] = o.cstart[v + 1] - cs[i];
+        }
+        mine += nch[i];
+    }
+    return mine;
+}
+
+// pass 1, one tile: tiles[t] = number of chunk ids tile t contributes
+template <class Marked>
+__device__ __forceinline__ void compact_count_tile(const ListOut &o, uint32_t n, uint32_t t, const Marked &marked,
+                                                   uint32_t *s_wave) {
+    uint32_t cs[CITEMS], nch[CITEMS], excl;
+    const uint32_t tot = block_excl_scan(tile_items(o, n, t, marked, cs, nch), excl, s_wave);
+    if (threadIdx.x == 0) o.tiles[t] = tot;
+}
+
+// pass 2 (one workgroup): exclusive scan of tiles[]; the total is the new list length
+__device__ __forceinline__ void compact_scan(const ListOut &o, uint32_t n, uint32_t *s_wave) {
+    const uint32_t ntiles = GGA_TILES(n);
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += CBLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t x = i < ntiles ? o.tiles[i] : 0u;
+        uint32_t excl;
+        const uint32_t tot = block_excl_scan(x, excl, s_wave);
+        if (i < ntiles) o.tiles[i] = carry + excl;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *o.tail = carry;
+}
+
+// pass 3, one tile: write.  A vertex with few chunks is written by its thread;
+// a hub's hundreds of chunk ids are written by the whole wave, 64 per round.
+template <class Marked>
+__device__ __forceinline__ void compact_write_tile(const ListOut &o, uint32_t n, uint32_t t, const Marked &marked,
+                                                   uint32_t *s_wave) {
+    uint32_t cs[CITEMS], nch[CITEMS], excl;
+    const uint32_t mine = tile_items(o, n, t, marked, cs, nch);
+    block_excl_scan(mine, excl, s_wave);
+    uint32_t pos = o.tiles[t] + excl;
+    uint32_t bpos[CITEMS];
+#pragma unroll
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        bpos[i] = pos;
+        if (nch[i] <= CSMALL)
+            for (uint32_t k = 0; k < nch[i]; k++) queue_store(o.q, pos + k, cs[i] + k, o.qcap, o.ctl);
+        pos += nch[i];
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        uint64_t big = __ballot(nch[i] > CSMALL);
+        while (big) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(big);
+            big &= big - 1;
+            const uint32_t p = wave_bcast(bpos[i], j), c0 = wave_bcast(cs[i], j);
+            const uint32_t cnt = wave_bcast(nch[i], j);
+            for (uint32_t k = lane_id(); k < cnt; k += GGA_WAVE) queue_store(o.q, p + k, c0 + k, o.qcap, o.ctl);
+        }
+    }
+}
+
+}  // namespace gga

@@ -1,0 +1,501 @@
+// ms.hip -- multi-source BFS/SSSP: 32 independent traversals per step.
+//
+// No reference counterpart: the reference runs one source per process
+// (bfs_sssp.cu:203).  Each lane is the same Bellman-Ford relaxation as the
+// single-source kernels (the guard and atomicMin of M_my_drelax,
+// cudaKernels.h:320-328), restricted to the lanes whose distance at the chunk
+// owner dropped in the previous step; a lane converges when no vertex carries
+// its bit, independently of the others, so lanes can be recycled between
+// steps (the serving loop admits a new query into a finished lane).
+//
+// Layout: D[v * 32 + lane] (one 128-byte line per vertex).  A half wave
+// relaxes one edge for all 32 lanes: lane s of the half reads dist[u][s] and
+// dist[v][s] (both one coalesced line), col[e] / w[e] are broadcast loads,
+// and the lanes that improved are published with one atomicOr into the
+// destination's next mask.  Lists are rebuilt each step by the shared
+// compaction passes from the next masks (flag steps only: a vertex enters the
+// list once however many lanes reached it).
+#include "../gga_kernels.h"
+#include "compact.hpp"
+#include "sync.hpp"
+#include "wave.hpp"
+
+using namespace gga;
+
+namespace {
+
+constexpr uint32_t BLOCK = 256;
+constexpr uint32_t WAVES = BLOCK / GGA_WAVE;
+constexpr uint32_t LANES = GGA_MS_LANES;
+#ifndef GGA_MS_UNROLL
+#define GGA_MS_UNROLL 4
+#endif
+constexpr uint32_t MS_UNROLL = GGA_MS_UNROLL;  // edges per half wave per round
+
+struct Counters {
+    uint64_t edges = 0, chunks = 0, relax = 0;
+};
+
+template <bool COUNT>
+__device__ __forceinline__ void publish_counters(const gga_dev_ms &ms, const Counters &c) {
+    if (COUNT && lane_id() == 0) {
+        const uint32_t slot = (blockIdx.x * WAVES + threadIdx.x / GGA_WAVE) % GGA_CNT_SLOTS;
+        uint64_t *s = ms.cnt + 4u * slot;
+        if (c.edges) atomic_add_u64(&s[0], c.edges);
+        if (c.chunks) atomic_add_u64(&s[1], c.chunks);
+        if (c.relax) atomic_add_u64(&s[2], c.relax);
+    }
+}
+
+__device__ __forceinline__ const uint32_t *ms_qin(const gga_dev_ms &ms, uint32_t step) {
+    return (step & 1u) ? ms.q1 : ms.q0;
+}
+__device__ __forceinline__ uint32_t *ms_mcur(const gga_dev_ms &ms, uint32_t step) {
+    return (step & 1u) ? ms.M1 : ms.M0;
+}
+__device__ __forceinline__ uint32_t *ms_mnxt(const gga_dev_ms &ms, uint32_t step) {
+    return (step & 1u) ? ms.M0 : ms.M1;
+}
+__device__ __forceinline__ uint32_t ms_len(const gga_dev_ms &ms, uint32_t step) {
+    const uint32_t t = ms.ctl[GGA_CTL_TAIL(step % 3u)];
+    return t < ms.qcap ? t : ms.qcap;
+}
+
+struct MaskMarked {
+    const uint32_t *mask;
+    __device__ __forceinline__ bool operator()(uint32_t v) const { return mask[v] != 0u; }
+};
+
+__device__ __forceinline__ ListOut ms_list_out(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step) {
+    ListOut o;
+    o.cstart = g.cstart;
+    o.q = (step & 1u) ? ms.q0 : ms.q1;
+    o.qcap = ms.qcap;
+    o.tiles = ms.tiles;
+    o.tail = &ms.ctl[GGA_CTL_TAIL((step + 1u) % 3u)];
+    o.ctl = ms.ctl;
+    return o;
+}
+
+__device__ __forceinline__ uint32_t wave_or(uint32_t x) {
+#pragma unroll
+    for (uint32_t o = 32; o >= 1; o >>= 1) x |= __shfl_xor(x, (int)o, GGA_WAVE);
+    return x;
+}
+
+// ---------------------------------------------------------------- relax
+template <bool COUNT>
+__device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
+                                         uint32_t nq, Counters &cnt) {
+    const uint32_t *qin = ms_qin(ms, step);
+    const uint32_t *mcur = ms_mcur(ms, step);
+    uint32_t *mnxt = ms_mnxt(ms, step);
+    uint32_t *__restrict__ D = ms.D;
+    const uint32_t inf = ms.inf;
+    const uint32_t waves = gridDim.x * WAVES;
+    const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
+    uint32_t bsz = nq / (8u * waves);
+    bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
+    const uint32_t nbatch = (nq + bsz - 1) / bsz;
+    const uint32_t l = lane_id(), half = l >> 5, s = l & (LANES - 1u);
+    const bool unit = (ms.unit_lanes >> s) & 1u;
+    for (uint32_t b = wid; b < nbatch; b += waves) {
+        const uint32_t idx = b * bsz + l;
+        const bool valid = l < bsz && idx < nq;
+        uint32_t eb = 0, ne = 0, u = 0, m = 0;
+        if (valid) {
+            const uint32_t c = stream_load(&qin[idx]);
+            u = stream_load(&g.cown[c]);
+            m = mcur[u];
+            eb = stream_load(&g.cbeg[c]);
+            ne = m ? stream_load(&g.cbeg[c + 1]) - eb : 0u;
+        }
+        const uint32_t incl = wave_incl_scan(ne);
+        const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+        const uint32_t excl = incl - ne;
+        if (COUNT) cnt.chunks += (uint32_t)__popcll(__ballot(ne > 0));
+        // edge slot k goes to half (k & 1); slots past the end are clamped to
+        // the last slot so every address below is in bounds
+        for (uint32_t base = 0; base < total; base += 2u * MS_UNROLL) {
+            uint32_t v[MS_UNROLL], wt[MS_UNROLL], du[MS_UNROLL], dv[MS_UNROLL], mm[MS_UNROLL];
+            bool act[MS_UNROLL];
+#pragma unroll
+            for (uint32_t i = 0; i < MS_UNROLL; i++) {
+                const uint32_t k = base + 2u * i + half;
+                act[i] = k < total;
+                const uint32_t kk = act[i] ? k : total - 1u;
+                const uint32_t j = wave_owner(incl, kk);
+                const uint32_t e = wave_bcast(eb, j) + (kk - wave_bcast(excl, j));
+                const uint32_t uu = wave_bcast(u, j);
+                mm[i] = wave_bcast(m, j);
+                v[i] = stream_load(&g.col[e]);
+                wt[i] = stream_load(&g.w[e]);
+                du[i] = D[(uint64_t)uu * LANES + s];
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < MS_UNROLL; i++) dv[i] = D[(uint64_t)v[i] * LANES + s];
+#pragma unroll
+            for (uint32_t i = 0; i < MS_UNROLL; i++) {
+                const bool on = act[i] && ((mm[i] >> s) & 1u) && du[i] < inf;
+                const uint32_t a = du[i] + (unit ? 1u : wt[i]);
+                bool imp = false;
+                if (on && a >= du[i] && a < inf && a < dv[i]) {
+                    const uint32_t old = atomic_min_u32(&D[(uint64_t)v[i] * LANES + s], a);
+                    imp = a < old;
+                }
+                const uint64_t bal = __ballot(imp);
+                const uint32_t hm = (uint32_t)(bal >> (32u * half));
+                if (s == 0 && hm)
+                    __hip_atomic_fetch_or(&mnxt[v[i]], hm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (COUNT) {
+                    cnt.edges += (uint32_t)__popcll(__ballot(on));
+                    cnt.relax += (uint32_t)__popcll(bal);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------- compaction passes
+// count: chunk ids per tile and the OR of the tile's next masks
+__device__ __forceinline__ void ms_count_tile(const gga_dev_graph &g, const gga_dev_ms &ms, const ListOut &o,
+                                              const uint32_t *mnxt, uint32_t t, uint32_t *s_wave,
+                                              uint32_t *s_or) {
+    compact_count_tile(o, g.n, t, MaskMarked{mnxt}, s_wave);
+    uint32_t x = 0;
+    const uint64_t v0 = (uint64_t)t * GGA_TILE;
+#pragma unroll
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        const uint64_t v = v0 + (uint64_t)i * CBLOCK + threadIdx.x;
+        if (v < g.n) x |= mnxt[v];
+    }
+    x = wave_or(x);
+    if (lane_id() == 0) s_or[threadIdx.x / GGA_WAVE] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t y = 0;
+        for (uint32_t i = 0; i < WAVES; i++) y |= s_or[i];
+        ms.tiles[GGA_TILES(g.n) + t] = y;
+    }
+    __syncthreads();
+}
+
+// scan (one workgroup): list offsets, list length and lane mask of the next slot
+__device__ __forceinline__ void ms_scan(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
+                                        uint32_t nq, uint32_t *s_wave, uint32_t *s_or) {
+    const uint32_t sout = (step + 1u) % 3u;
+    if (nq == 0) {
+        if (threadIdx.x == 0) {
+            ms.ctl[GGA_CTL_TAIL(sout)] = 0u;
+            ms.ctl[GGA_CTL_LANES(sout)] = 0u;
+        }
+        return;
+    }
+    compact_scan(ms_list_out(g, ms, step), g.n, s_wave);
+    const uint32_t ntiles = GGA_TILES(g.n);
+    uint32_t x = 0;
+    for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) x |= ms.tiles[ntiles + i];
+    x = wave_or(x);
+    if (lane_id() == 0) s_or[threadIdx.x / GGA_WAVE] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t y = 0;
+        for (uint32_t i = 0; i < WAVES; i++) y |= s_or[i];
+        ms.ctl[GGA_CTL_LANES(sout)] = y;
+        ms.ctl[GGA_CTL_STEPS] += 1u;
+    }
+    __syncthreads();
+}
+
+// write: chunk ids of the marked vertices; the consumed masks are cleared so
+// that they can be the next masks of the following step
+__device__ __forceinline__ void ms_write_tile(const gga_dev_graph &g, const gga_dev_ms &ms, const ListOut &o,
+                                              const uint32_t *mnxt, uint32_t *mcur, uint32_t t,
+                                              uint32_t *s_wave) {
+    compact_write_tile(o, g.n, t, MaskMarked{mnxt}, s_wave);
+    const uint64_t v0 = (uint64_t)t * GGA_TILE;
+#pragma unroll
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        const uint64_t v = v0 + (uint64_t)i * CBLOCK + threadIdx.x;
+        if (v < g.n) mcur[v] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------- kernels
+// Lanes in `lanes` restart: dist = inf except 0 at the lane's source.
+__global__ __launch_bounds__(BLOCK) void k_ms_reset(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes,
+                                                    gga_ms_sources src) {
+    const uint64_t total = (uint64_t)g.n * LANES;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = (uint32_t)(i & (LANES - 1u)), v = (uint32_t)(i / LANES);
+        if ((lanes >> s) & 1u) ms.D[i] = v == src.v[s] ? 0u : ms.inf;
+    }
+}
+
+// Seeds the restarted lanes into the list of ring slot `step`: the source's
+// mask gets the lane bit; the first lane to reach an unlisted source appends
+// its chunks.
+__global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes,
+                                                gga_ms_sources src, uint32_t step) {
+    const uint32_t s = threadIdx.x;
+    if (s >= LANES || !((lanes >> s) & 1u)) return;
+    const uint32_t x = src.v[s];
+    uint32_t *mcur = ms_mcur(ms, step);
+    const uint32_t old = __hip_atomic_fetch_or(&mcur[x], 1u << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_LANES(step % 3u)], 1u << s, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+    if (old != 0u) return;  // already listed
+    const uint32_t cs = g.cstart[x], nch = g.cstart[x + 1] - cs;
+    const uint32_t pos = atomic_add_u32(&ms.ctl[GGA_CTL_TAIL(step % 3u)], nch);
+    uint32_t *q = (step & 1u) ? ms.q1 : ms.q0;
+    for (uint32_t k = 0; k < nch; k++) queue_store(q, pos + k, cs + k, ms.qcap, ms.ctl);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    const uint32_t nq = ms_len(ms, step);
+    if (nq == 0) return;
+    Counters cnt;
+    ms_relax<COUNT>(g, ms, step, nq, cnt);
+    publish_counters<COUNT>(ms, cnt);
+}
+
+__global__ __launch_bounds__(CBLOCK) void k_ms_count(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE], s_or[CBLOCK / GGA_WAVE];
+    if (ms_len(ms, step) == 0) return;
+    const ListOut o = ms_list_out(g, ms, step);
+    const uint32_t *mnxt = ms_mnxt(ms, step);
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) ms_count_tile(g, ms, o, mnxt, t, s_wave, s_or);
+}
+
+__global__ __launch_bounds__(CBLOCK) void k_ms_scan(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE], s_or[CBLOCK / GGA_WAVE];
+    ms_scan(g, ms, step, ms_len(ms, step), s_wave, s_or);
+}
+
+__global__ __launch_bounds__(CBLOCK) void k_ms_write(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    if (ms_len(ms, step) == 0) return;
+    const ListOut o = ms_list_out(g, ms, step);
+    const uint32_t *mnxt = ms_mnxt(ms, step);
+    uint32_t *mcur = ms_mcur(ms, step);
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
+        ms_write_tile(g, ms, o, mnxt, mcur, t, s_wave);
+}
+
+// All steps in one cooperative launch: relax, then the three compaction
+// passes, separated by grid barriers.  Step numbers continue from `step0`.
+template <bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0,
+                                                         uint32_t max_steps, uint64_t spin_ticks) {
+    __shared__ uint32_t s_wave[WAVES], s_or[WAVES], s_state[2];
+    uint32_t gen = 0;
+    if (threadIdx.x == 0)
+        gen = __hip_atomic_load(&ms.ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Counters cnt;
+    for (uint32_t k = 0; k < max_steps; k++) {
+        const uint32_t step = step0 + k;
+        if (threadIdx.x == 0) {
+            s_state[0] = __hip_atomic_load(&ms.ctl[GGA_CTL_TAIL(step % 3u)], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            s_state[1] = __hip_atomic_load(&ms.ctl[GGA_CTL_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        const uint32_t nq = s_state[0] < ms.qcap ? s_state[0] : ms.qcap, err = s_state[1];
+        __syncthreads();
+        if (nq == 0 || err) break;
+        ms_relax<COUNT>(g, ms, step, nq, cnt);
+        grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
+        const ListOut o = ms_list_out(g, ms, step);
+        const uint32_t *mnxt = ms_mnxt(ms, step);
+        for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
+            ms_count_tile(g, ms, o, mnxt, t, s_wave, s_or);
+        grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
+        if (blockIdx.x == 0) ms_scan(g, ms, step, nq, s_wave, s_or);
+        grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
+        uint32_t *mcur = ms_mcur(ms, step);
+        for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
+            ms_write_tile(g, ms, o, mnxt, mcur, t, s_wave);
+        grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
+    }
+    publish_counters<COUNT>(ms, cnt);
+}
+
+// Lane results out of the vertex-major table: a workgroup transposes 64
+// vertices x 32 lanes through LDS and writes each selected lane's 64 values as
+// one contiguous run of out.p[lane].
+constexpr uint32_t HV = 64;
+__global__ __launch_bounds__(BLOCK) void k_ms_harvest(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes,
+                                                      gga_ms_outputs out) {
+    __shared__ uint32_t tile[HV][LANES + 1];
+    const uint64_t v0 = (uint64_t)blockIdx.x * HV;
+    for (uint32_t i = threadIdx.x; i < HV * LANES; i += blockDim.x) {
+        const uint64_t v = v0 + i / LANES;
+        tile[i / LANES][i % LANES] = v < g.n ? ms.D[v * LANES + (i % LANES)] : 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < HV * LANES; i += blockDim.x) {
+        const uint32_t s = i / HV, j = i % HV;
+        const uint64_t v = v0 + j;
+        if (((lanes >> s) & 1u) && v < g.n) out.p[s][v] = tile[j][s];
+    }
+}
+
+// Reached (vertex, lane) pairs of `lanes` and the out-degree sum over them
+// (the GTEPS numerator): per-workgroup partials in word 3 of cnt slots
+// [0, REACH) and [REACH, 2 REACH), folded by k_ms_fold.
+constexpr uint32_t REACH = GGA_CNT_SLOTS / 2;
+__global__ __launch_bounds__(BLOCK) void k_ms_reach(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes) {
+    __shared__ uint64_t s_red[2][WAVES];
+    uint64_t r = 0, mr = 0;
+    const uint32_t s = threadIdx.x & (LANES - 1u);
+    const bool mine = (lanes >> s) & 1u;
+    // thread i of the grid reads D[i]: lane i % 32 of vertex i / 32 (coalesced)
+    const uint64_t total = (uint64_t)g.n * LANES;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (mine && ms.D[i] < ms.inf) {
+            const uint32_t v = (uint32_t)(i / LANES);
+            r += 1;
+            mr += g.row[v + 1] - g.row[v];
+        }
+    }
+    r = wave_sum64(r);
+    mr = wave_sum64(mr);
+    if (lane_id() == 0) {
+        s_red[0][threadIdx.x / GGA_WAVE] = r;
+        s_red[1][threadIdx.x / GGA_WAVE] = mr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0, b = 0;
+        for (uint32_t i = 0; i < WAVES; i++) {
+            a += s_red[0][i];
+            b += s_red[1][i];
+        }
+        ms.cnt[4u * blockIdx.x + 3u] = a;
+        ms.cnt[4u * (REACH + blockIdx.x) + 3u] = b;
+    }
+}
+
+// Work counters: fold the striped slots into ctl[CNT] (one workgroup) and the
+// reach partials into ctl[RED] (2 x uint64).
+__global__ __launch_bounds__(BLOCK) void k_ms_fold(gga_dev_ms ms) {
+    __shared__ uint64_t s_red[5][WAVES];
+    uint64_t x[5] = {0, 0, 0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < GGA_CNT_SLOTS; i += blockDim.x) {
+        x[0] += ms.cnt[4u * i];
+        x[1] += ms.cnt[4u * i + 1u];
+        x[2] += ms.cnt[4u * i + 2u];
+        x[i < REACH ? 3 : 4] += ms.cnt[4u * i + 3u];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 5; k++) {
+        x[k] = wave_sum64(x[k]);
+        if (lane_id() == 0) s_red[k][threadIdx.x / GGA_WAVE] = x[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        uint64_t t = 0;
+        for (uint32_t i = 0; i < WAVES; i++) t += s_red[threadIdx.x][i];
+        if (threadIdx.x < 3) reinterpret_cast<uint64_t *>(ms.ctl + GGA_CTL_CNT)[threadIdx.x] = t;
+        else reinterpret_cast<uint64_t *>(ms.ctl + GGA_CTL_RED)[threadIdx.x - 3] = t;
+    }
+}
+
+// Counters, ring slots, lane masks and barrier words start at zero.
+__global__ __launch_bounds__(BLOCK) void k_ms_clear(gga_dev_ms ms) {
+    for (uint32_t i = threadIdx.x; i < 4u * GGA_CNT_SLOTS; i += blockDim.x) ms.cnt[i] = 0;
+    for (uint32_t i = threadIdx.x; i < GGA_CTL_WORDS; i += blockDim.x) ms.ctl[i] = 0;
+}
+
+inline int launch_status() { return (int)hipGetLastError(); }
+
+inline uint32_t grid_for(uint64_t items, uint32_t cap) {
+    uint64_t b = (items + BLOCK - 1) / BLOCK;
+    if (b == 0) b = 1;
+    return (uint32_t)(b < cap ? b : cap);
+}
+
+inline uint32_t compact_grid(uint32_t n) {
+    const uint32_t ntiles = GGA_TILES(n);
+    return ntiles < 2048u ? (ntiles ? ntiles : 1u) : 2048u;
+}
+
+}  // namespace
+
+extern "C" int ggak_ms_clear(gga_dev_ms ms, hipStream_t s) {
+    k_ms_clear<<<1, BLOCK, 0, s>>>(ms);
+    return launch_status();
+}
+
+extern "C" int ggak_ms_start(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, gga_ms_sources src, uint32_t step,
+                             hipStream_t s) {
+    if (lanes == 0) return 0;
+    for (uint32_t i = 0; i < LANES; i++)
+        if (((lanes >> i) & 1u) && src.v[i] >= g.n) return (int)hipErrorInvalidValue;
+    k_ms_reset<<<grid_for((uint64_t)g.n * LANES, 8192), BLOCK, 0, s>>>(g, ms, lanes, src);
+    k_ms_seed<<<1, 64, 0, s>>>(g, ms, lanes, src, step);
+    return launch_status();
+}
+
+extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int count, uint32_t grid,
+                            hipStream_t s) {
+    if (grid == 0) return (int)hipErrorInvalidValue;
+    if (count) k_ms_step<true><<<grid, BLOCK, 0, s>>>(g, ms, step);
+    else k_ms_step<false><<<grid, BLOCK, 0, s>>>(g, ms, step);
+    const uint32_t cg = compact_grid(g.n);
+    k_ms_count<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+    k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
+    k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+    return launch_status();
+}
+
+extern "C" uint32_t ggak_ms_grid(int device, int persistent, int count) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    const void *fn = persistent ? (count ? reinterpret_cast<const void *>(&k_ms_persistent<true>)
+                                         : reinterpret_cast<const void *>(&k_ms_persistent<false>))
+                                : (count ? reinterpret_cast<const void *>(&k_ms_step<true>)
+                                         : reinterpret_cast<const void *>(&k_ms_step<false>));
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, 0) != hipSuccess || per <= 0) return 0;
+    // one workgroup per CU for the cooperative launch (see ggak_wl_grid)
+    if (persistent) per = 1;
+    return (uint32_t)cus * (uint32_t)per;
+}
+
+extern "C" int ggak_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0, uint32_t max_steps, int count,
+                                  uint32_t grid, uint32_t spin_limit_ms, hipStream_t s) {
+    if (grid == 0) return (int)hipErrorInvalidValue;
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;
+    uint64_t spin_ticks = (uint64_t)spin_limit_ms * (uint64_t)khz;
+    void *args[] = {&g, &ms, &step0, &max_steps, &spin_ticks};
+    const void *fn = count ? reinterpret_cast<const void *>(&k_ms_persistent<true>)
+                           : reinterpret_cast<const void *>(&k_ms_persistent<false>);
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(BLOCK), args, 0, s);
+    if (e != hipSuccess) return (int)e;
+    return launch_status();
+}
+
+extern "C" int ggak_ms_harvest(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, gga_ms_outputs out,
+                               hipStream_t s) {
+    if (lanes == 0) return 0;
+    const uint32_t grid = (uint32_t)(((uint64_t)g.n + HV - 1) / HV);
+    k_ms_harvest<<<grid, BLOCK, 0, s>>>(g, ms, lanes, out);
+    return launch_status();
+}
+
+extern "C" int ggak_ms_reach(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, hipStream_t s) {
+    // REACH partial slots are always all written (idle workgroups store 0)
+    k_ms_reach<<<REACH, BLOCK, 0, s>>>(g, ms, lanes);
+    return launch_status();
+}
+
+extern "C" int ggak_ms_fold(gga_dev_ms ms, hipStream_t s) {
+    k_ms_fold<<<1, BLOCK, 0, s>>>(ms);
+    return launch_status();
+}

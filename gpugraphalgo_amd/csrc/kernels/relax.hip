@@ -41,6 +41,8 @@
 // dist[u] 4 (+ 4 for the worklist id); per relaxation 4 + 4 (queue / stamp).
 #include "../gga_kernels.h"
 #include "wave.hpp"
+#include "sync.hpp"
+#include "compact.hpp"
 
 using namespace gga;
 
@@ -49,8 +51,6 @@ namespace {
 constexpr uint32_t BLOCK = 256;
 constexpr uint32_t WAVES = BLOCK / GGA_WAVE;
 constexpr uint32_t PBUF = 256;          // queued vertices staged per wave before a flush
-constexpr uint32_t CITEMS = GGA_TILE / BLOCK;  // compaction: vertices per thread per tile
-constexpr uint32_t CSMALL = 4;          // compaction: a thread writes at most this many ids itself
 #ifndef GGA_UNROLL
 #define GGA_UNROLL 4
 #endif
@@ -61,15 +61,6 @@ enum { OUT_CHANGED = 0, OUT_QUEUE = 1, OUT_FLAG = 2 };
 struct Counters {
     uint64_t edges = 0, chunks = 0, relax = 0;
 };
-
-__device__ __forceinline__ void queue_store(uint32_t *qout, uint32_t pos, uint32_t id, uint32_t qcap,
-                                            uint32_t *ctl) {
-    // a vertex is queued at most once per step, so pos < C <= qcap; the guard
-    // only turns a broken invariant into an error bit
-    if (pos < qcap) qout[pos] = id;
-    else __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_QFULL, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Append the wave's staged vertices to the chunk worklist: every staged vertex
 // contributes its chunk range [cstart[v], cstart[v + 1]).
@@ -202,24 +193,6 @@ __device__ __forceinline__ void publish_counters(const gga_dev_work &w, const Co
     }
 }
 
-// Block-wide exclusive scan of one value per thread; returns the block total.
-// s_wave needs WAVES entries.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t &excl, uint32_t *s_wave) {
-    const uint32_t wv = threadIdx.x / GGA_WAVE;
-    const uint32_t incl = wave_incl_scan(x);
-    if (lane_id() == GGA_WAVE - 1) s_wave[wv] = incl;
-    __syncthreads();
-    uint32_t off = 0, tot = 0;
-    for (uint32_t i = 0; i < WAVES; i++) {
-        const uint32_t t = s_wave[i];
-        off += i < wv ? t : 0u;
-        tot += t;
-    }
-    __syncthreads();
-    excl = off + incl - x;
-    return tot;
-}
-
 // ---------------------------------------------------------------- init
 __global__ __launch_bounds__(BLOCK) void k_fill(uint32_t *p, uint32_t value, uint64_t count) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
@@ -310,86 +283,12 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
 }
 
-// ---- list compaction after a flag step: chunks of every vertex stamped `it`
-// go to one output list.
-struct ListOut {
-    const uint32_t *cstart;  // first chunk of each vertex (n + 1)
-    uint32_t *q;             // output list
-    uint32_t qcap;           // its capacity
-    uint32_t *tiles;         // per-tile counts, then offsets (GGA_TILES(n) entries)
-    uint32_t *tail;          // list length
+// A vertex belongs to the next list when the current step stamped it.
+struct StampMarked {
+    const uint32_t *mark;
+    uint32_t it;
+    __device__ __forceinline__ bool operator()(uint32_t v) const { return mark[v] == it; }
 };
-
-// Per-thread view of one tile: CITEMS vertices v0 + i * BLOCK + tid (coalesced).
-__device__ __forceinline__ uint32_t tile_items(const ListOut &o, const gga_dev_work &w, uint32_t n,
-                                               uint32_t t, uint32_t it, uint32_t *cs, uint32_t *nch) {
-    const uint64_t v0 = (uint64_t)t * GGA_TILE;
-    uint32_t mine = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < CITEMS; i++) {
-        const uint64_t v = v0 + (uint64_t)i * BLOCK + threadIdx.x;
-        cs[i] = 0;
-        nch[i] = 0;
-        if (v < n && w.mark[v] == it) {
-            cs[i] = o.cstart[v];
-            nch[i] = o.cstart[v + 1] - cs[i];
-        }
-        mine += nch[i];
-    }
-    return mine;
-}
-
-// pass 1, one tile: tiles[t] = number of chunk ids tile t contributes
-__device__ __forceinline__ void compact_count_tile(const ListOut &o, const gga_dev_work &w, uint32_t n,
-                                                   uint32_t t, uint32_t it, uint32_t *s_wave) {
-    uint32_t cs[CITEMS], nch[CITEMS], excl;
-    const uint32_t tot = block_excl_scan(tile_items(o, w, n, t, it, cs, nch), excl, s_wave);
-    if (threadIdx.x == 0) o.tiles[t] = tot;
-}
-
-// pass 2 (one workgroup): exclusive scan of tiles[]; the total is the new list length
-__device__ __forceinline__ void compact_scan(const ListOut &o, uint32_t n, uint32_t *s_wave) {
-    const uint32_t ntiles = GGA_TILES(n);
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < ntiles; base += BLOCK) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t x = i < ntiles ? o.tiles[i] : 0u;
-        uint32_t excl;
-        const uint32_t tot = block_excl_scan(x, excl, s_wave);
-        if (i < ntiles) o.tiles[i] = carry + excl;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) *o.tail = carry;
-}
-
-// pass 3, one tile: write.  A vertex with few chunks is written by its thread;
-// a hub's hundreds of chunk ids are written by the whole wave, 64 per round.
-__device__ __forceinline__ void compact_write_tile(const ListOut &o, const gga_dev_work &w, uint32_t n,
-                                                   uint32_t t, uint32_t it, uint32_t *s_wave) {
-    uint32_t cs[CITEMS], nch[CITEMS], excl;
-    const uint32_t mine = tile_items(o, w, n, t, it, cs, nch);
-    block_excl_scan(mine, excl, s_wave);
-    uint32_t pos = o.tiles[t] + excl;
-    uint32_t bpos[CITEMS];
-#pragma unroll
-    for (uint32_t i = 0; i < CITEMS; i++) {
-        bpos[i] = pos;
-        if (nch[i] <= CSMALL)
-            for (uint32_t k = 0; k < nch[i]; k++) queue_store(o.q, pos + k, cs[i] + k, o.qcap, w.ctl);
-        pos += nch[i];
-    }
-#pragma unroll
-    for (uint32_t i = 0; i < CITEMS; i++) {
-        uint64_t big = __ballot(nch[i] > CSMALL);
-        while (big) {
-            const uint32_t j = (uint32_t)__builtin_ctzll(big);
-            big &= big - 1;
-            const uint32_t p = wave_bcast(bpos[i], j), c0 = wave_bcast(cs[i], j);
-            const uint32_t cnt = wave_bcast(nch[i], j);
-            for (uint32_t k = lane_id(); k < cnt; k += GGA_WAVE) queue_store(o.q, p + k, c0 + k, o.qcap, w.ctl);
-        }
-    }
-}
 
 // The single list of the plain layout for step `step`.
 __device__ __forceinline__ ListOut list_out(const gga_dev_graph &g, const gga_dev_work &w, uint32_t step) {
@@ -399,6 +298,7 @@ __device__ __forceinline__ ListOut list_out(const gga_dev_graph &g, const gga_de
     o.qcap = w.qcap;
     o.tiles = w.tiles;
     o.tail = &w.ctl[GGA_CTL_TAIL((step + 1u) % 3u)];
+    o.ctl = w.ctl;
     return o;
 }
 
@@ -437,7 +337,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_de
     if (nq == 0 || !flag_step(w, nq)) return;
     const ListOut o = list_out(g, w, step);
     const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, w, g.n, t, it, s_wave);
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_compact_scan(gga_dev_graph g, gga_dev_work w, uint32_t step) {
@@ -453,58 +353,10 @@ __global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_de
     if (nq == 0 || !flag_step(w, nq)) return;
     const ListOut o = list_out(g, w, step);
     const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, w, g.n, t, it, s_wave);
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
 }
 
 // ------------------------------------------------------ persistent kernel
-// Two-level grid barrier for a cooperative launch: workgroup b arrives at the
-// counter of group b % GGA_BAR_GROUPS; the last arrival of a group arrives at
-// the top counter; the last group leader releases everyone by bumping the
-// generation.  Each counter sees at most ~nblocks / 32 + 32 atomics per
-// barrier instead of nblocks.  Every wait is bounded by `spin_ticks` of the
-// 100 MHz wall clock; on expiry the error bit is set and every workgroup
-// leaves the step loop at its next check.
-// Agent-scope fences cost an L2 write-back / invalidate on gfx950, so only
-// thread 0 fences: its acq_rel arrival publishes the workgroup's writes (the
-// workgroup barrier orders them first), it polls with relaxed coherent loads
-// (an acquire load per poll would invalidate L2 every iteration), and one
-// acquire fence after the release covers the whole workgroup (shared L1/L2).
-__device__ __forceinline__ void grid_barrier(uint32_t *ctl, uint32_t nblocks, uint32_t &gen,
-                                             uint64_t spin_ticks) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t ngroups = nblocks < GGA_BAR_GROUPS ? nblocks : GGA_BAR_GROUPS;
-        const uint32_t grp = blockIdx.x % GGA_BAR_GROUPS;
-        const uint32_t gsize = (nblocks - grp + GGA_BAR_GROUPS - 1u) / GGA_BAR_GROUPS;
-        bool release = false;
-        const uint32_t arrived = __hip_atomic_fetch_add(&ctl[GGA_CTL_BARG(grp)], 1u, __ATOMIC_ACQ_REL,
-                                                        __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        if (arrived == gsize) {
-            __hip_atomic_store(&ctl[GGA_CTL_BARG(grp)], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t top = __hip_atomic_fetch_add(&ctl[GGA_CTL_BAR], 1u, __ATOMIC_ACQ_REL,
-                                                        __HIP_MEMORY_SCOPE_AGENT) + 1u;
-            release = top == ngroups;
-        }
-        if (release) {
-            __hip_atomic_store(&ctl[GGA_CTL_BAR], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl[GGA_CTL_BARGEN], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(&ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(2);
-                if (wall_clock64() - t0 > spin_ticks) {
-                    __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_SPIN, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-        gen += 1u;
-    }
-    __syncthreads();
-}
-
 template <bool BFS, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_work w,
                                                       uint32_t max_steps, uint64_t spin_ticks) {
@@ -532,12 +384,12 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
             const ListOut o = list_out(g, w, step);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
             for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
-                compact_count_tile(o, w, g.n, t, it, s_wave);
+                compact_count_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
             if (blockIdx.x == 0) compact_scan(o, g.n, s_wave);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
             for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
-                compact_write_tile(o, w, g.n, t, it, s_wave);
+                compact_write_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
         }
         grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
     }

@@ -15,6 +15,7 @@
 // about 8n + 4n + 8n + 8C bytes in total.
 #include "../gga_kernels.h"
 #include "wave.hpp"
+#include "sync.hpp"
 
 using namespace gga;
 
@@ -29,22 +30,6 @@ __device__ __forceinline__ uint32_t nchunks(uint32_t deg, uint32_t mda) {
     return deg == 0 ? 1u : (deg - 1u) / mda + 1u;
 }
 
-// Block-wide exclusive scan of one value per thread; returns the block total.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t &excl, uint32_t *s_wave) {
-    const uint32_t l = lane_id(), wv = threadIdx.x / GGA_WAVE, nw = blockDim.x / GGA_WAVE;
-    uint32_t incl = wave_incl_scan(x);
-    if (l == GGA_WAVE - 1) s_wave[wv] = incl;
-    __syncthreads();
-    uint32_t off = 0, tot = 0;
-    for (uint32_t i = 0; i < nw; i++) {
-        uint32_t t = s_wave[i];
-        if (i < wv) off += t;
-        tot += t;
-    }
-    __syncthreads();
-    excl = off + incl - x;
-    return tot;
-}
 
 __global__ __launch_bounds__(256) void k_degree_max(const uint32_t *__restrict__ row, uint32_t n,
                                                      uint64_t *key_out) {

@@ -213,12 +213,7 @@ class DeviceGraph:
 
     def _run(self, fn, src: int, mode: str, inf: int, count: bool, keep_on_device: bool,
              **kw) -> tuple[np.ndarray | None, dict]:
-        o = _lib.Opts()
-        o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
-        o.inf = inf
-        o.count = 1 if count else 0
-        for k, v in kw.items():
-            setattr(o, k, v)
+        o = _opts(mode, inf, count, kw)
         st = _lib.Stats()
         out = None if keep_on_device else np.empty(self.n, dtype=_u32)
         check(fn(self._h, src, _ptr(out), ctypes.byref(o), ctypes.byref(st)), fn.__name__)
@@ -244,7 +239,98 @@ class DeviceGraph:
         return {"nerr": r.nerr, "untight": r.untight, "src_ok": bool(r.src_ok),
                 "reached": r.reached}
 
+    def multi_sp(self, srcs, bfs_lanes=0, mode: str = "persistent", inf: int = INF,
+                 count: bool = False, keep_on_device: bool = False, **kw):
+        """gga_multi_sp: up to 32 sources in one traversal.  bfs_lanes: bit i set
+        -> source i is a BFS (hop counts).  Returns (k x n array or None, stats)."""
+        srcs = np.ascontiguousarray(srcs, dtype=_u32)
+        k = int(srcs.size)
+        o = _opts(mode, inf, count, kw)
+        st = _lib.Stats()
+        out = None if keep_on_device else np.empty((k, self.n), dtype=_u32)
+        check(load().gga_multi_sp(self._h, _ptr(srcs), k, int(bfs_lanes), _ptr(out), ctypes.byref(o),
+                                  ctypes.byref(st)), "gga_multi_sp")
+        return out, _stats_dict(st)
 
-__all__ = ["CsrGraph", "DeviceGraph", "csr_from_arrays", "read_gr", "write_gr", "gen_rmat",
+
+def _opts(mode, inf: int, count: bool, kw: dict) -> _lib.Opts:
+    o = _lib.Opts()
+    o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+    o.inf = inf
+    o.count = 1 if count else 0
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class MultiSourceEngine:
+    """gga_ms: 32 traversal lanes on one DeviceGraph; lanes start, converge and
+    restart independently (the serving loop's unit of continuous batching)."""
+
+    LANES = 32
+
+    def __init__(self, dg: DeviceGraph, inf: int = INF):
+        self.dg = dg
+        self.n = dg.n
+        self._h = ctypes.c_void_p()
+        check(load().gga_ms_create(dg._h, inf, ctypes.byref(self._h)), "gga_ms_create")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            load().gga_ms_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def start(self, lanes: dict, bfs: set | frozenset = frozenset()) -> None:
+        """lanes: {lane: source}; bfs: lanes that count hops."""
+        src = np.zeros(self.LANES, dtype=_u32)
+        mask = unit = 0
+        for lane, s in lanes.items():
+            src[lane] = s
+            mask |= 1 << lane
+            if lane in bfs:
+                unit |= 1 << lane
+        check(load().gga_ms_start(self._h, mask, _ptr(src), unit), "gga_ms_start")
+
+    def run(self, max_steps: int = 0, mode: str = "worklist", count: bool = False, **kw) -> int:
+        """Runs steps; returns the mask of lanes with work left."""
+        o = _opts(mode, 0, count, kw)
+        live = ctypes.c_uint32()
+        check(load().gga_ms_run(self._h, max_steps, ctypes.byref(o), ctypes.byref(live)), "gga_ms_run")
+        return int(live.value)
+
+    @property
+    def done(self) -> int:
+        return int(load().gga_ms_done(self._h))
+
+    def fetch(self, lanes) -> dict:
+        """{lane: distances} for finished lanes."""
+        arrs = {i: np.empty(self.n, dtype=_u32) for i in lanes}
+        ptrs = (ctypes.c_void_p * self.LANES)()
+        mask = 0
+        for i, a in arrs.items():
+            ptrs[i] = a.ctypes.data
+            mask |= 1 << i
+        check(load().gga_ms_fetch(self._h, mask, ptrs, 0), "gga_ms_fetch")
+        return arrs
+
+    def stats(self) -> dict:
+        st = _lib.Stats()
+        check(load().gga_ms_stats(self._h, ctypes.byref(st)), "gga_ms_stats")
+        return _stats_dict(st)
+
+
+__all__ = ["CsrGraph", "DeviceGraph", "MultiSourceEngine", "csr_from_arrays", "read_gr", "write_gr", "gen_rmat",
            "gen_grid", "verify", "write_dist", "write_result_row", "device_count", "INF",
            "REF_INF"]

@@ -228,6 +228,49 @@ int gga_write_result_row(const char *path, const gga_result_row *r);
 /* Graph name = last path component (getGraphname, myutils.h:76-84). */
 void gga_graph_name(const char *path, char *out, size_t cap);
 
+/* ------------------------------------------- multi-source engine (32 lanes)
+ * No reference counterpart (the reference runs one source per process,
+ * bfs_sssp.cu:203).  32 independent BFS/SSSP traversals ("lanes") share every
+ * relax step of one graph: distances are stored vertex-major (the 32 lanes of
+ * a vertex are one 128-byte line), a step expands each listed chunk only for
+ * the lanes whose distance at its owner dropped in the previous step, and each
+ * lane converges on its own.  A finished lane can be restarted with a new
+ * source while the others keep running (the serving loop's continuous
+ * batching).  Results per lane equal gga_bfs / gga_sssp from that source. */
+#define GGA_MS_LANES 32
+typedef struct gga_ms gga_ms;
+/* Engine on a resident graph (split with the defaults first if it is not).
+ * Device memory: 136 B per vertex + 8 B per chunk + the per-tile scratch.
+ * inf: unreached value (0 -> GGA_INF).  Re-splitting the graph invalidates it. */
+int gga_ms_create(gga_graph *g, uint32_t inf, gga_ms **out);
+int gga_ms_free(gga_ms *e);
+/* Restarts the lanes in `lanes` (bit i = lane i) at src[i] (src holds 32
+ * entries; only those of `lanes` are read).  Lanes in unit_lanes count hops
+ * (BFS levels), the others add edge weights.  A lane still running is EINVAL. */
+int gga_ms_start(gga_ms *e, uint32_t lanes, const uint32_t *src, uint32_t unit_lanes);
+/* Runs up to max_steps steps (0: until every lane converged).  mode:
+ * GGA_MODE_WORKLIST (four kernels per step, `sync_every` steps per host check,
+ * replayed from a captured hipGraph when use_graph) or GGA_MODE_PERSISTENT (one
+ * cooperative launch).  o may be NULL.  *live = lanes with work left; a lane
+ * started and not live has finished. */
+int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live);
+/* Lanes started and finished since their last start (not yet restarted). */
+uint32_t gga_ms_done(const gga_ms *e);
+/* Copies the distances of finished lanes: out[i] (n uint32) for each lane i in
+ * `lanes`; host pointers, or device pointers when on_device. */
+int gga_ms_fetch(gga_ms *e, uint32_t lanes, uint32_t *const *out, int on_device);
+/* Engine totals since creation: iterations = steps, edges_inspected =
+ * (edge, lane) relax checks, relaxations, ktime_ms = step kernel time. */
+int gga_ms_stats(const gga_ms *e, gga_stats *st);
+
+/* k <= 32 sources in one traversal (config: batched multi-source SSSP / BFS):
+ * out = k x n row-major (row i = source srcs[i]) or NULL to keep the results on
+ * the device; lanes in unit_lanes are BFS.  o->mode GGA_MODE_WORKLIST or
+ * GGA_MODE_PERSISTENT (AUTO -> PERSISTENT).  st: iterations = steps, time_ms =
+ * device time start -> convergence, reached / m_reach summed over sources. */
+int gga_multi_sp(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t unit_lanes, uint32_t *out,
+                 const gga_opts *o, gga_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

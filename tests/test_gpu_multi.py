@@ -1,0 +1,160 @@
+"""GPU parity of the 32-lane multi-source engine (gga_ms_*, gga_multi_sp).
+
+Every lane is checked against the oracle's single-source result from the same
+source: batched runs in both launch modes, hipGraph-replayed steps, duplicate
+and edge-less sources, and lanes recycled while other lanes keep running
+(a lane's result alone equals its result in the mix).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gpugraphalgo_amd as gga
+from gpugraphalgo_amd import _lib
+import oracle
+from conftest import golden_graph
+
+pytestmark = pytest.mark.gpu
+
+INF = gga.INF
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def want_for(g, src, bfs):
+    if bfs:
+        return oracle.bfs_queue(g.row_ptr, g.dst, int(src))
+    return oracle.dijkstra(g.row_ptr, g.dst, g.w, int(src))
+
+
+def nonisolated(g, k, seed):
+    deg = g.degrees()
+    cand = np.flatnonzero(deg > 0)
+    return np.random.default_rng(seed).choice(cand, size=k, replace=False).astype(np.uint32)
+
+
+@pytest.mark.parametrize("mode", ["worklist", "persistent"])
+@pytest.mark.parametrize("T", [1, 16, 256])
+def test_multi_sp_rmat_each_lane_equals_oracle(gpu, mode, T):
+    g = gga.gen_rmat(14, 16, seed=13, wlo=1, whi=100)
+    srcs = nonisolated(g, 32, T)
+    srcs[5] = srcs[4]                    # duplicate source in two lanes
+    bfs_lanes = 0xA5A5A5A5
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=T)
+        announce(f"multi_sp rmat14 T={T} mode={mode}")
+        out, st = dg.multi_sp(srcs, bfs_lanes=bfs_lanes, mode=mode, count=True)
+        reached = 0
+        mreach = 0
+        for i, s in enumerate(srcs):
+            want = want_for(g, s, (bfs_lanes >> i) & 1)
+            np.testing.assert_array_equal(out[i], want, err_msg=f"lane {i} src {s}")
+            reached += int((want != INF).sum())
+            mreach += oracle.m_reach(g.row_ptr, want)
+        assert st["reached"] == reached
+        assert st["m_reach"] == mreach
+        assert st["edges_inspected"] >= mreach
+
+
+def test_multi_sp_golden_graphs_and_small_k(gpu, golden):
+    index, arrays, _ = golden
+    for name in sorted({e["graph"] for e in index}):
+        rp, dst, w = golden_graph(arrays, name)
+        g = gga.csr_from_arrays(rp, dst, w)
+        n = g.n
+        srcs = np.arange(min(n, 32), dtype=np.uint32)
+        with gga.DeviceGraph(g) as dg:
+            for mda in (1, 3):
+                dg.split(mda=mda)
+                for mode in ("worklist", "persistent"):
+                    for k in sorted({1, srcs.size}):
+                        announce(f"multi_sp golden {name} mda={mda} mode={mode} k={k}")
+                        out, _ = dg.multi_sp(srcs[:k], bfs_lanes=0x3, mode=mode)
+                        for i in range(k):
+                            want = want_for(g, srcs[i], i < 2)
+                            np.testing.assert_array_equal(out[i], want, err_msg=f"{name} lane {i}")
+
+
+def test_multi_sp_hipgraph_replay_and_ref_inf(gpu):
+    g = gga.gen_grid(60, 70, 0.01, seed=11, wlo=1, whi=1000)
+    srcs = np.array([0, 4199, 77, 2100] * 8, dtype=np.uint32)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=4)
+        for rep in range(3):
+            announce(f"multi_sp grid60x70 hipgraph replay {rep}")
+            out, st = dg.multi_sp(srcs, bfs_lanes=0xFFFF0000, mode="worklist", use_graph=1)
+            for i, s in enumerate(srcs):
+                np.testing.assert_array_equal(out[i], want_for(g, s, i >= 16), err_msg=f"lane {i}")
+        announce("multi_sp grid60x70 inf=1e9")
+        out, _ = dg.multi_sp(srcs[:3], mode="persistent", inf=gga.REF_INF)
+        for i in range(3):
+            want = want_for(g, srcs[i], False)
+            want = np.where(want == INF, gga.REF_INF, want).astype(np.uint32)
+            np.testing.assert_array_equal(out[i], want)
+
+
+def test_engine_lanes_recycled_while_others_run(gpu):
+    """Continuous batching at the engine level: lanes finish at different
+    steps, are harvested and restarted with new sources (BFS or SSSP) while the
+    remaining lanes keep running; every result equals the single-source one."""
+    g = gga.gen_rmat(13, 8, seed=5, wlo=1, whi=100)
+    rng = np.random.default_rng(1)
+    queue = [(int(s), bool(rng.integers(0, 2))) for s in nonisolated(g, 120, 3)]
+    queue += [(int(np.flatnonzero(g.degrees() == 0)[0]), False)] if (g.degrees() == 0).any() else []
+    results = {}
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=32)
+        with gga.MultiSourceEngine(dg) as eng:
+            lane_q = {}
+            pending = list(enumerate(queue))
+            steps = 0
+            while pending or lane_q:
+                free = [l for l in range(32) if l not in lane_q]
+                start = {}
+                bfs = set()
+                for l in free:
+                    if not pending:
+                        break
+                    qi, (s, b) = pending.pop(0)
+                    lane_q[l] = qi
+                    start[l] = s
+                    if b:
+                        bfs.add(l)
+                if start:
+                    eng.start(start, bfs)
+                live = eng.run(max_steps=2, mode="worklist" if steps % 2 else "persistent")
+                steps += 1
+                done = [l for l in lane_q if not (live >> l) & 1]
+                if done:
+                    assert all((eng.done >> l) & 1 for l in done)
+                    got = eng.fetch(done)
+                    for l in done:
+                        results[lane_q.pop(l)] = got[l]
+                assert steps < 10000
+            announce(f"engine recycling: {len(results)} queries in {steps} rounds")
+            with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+                eng.start({0: g.n})          # source out of range
+    for qi, (s, b) in enumerate(queue):
+        np.testing.assert_array_equal(results[qi], want_for(g, s, b), err_msg=f"query {qi} src {s}")
+
+
+def test_engine_errors(gpu):
+    g = gga.gen_rmat(10, 8, seed=2, wlo=1, whi=10)
+    with gga.DeviceGraph(g) as dg:
+        with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+            dg.multi_sp(np.arange(33, dtype=np.uint32))
+        with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+            dg.multi_sp(np.array([g.n], dtype=np.uint32))
+        with gga.MultiSourceEngine(dg) as eng:
+            eng.start({0: 1})
+            with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+                eng.start({0: 2})            # lane 0 is running
+            with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+                eng.fetch([0])               # not finished
+            assert eng.run() == 0
+            np.testing.assert_array_equal(eng.fetch([0])[0], want_for(g, 1, False))
+        dg.split(mda=4)                      # engines of the old chunk table are stale
