@@ -46,7 +46,7 @@ class SplitStats(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("mode", i32), ("inf", u32), ("max_iters", u32), ("block", u32),
                 ("sync_every", u32), ("use_graph", u32), ("count", u32), ("grid", u32),
-                ("dense_min", u32)]
+                ("dense_min", u32), ("pull_alpha", u32)]
 
 
 class Stats(ctypes.Structure):
@@ -54,7 +54,7 @@ class Stats(ctypes.Structure):
                 ("ktime_ms", ctypes.c_float), ("sktime_ms", ctypes.c_float),
                 ("ctime_ms", ctypes.c_float), ("mode", i32),
                 ("edges_inspected", u64), ("chunks_expanded", u64), ("relaxations", u64),
-                ("m_reach", u64), ("launches", u64)]
+                ("m_reach", u64), ("launches", u64), ("pull_levels", u32)]
 
 
 class VerifyResult(ctypes.Structure):
@@ -111,6 +111,7 @@ SIGNATURES = [
     ("gga_ms_fetch", ctypes.c_int, [vp, u32, vp, ctypes.c_int]),
     ("gga_ms_stats", ctypes.c_int, [vp, P(Stats)]),
     ("gga_multi_sp", ctypes.c_int, [vp, vp, u32, u32, vp, P(Opts), P(Stats)]),
+    ("gga_multi_bfs", ctypes.c_int, [vp, vp, u32, vp, P(Opts), P(Stats)]),
 ]
 
 _LIB = None

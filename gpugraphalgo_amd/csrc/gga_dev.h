@@ -53,9 +53,11 @@ typedef struct gga_dev_work {
  *               128-byte line, so relaxing an edge for every lane touches one
  *               line of dist instead of 32
  *   M[2][n]     lane masks: M[s & 1][v] = lanes whose distance at v dropped in
- *               step s - 1 (step s expands exactly those lanes from v)
- *   q0, q1      chunk lists of the vertices with a non-zero mask
- *   tiles[2 * GGA_TILES(n)]  compaction counts / offsets, then per-tile lane ORs */
+ *               step s - 1 (step s expands exactly those lanes from v); a
+ *               vertex is listed for step s iff its mask is non-zero, and the
+ *               finish pass of step s clears the masks it consumed
+ *   q0, q1      chunk lists (step s reads q[s & 1])
+ *   tiles[GGA_TILES(n) + 1]  compaction counts / offsets */
 #ifndef GGA_MS_LANES
 #define GGA_MS_LANES 32   /* same value as include/gga.h */
 #endif
@@ -71,10 +73,40 @@ typedef struct gga_dev_ms {
     uint32_t inf;
     uint32_t qcap;
     uint32_t unit_lanes; /* lanes that relax with weight 1 (BFS levels) */
-    uint32_t pad;
+    uint32_t dense_min;  /* a step whose list holds >= dense_min chunks rebuilds the next list
+                            by compaction of the masks; smaller ones push */
 } gga_dev_ms;
 typedef struct gga_ms_sources { uint32_t v[GGA_MS_LANES]; } gga_ms_sources;
 typedef struct gga_ms_outputs { uint32_t *p[GGA_MS_LANES]; } gga_ms_outputs;
+
+/* Multi-source BFS with bitmask lanes (kernels/msbfs.hip): up to 256 BFS share
+ * every level; a vertex holds 4 x 64-bit words per bitmap.
+ *   seen / front / next [n * 4]  visited lanes, lanes discovered by the last
+ *                                level, lanes reached in the current level
+ *   lev[k * n]   levels, lane-major (row i = source i), written once per
+ *                (vertex, lane): at discovery or, unreached, inf at the end
+ *   q            push list (forward chunk ids), tiles: its compaction
+ *   part[grid * 4]  per-workgroup level partials (frontier vertices,
+ *                frontier out-edges, newly reached pairs, their out-edges) */
+#define GGA_MSBFS_MAX 256u
+typedef struct gga_dev_msbfs {
+    uint64_t *seen;
+    uint64_t *front;
+    uint64_t *next;
+    uint32_t *lev;
+    uint32_t *q;
+    uint32_t *tiles;
+    uint32_t *ctl;       /* GGA_CTL_WORDS words */
+    uint64_t *part;
+    uint32_t k;          /* lanes in use */
+    uint32_t inf;
+    uint32_t qcap;
+    uint32_t alpha;      /* pull (bottom-up) level when frontier out-edges * alpha > m; 0: never */
+} gga_dev_msbfs;
+/* multi-source BFS control words (within the ctl line of CHANGED / STEPS) */
+#define GGA_CTL_DIR       (7u * GGA_CTL_LINE + 16u)   /* 0: next level pushes, 1: pulls */
+#define GGA_CTL_DONE      (7u * GGA_CTL_LINE + 17u)   /* frontier empty */
+#define GGA_CTL_NPULL     (7u * GGA_CTL_LINE + 18u)   /* levels run bottom-up */
 
 /* Device-scope atomics are performed past the per-XCD L2 and same-address
  * ones serialize (about 16 ns each on MI355X), so nothing hot shares one

@@ -33,6 +33,10 @@ int ggak_split_scan(const uint32_t *row, uint32_t n, uint32_t mda, uint32_t *cst
 int ggak_split_fill(const uint32_t *row, const uint32_t *cstart, uint32_t n, uint32_t m,
                     uint32_t *cbeg, uint32_t *cown, uint32_t *d_maxchunk, hipStream_t s);
 
+/* out[len + 1] = exclusive scan of in[len] (out must not alias in); d_tmp needs
+ * ggak_split_tmp_words(len) words. */
+int ggak_scan_u32(const uint32_t *in, uint32_t len, uint32_t *out, uint32_t *d_tmp, hipStream_t s);
+
 /* ---- traversal (relax.hip) ---- */
 int ggak_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t s);
 int ggak_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src, uint32_t epoch, hipStream_t s);
@@ -74,6 +78,18 @@ int ggak_ms_harvest(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, gga_ms_outpu
 int ggak_ms_reach(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, hipStream_t s);
 /* Striped work counters -> ctl[CNT] (3 x uint64), reach partials -> ctl[RED]. */
 int ggak_ms_fold(gga_dev_ms ms, hipStream_t s);
+
+/* ---- multi-source BFS with bitmask lanes (msbfs.hip) */
+/* indeg[col[e]] += 1 for every edge (indeg zeroed first). */
+int ggak_indeg(const uint32_t *col, uint32_t m, uint32_t *indeg, hipStream_t s);
+/* tcol[pos[v]++] = u for every edge (u, v) of the split graph g. */
+int ggak_transpose_fill(gga_dev_graph g, uint32_t *pos, uint32_t *tcol, hipStream_t s);
+uint32_t ggak_msbfs_grid(int device);
+/* Clears the bitmaps, seeds the k sources (d_src, device), builds level 0's push list. */
+int ggak_msbfs_init(gga_dev_graph g, gga_dev_msbfs b, const uint32_t *d_src, hipStream_t s);
+/* All levels in one cooperative launch (t: transposed graph with its chunk table). */
+int ggak_msbfs_run(gga_dev_graph g, gga_dev_graph t, gga_dev_msbfs b, uint32_t max_levels, uint32_t grid,
+                   uint32_t spin_limit_ms, hipStream_t s);
 
 #ifdef __cplusplus
 }

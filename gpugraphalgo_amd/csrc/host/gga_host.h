@@ -22,6 +22,9 @@ typedef struct gga_graph_view_t {
 int gga_graph_view(gga_graph *g, gga_graph_view_t *v);
 /* The graph's cached engine for gga_multi_sp (freed with the graph or a re-split). */
 gga_ms **gga_graph_ms_slot(gga_graph *g);
+/* State of gga_multi_bfs (msbfs.c), owned by the graph. */
+void **gga_graph_msbfs_slot(gga_graph *g);
+void gga_msbfs_release(void *state);
 
 /* thread-local detail for gga_last_error() */
 void gga_set_error(const char *fmt, ...);

@@ -40,6 +40,7 @@ struct gga_graph {
     uint32_t p_grid[2][2];
     uint32_t split_gen;       /* bumped by every split; engines built on an older table are stale */
     gga_ms *ms_cache;         /* engine of gga_multi_sp */
+    void *msbfs;              /* transposed graph + state of gga_multi_bfs (msbfs.c) */
     hipEvent_t ev[4];
     hipEvent_t *kev;          /* per-launch timing events */
     uint32_t kev_cap;
@@ -159,6 +160,7 @@ int gga_graph_free(gga_graph *g)
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
     if (g->ms_cache) gga_ms_free(g->ms_cache);
+    gga_msbfs_release(g->msbfs);
     dfree(g, (void **)&g->d_row, 4ull * ((uint64_t)g->n + 1));
     dfree(g, (void **)&g->d_col, 4ull * g->m);
     dfree(g, (void **)&g->d_w, 4ull * g->m);
@@ -449,6 +451,7 @@ int gga_graph_view(gga_graph *g, gga_graph_view_t *v)
 }
 
 gga_ms **gga_graph_ms_slot(gga_graph *g) { return &g->ms_cache; }
+void **gga_graph_msbfs_slot(gga_graph *g) { return &g->msbfs; }
 
 /* -------------------------------------------------------------- traversal */
 void gga_opts_init(gga_opts *o)

@@ -26,8 +26,9 @@ struct gga_ms {
     uint32_t *h_pin;          /* GGA_CTL_WORDS */
     uint32_t *d_stage;        /* 32 x n staging for host fetches (lazy) */
     hipGraphExec_t gx[6];     /* six-step groups captured per phase */
-    uint32_t gx_grid[6];
+    uint32_t gx_grid[6], gx_dense[6];
     int gx_count[6];
+    uint32_t dense_default;
     uint64_t dev_bytes;
     hipEvent_t ev[3];         /* run timing; [2] marks gga_multi_sp's start */
     float ktime_ms;
@@ -116,6 +117,8 @@ int gga_ms_create(gga_graph *g, uint32_t inf, gga_ms **out)
     e->ntiles = GGA_TILES(dg->n);
     e->d.inf = inf ? inf : GGA_INF;
     e->d.qcap = dg->C;
+    e->dense_default = e->n / 64u > 4096u ? e->n / 64u : 4096u;
+    e->d.dense_min = e->dense_default;
     HIP_TRY(hipSetDevice(e->v.device));
     for (int i = 0; i < 3; i++) HIP_TRY(hipEventCreate(&e->ev[i]));
     HIP_TRY(hipHostMalloc((void **)&e->h_pin, 4ull * GGA_CTL_WORDS, hipHostMallocDefault));
@@ -125,7 +128,7 @@ int gga_ms_create(gga_graph *g, uint32_t inf, gga_ms **out)
     if ((rc = ms_alloc(e, (void **)&e->d.q0, 4ull * dg->C))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.q1, 4ull * dg->C))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.ctl, 4ull * GGA_CTL_WORDS))) goto fail;
-    if ((rc = ms_alloc(e, (void **)&e->d.tiles, 4ull * (2ull * e->ntiles + 2)))) goto fail;
+    if ((rc = ms_alloc(e, (void **)&e->d.tiles, 4ull * (e->ntiles + 1ull)))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.cnt, 32ull * GGA_CNT_SLOTS))) goto fail;
     K_TRY(ggak_fill_u32(e->d.D, e->d.inf, (uint64_t)GGA_MS_LANES * e->n, e->v.stream));
     K_TRY(ggak_fill_u32(e->d.M0, 0u, e->n, e->v.stream));
@@ -194,7 +197,8 @@ static uint32_t ms_grid(gga_ms *e, int persistent, int count)
 static int ms_graph(gga_ms *e, uint32_t p, int count, uint32_t grid)
 {
     int rc = GGA_OK;
-    if (e->gx[p] && e->gx_grid[p] == grid && e->gx_count[p] == count) return GGA_OK;
+    if (e->gx[p] && e->gx_grid[p] == grid && e->gx_count[p] == count && e->gx_dense[p] == e->d.dense_min)
+        return GGA_OK;
     if (e->gx[p]) {
         hipGraphExecDestroy(e->gx[p]);
         e->gx[p] = NULL;
@@ -215,6 +219,7 @@ static int ms_graph(gga_ms *e, uint32_t p, int count, uint32_t grid)
     hipGraphDestroy(gr);
     e->gx_grid[p] = grid;
     e->gx_count[p] = count;
+    e->gx_dense[p] = e->d.dense_min;
     return GGA_OK;
 fail:
     if (gr) hipGraphDestroy(gr);
@@ -259,6 +264,7 @@ int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live_
     const int count = oo.count ? 1 : 0;
     const int persistent = oo.mode == GGA_MODE_PERSISTENT;
     const uint32_t cap = max_steps ? max_steps : e->n + 1u;
+    e->d.dense_min = oo.dense_min ? oo.dense_min : e->dense_default;
     uint32_t live = 0, tail = 0, ran = 0;
     float t = 0.f;
     HIP_TRY(hipSetDevice(e->v.device));
@@ -391,7 +397,9 @@ int gga_multi_sp(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t unit_l
     gga_clear_error();
     if (o) oo = *o;
     else gga_opts_init(&oo);
-    if (oo.mode != GGA_MODE_WORKLIST) oo.mode = GGA_MODE_PERSISTENT;
+    /* the step kernels run at full occupancy; the cooperative launch is
+     * capped at 2 workgroups per CU, too few waves to hide the gathers */
+    if (oo.mode != GGA_MODE_PERSISTENT) oo.mode = GGA_MODE_WORKLIST;
     const uint32_t inf = oo.inf ? oo.inf : GGA_INF;
     gga_ms **slot = gga_graph_ms_slot(g);
     if (*slot && ((*slot)->d.inf != inf || (*slot)->running)) {

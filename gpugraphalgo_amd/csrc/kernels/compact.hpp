@@ -25,6 +25,38 @@ __device__ __forceinline__ void queue_store(uint32_t *qout, uint32_t pos, uint32
                                __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Append the wave's staged vertices to the chunk worklist: every staged vertex
+// contributes its chunk range [cstart[v], cstart[v + 1]).
+__device__ __forceinline__ void flush_pushes(const gga_dev_graph &g, const uint32_t *pbuf,
+                                             uint32_t pcount, uint32_t *qout, uint32_t *tail,
+                                             uint32_t qcap, uint32_t *ctl) {
+    const uint32_t l = lane_id();
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i0 = 0; i0 < pcount; i0 += GGA_WAVE) {
+        const uint32_t idx = i0 + l;
+        uint32_t cs = 0, nch = 0;
+        if (idx < pcount) {
+            const uint32_t v = pbuf[idx];
+            cs = g.cstart[v];
+            nch = g.cstart[v + 1] - cs;
+        }
+        const uint32_t incl = wave_incl_scan(nch);
+        const uint32_t tot = wave_bcast(incl, GGA_WAVE - 1);
+        uint32_t base = 0;
+        if (l == 0) base = atomic_add_u32(tail, tot);
+        base = wave_bcast(base, 0);
+        const uint32_t excl = incl - nch;
+        for (uint32_t b = 0; b < tot; b += GGA_WAVE) {
+            const uint32_t k = b + l;
+            const uint32_t kk = k < tot ? k : tot - 1;
+            const uint32_t j = wave_owner(incl, kk);
+            const uint32_t id = wave_bcast(cs, j) + (kk - wave_bcast(excl, j));
+            if (k < tot) queue_store(qout, base + k, id, qcap, ctl);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 struct ListOut {
     const uint32_t *cstart;  // first chunk of each vertex (n + 1)
     uint32_t *q;             // output list

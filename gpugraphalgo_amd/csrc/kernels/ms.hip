@@ -12,9 +12,13 @@
 // relaxes one edge for all 32 lanes: lane s of the half reads dist[u][s] and
 // dist[v][s] (both one coalesced line), col[e] / w[e] are broadcast loads,
 // and the lanes that improved are published with one atomicOr into the
-// destination's next mask.  Lists are rebuilt each step by the shared
-// compaction passes from the next masks (flag steps only: a vertex enters the
-// list once however many lanes reached it).
+// destination's next mask.  The next list, chosen per step from the length of
+// the current one: a small list pushes each vertex whose next mask was empty
+// (the atomicOr's old value says which improver is first, so a vertex is
+// listed once however many lanes reach it); a large list lets the compaction
+// passes rebuild it from the non-zero next masks.  A finish pass then clears
+// the masks the step consumed and ORs the next list's masks into its live-lane
+// word.
 #include "../gga_kernels.h"
 #include "compact.hpp"
 #include "sync.hpp"
@@ -31,6 +35,7 @@ constexpr uint32_t LANES = GGA_MS_LANES;
 #define GGA_MS_UNROLL 4
 #endif
 constexpr uint32_t MS_UNROLL = GGA_MS_UNROLL;  // edges per half wave per round
+constexpr uint32_t PBUF = 256;                 // pushed vertices staged per wave before a flush
 
 struct Counters {
     uint64_t edges = 0, chunks = 0, relax = 0;
@@ -84,9 +89,15 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t x) {
 }
 
 // ---------------------------------------------------------------- relax
+__device__ __forceinline__ bool ms_flag_step(const gga_dev_ms &ms, uint32_t nq) { return nq >= ms.dense_min; }
+
 template <bool COUNT>
 __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
-                                         uint32_t nq, Counters &cnt) {
+                                         uint32_t nq, uint32_t *pbuf, Counters &cnt) {
+    const bool push = !ms_flag_step(ms, nq);
+    uint32_t *qout = (step & 1u) ? ms.q0 : ms.q1;
+    uint32_t *tail = &ms.ctl[GGA_CTL_TAIL((step + 1u) % 3u)];
+    uint32_t pcount = 0;
     const uint32_t *qin = ms_qin(ms, step);
     const uint32_t *mcur = ms_mcur(ms, step);
     uint32_t *mnxt = ms_mnxt(ms, step);
@@ -145,80 +156,68 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                 }
                 const uint64_t bal = __ballot(imp);
                 const uint32_t hm = (uint32_t)(bal >> (32u * half));
+                bool first = false;
                 if (s == 0 && hm)
-                    __hip_atomic_fetch_or(&mnxt[v[i]], hm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    first = __hip_atomic_fetch_or(&mnxt[v[i]], hm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
                 if (COUNT) {
                     cnt.edges += (uint32_t)__popcll(__ballot(on));
                     cnt.relax += (uint32_t)__popcll(bal);
                 }
+                if (push) {
+                    // the improver that found the next mask empty lists the vertex
+                    const uint64_t pm = __ballot(first);
+                    if (pm) {
+                        const uint32_t np = (uint32_t)__popcll(pm);
+                        if (pcount + np > PBUF) {
+                            flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
+                            pcount = 0;
+                        }
+                        if (first) {
+                            const uint32_t pos = pcount + __builtin_amdgcn_mbcnt_hi(
+                                                              (uint32_t)(pm >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+                            pbuf[pos] = v[i];
+                        }
+                        pcount += np;
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
             }
         }
     }
+    if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
 }
 
-// ------------------------------------------------------- compaction passes
-// count: chunk ids per tile and the OR of the tile's next masks
-__device__ __forceinline__ void ms_count_tile(const gga_dev_graph &g, const gga_dev_ms &ms, const ListOut &o,
-                                              const uint32_t *mnxt, uint32_t t, uint32_t *s_wave,
-                                              uint32_t *s_or) {
-    compact_count_tile(o, g.n, t, MaskMarked{mnxt}, s_wave);
+// ------------------------------------------------------- finish pass
+// Clears the masks step `step` consumed (owners of its list) and ORs the next
+// list's masks into LANES((step + 1) % 3); clears the slot the step after
+// next fills (its length and lanes), which held the list of step - 1.
+__device__ __forceinline__ void ms_finish(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
+                                          uint32_t *s_or) {
+    const uint32_t nin = ms_len(ms, step), nout = ms_len(ms, step + 1u);
+    const uint32_t *qin = ms_qin(ms, step), *qout = ms_qin(ms, step + 1u);
+    uint32_t *mcur = ms_mcur(ms, step);
+    const uint32_t *mnxt = ms_mnxt(ms, step);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = tid; i < nin; i += stride) mcur[g.cown[qin[i]]] = 0u;
     uint32_t x = 0;
-    const uint64_t v0 = (uint64_t)t * GGA_TILE;
-#pragma unroll
-    for (uint32_t i = 0; i < CITEMS; i++) {
-        const uint64_t v = v0 + (uint64_t)i * CBLOCK + threadIdx.x;
-        if (v < g.n) x |= mnxt[v];
-    }
+    for (uint64_t i = tid; i < nout; i += stride) x |= mnxt[g.cown[qout[i]]];
     x = wave_or(x);
     if (lane_id() == 0) s_or[threadIdx.x / GGA_WAVE] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t y = 0;
-        for (uint32_t i = 0; i < WAVES; i++) y |= s_or[i];
-        ms.tiles[GGA_TILES(g.n) + t] = y;
-    }
-    __syncthreads();
-}
-
-// scan (one workgroup): list offsets, list length and lane mask of the next slot
-__device__ __forceinline__ void ms_scan(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
-                                        uint32_t nq, uint32_t *s_wave, uint32_t *s_or) {
-    const uint32_t sout = (step + 1u) % 3u;
-    if (nq == 0) {
-        if (threadIdx.x == 0) {
-            ms.ctl[GGA_CTL_TAIL(sout)] = 0u;
-            ms.ctl[GGA_CTL_LANES(sout)] = 0u;
+        for (uint32_t i = 0; i < blockDim.x / GGA_WAVE; i++) y |= s_or[i];
+        if (y) __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_LANES((step + 1u) % 3u)], y, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        if (blockIdx.x == 0) {
+            ms.ctl[GGA_CTL_TAIL((step + 2u) % 3u)] = 0u;
+            ms.ctl[GGA_CTL_LANES((step + 2u) % 3u)] = 0u;
+            if (nin) ms.ctl[GGA_CTL_STEPS] += 1u;
         }
-        return;
-    }
-    compact_scan(ms_list_out(g, ms, step), g.n, s_wave);
-    const uint32_t ntiles = GGA_TILES(g.n);
-    uint32_t x = 0;
-    for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) x |= ms.tiles[ntiles + i];
-    x = wave_or(x);
-    if (lane_id() == 0) s_or[threadIdx.x / GGA_WAVE] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t y = 0;
-        for (uint32_t i = 0; i < WAVES; i++) y |= s_or[i];
-        ms.ctl[GGA_CTL_LANES(sout)] = y;
-        ms.ctl[GGA_CTL_STEPS] += 1u;
     }
     __syncthreads();
-}
-
-// write: chunk ids of the marked vertices; the consumed masks are cleared so
-// that they can be the next masks of the following step
-__device__ __forceinline__ void ms_write_tile(const gga_dev_graph &g, const gga_dev_ms &ms, const ListOut &o,
-                                              const uint32_t *mnxt, uint32_t *mcur, uint32_t t,
-                                              uint32_t *s_wave) {
-    compact_write_tile(o, g.n, t, MaskMarked{mnxt}, s_wave);
-    const uint64_t v0 = (uint64_t)t * GGA_TILE;
-#pragma unroll
-    for (uint32_t i = 0; i < CITEMS; i++) {
-        const uint64_t v = v0 + (uint64_t)i * CBLOCK + threadIdx.x;
-        if (v < g.n) mcur[v] = 0u;
-    }
 }
 
 // ---------------------------------------------------------------- kernels
@@ -254,34 +253,43 @@ __global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, 
 
 template <bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_pbuf[WAVES][PBUF];
     const uint32_t nq = ms_len(ms, step);
     if (nq == 0) return;
     Counters cnt;
-    ms_relax<COUNT>(g, ms, step, nq, cnt);
+    ms_relax<COUNT>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
     publish_counters<COUNT>(ms, cnt);
 }
 
+// The three compaction passes after a flag step (no-ops after a push step).
 __global__ __launch_bounds__(CBLOCK) void k_ms_count(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE], s_or[CBLOCK / GGA_WAVE];
-    if (ms_len(ms, step) == 0) return;
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    const uint32_t nq = ms_len(ms, step);
+    if (nq == 0 || !ms_flag_step(ms, nq)) return;
     const ListOut o = ms_list_out(g, ms, step);
-    const uint32_t *mnxt = ms_mnxt(ms, step);
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) ms_count_tile(g, ms, o, mnxt, t, s_wave, s_or);
+    const MaskMarked mk{ms_mnxt(ms, step)};
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, mk, s_wave);
 }
 
 __global__ __launch_bounds__(CBLOCK) void k_ms_scan(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE], s_or[CBLOCK / GGA_WAVE];
-    ms_scan(g, ms, step, ms_len(ms, step), s_wave, s_or);
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    const uint32_t nq = ms_len(ms, step);
+    if (nq == 0 || !ms_flag_step(ms, nq)) return;
+    compact_scan(ms_list_out(g, ms, step), g.n, s_wave);
 }
 
 __global__ __launch_bounds__(CBLOCK) void k_ms_write(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
-    if (ms_len(ms, step) == 0) return;
+    const uint32_t nq = ms_len(ms, step);
+    if (nq == 0 || !ms_flag_step(ms, nq)) return;
     const ListOut o = ms_list_out(g, ms, step);
-    const uint32_t *mnxt = ms_mnxt(ms, step);
-    uint32_t *mcur = ms_mcur(ms, step);
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
-        ms_write_tile(g, ms, o, mnxt, mcur, t, s_wave);
+    const MaskMarked mk{ms_mnxt(ms, step)};
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ms_finish(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_or[WAVES];
+    ms_finish(g, ms, step, s_or);
 }
 
 // All steps in one cooperative launch: relax, then the three compaction
@@ -289,6 +297,7 @@ __global__ __launch_bounds__(CBLOCK) void k_ms_write(gga_dev_graph g, gga_dev_ms
 template <bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0,
                                                          uint32_t max_steps, uint64_t spin_ticks) {
+    __shared__ uint32_t s_pbuf[WAVES][PBUF];
     __shared__ uint32_t s_wave[WAVES], s_or[WAVES], s_state[2];
     uint32_t gen = 0;
     if (threadIdx.x == 0)
@@ -305,18 +314,19 @@ __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_de
         const uint32_t nq = s_state[0] < ms.qcap ? s_state[0] : ms.qcap, err = s_state[1];
         __syncthreads();
         if (nq == 0 || err) break;
-        ms_relax<COUNT>(g, ms, step, nq, cnt);
+        ms_relax<COUNT>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
         grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
-        const ListOut o = ms_list_out(g, ms, step);
-        const uint32_t *mnxt = ms_mnxt(ms, step);
-        for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
-            ms_count_tile(g, ms, o, mnxt, t, s_wave, s_or);
-        grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
-        if (blockIdx.x == 0) ms_scan(g, ms, step, nq, s_wave, s_or);
-        grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
-        uint32_t *mcur = ms_mcur(ms, step);
-        for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
-            ms_write_tile(g, ms, o, mnxt, mcur, t, s_wave);
+        if (ms_flag_step(ms, nq)) {
+            const ListOut o = ms_list_out(g, ms, step);
+            const MaskMarked mk{ms_mnxt(ms, step)};
+            for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, mk, s_wave);
+            grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
+            if (blockIdx.x == 0) compact_scan(o, g.n, s_wave);
+            grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
+            for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
+            grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
+        }
+        ms_finish(g, ms, step, s_or);
         grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
     }
     publish_counters<COUNT>(ms, cnt);
@@ -449,6 +459,7 @@ extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int c
     k_ms_count<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
     k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+    k_ms_finish<<<grid_for(g.C, 1024), BLOCK, 0, s>>>(g, ms, step);
     return launch_status();
 }
 
@@ -460,8 +471,9 @@ extern "C" uint32_t ggak_ms_grid(int device, int persistent, int count) {
                                 : (count ? reinterpret_cast<const void *>(&k_ms_step<true>)
                                          : reinterpret_cast<const void *>(&k_ms_step<false>));
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, 0) != hipSuccess || per <= 0) return 0;
-    // one workgroup per CU for the cooperative launch (see ggak_wl_grid)
-    if (persistent) per = 1;
+    // cooperative launch: stay well under the co-residency measured for these
+    // kernels (6 per CU); 2 per CU gives the relax twice the waves of 1 per CU
+    if (persistent) per = per < 2 ? per : 2;
     return (uint32_t)cus * (uint32_t)per;
 }
 

@@ -1,0 +1,415 @@
+// msbfs.hip -- multi-source BFS with bitmask lanes: up to 256 BFS per launch.
+//
+// No reference counterpart (the reference runs one BFS per process,
+// bfs_sssp.cu:203); each lane's levels equal the single-source BFS of the
+// reference (graph.h:150-184 / M_my_drelax with unit weights).  Level-
+// synchronous: at level L every lane that reached a vertex at level L passes
+// it on along every out-edge, so a lane's bit first appears at v at the
+// level equal to its BFS distance.
+//
+//   push level (small frontier): the chunks of frontier vertices (one list,
+//     built by the shared compaction) OR their front bits into next[v] of each
+//     destination, masked by the destination's seen bits; 4 lanes of a wave
+//     per edge, one per 64-bit word.
+//   pull level (large frontier): every in-chunk of the transposed graph whose
+//     owner still misses lanes ORs the front bits of its in-neighbours until
+//     the missing lanes are all found (direction-optimising BFS, Beamer et al.
+//     2012, on bitmaps as in multi-source BFS, Then et al. 2014).
+//   finalize: new = next & ~seen; seen |= new; front = new; levels written for
+//     the new bits (the level loop is uniform over bit positions so that each
+//     store instruction writes consecutive vertices of one lane row).
+// One cooperative launch runs all levels; grid barriers separate the phases.
+#include "../gga_kernels.h"
+#include "compact.hpp"
+#include "sync.hpp"
+#include "wave.hpp"
+
+using namespace gga;
+
+namespace {
+
+constexpr uint32_t BLOCK = 256;
+constexpr uint32_t WAVES = BLOCK / GGA_WAVE;
+constexpr uint32_t WORDS = 4;            // 64-bit words per vertex bitmap
+constexpr uint32_t GROUPS = GGA_WAVE / WORDS;  // edges (or chunks) per wave round
+#ifndef GGA_MSBFS_UNROLL
+#define GGA_MSBFS_UNROLL 4
+#endif
+constexpr uint32_t UNROLL = GGA_MSBFS_UNROLL;
+
+__device__ __forceinline__ void atomic_or_u64(uint64_t *p, uint64_t v) {
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// bits of the lanes in use within word j
+__device__ __forceinline__ uint64_t lane_mask(uint32_t k, uint32_t j) {
+    const uint32_t lo = j * 64u;
+    if (k >= lo + 64u) return ~0ull;
+    if (k <= lo) return 0ull;
+    return (1ull << (k - lo)) - 1ull;
+}
+
+struct FrontMarked {
+    const uint64_t *front;
+    __device__ __forceinline__ bool operator()(uint32_t v) const {
+        const uint64_t *f = front + (uint64_t)v * WORDS;
+        return (f[0] | f[1] | f[2] | f[3]) != 0ull;
+    }
+};
+
+__device__ __forceinline__ ListOut bfs_list(const gga_dev_graph &g, const gga_dev_msbfs &b) {
+    ListOut o;
+    o.cstart = g.cstart;
+    o.q = b.q;
+    o.qcap = b.qcap;
+    o.tiles = b.tiles;
+    o.tail = &b.ctl[GGA_CTL_TAIL(0)];
+    o.ctl = b.ctl;
+    return o;
+}
+
+// ------------------------------------------------------------- push level
+__device__ __forceinline__ void push_level(const gga_dev_graph &g, const gga_dev_msbfs &b, uint32_t nq) {
+    const uint32_t waves = gridDim.x * WAVES;
+    const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
+    uint32_t bsz = nq / (8u * waves);
+    bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
+    const uint32_t nbatch = (nq + bsz - 1) / bsz;
+    const uint32_t l = lane_id(), grp = l / WORDS, j = l % WORDS;
+    for (uint32_t bt = wid; bt < nbatch; bt += waves) {
+        const uint32_t idx = bt * bsz + l;
+        const bool valid = l < bsz && idx < nq;
+        uint32_t eb = 0, ne = 0, u = 0;
+        if (valid) {
+            const uint32_t c = stream_load(&b.q[idx]);
+            u = stream_load(&g.cown[c]);
+            eb = stream_load(&g.cbeg[c]);
+            ne = stream_load(&g.cbeg[c + 1]) - eb;
+        }
+        const uint32_t incl = wave_incl_scan(ne);
+        const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+        const uint32_t excl = incl - ne;
+        for (uint32_t base = 0; base < total; base += GROUPS * UNROLL) {
+            uint32_t v[UNROLL];
+            uint64_t f[UNROLL], sn[UNROLL];
+            bool act[UNROLL];
+#pragma unroll
+            for (uint32_t i = 0; i < UNROLL; i++) {
+                const uint32_t k = base + i * GROUPS + grp;
+                act[i] = k < total;
+                const uint32_t kk = act[i] ? k : total - 1u;
+                const uint32_t o = wave_owner(incl, kk);
+                const uint32_t e = wave_bcast(eb, o) + (kk - wave_bcast(excl, o));
+                const uint32_t uu = wave_bcast(u, o);
+                v[i] = stream_load(&g.col[e]);
+                f[i] = b.front[(uint64_t)uu * WORDS + j];
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < UNROLL; i++) sn[i] = b.seen[(uint64_t)v[i] * WORDS + j];
+#pragma unroll
+            for (uint32_t i = 0; i < UNROLL; i++) {
+                const uint64_t x = f[i] & ~sn[i];
+                if (act[i] && x) atomic_or_u64(&b.next[(uint64_t)v[i] * WORDS + j], x);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------- pull level
+// t: transposed graph with its own chunk table (in-edges of each vertex).
+__device__ __forceinline__ void pull_level(const gga_dev_graph &t, const gga_dev_msbfs &b) {
+    const uint32_t l = lane_id(), grp = l / WORDS, j = l % WORDS;
+    const uint64_t lm = lane_mask(b.k, j);
+    const uint64_t gstride = (uint64_t)gridDim.x * WAVES * GROUPS;
+    for (uint64_t c = ((uint64_t)blockIdx.x * WAVES + threadIdx.x / GGA_WAVE) * GROUPS + grp; c < t.C;
+         c += gstride) {
+        const uint32_t v = t.cown[c];
+        const uint64_t miss = ~b.seen[(uint64_t)v * WORDS + j] & lm;
+        if (!miss) continue;
+        const uint32_t eb = t.cbeg[c], ee = t.cbeg[c + 1];
+        uint64_t acc = 0;
+        uint32_t e = eb;
+        for (; e + UNROLL <= ee && (acc & miss) != miss; e += UNROLL) {
+            uint32_t u[UNROLL];
+#pragma unroll
+            for (uint32_t i = 0; i < UNROLL; i++) u[i] = t.col[e + i];
+#pragma unroll
+            for (uint32_t i = 0; i < UNROLL; i++) acc |= b.front[(uint64_t)u[i] * WORDS + j];
+        }
+        for (; e < ee && (acc & miss) != miss; e++) acc |= b.front[(uint64_t)t.col[e] * WORDS + j];
+        acc &= miss;
+        if (acc) atomic_or_u64(&b.next[(uint64_t)v * WORDS + j], acc);
+    }
+}
+
+// Level values of the set bits of `bits` (word j of vertex v), uniform loop.
+__device__ __forceinline__ void write_levels(const gga_dev_msbfs &b, uint32_t n, uint32_t v, bool valid,
+                                             uint32_t j, uint64_t bits, uint32_t value) {
+    if (!__ballot(valid && bits)) return;
+    for (uint32_t t = 0; t < 64u; t++) {
+        const uint32_t lane = j * 64u + t;
+        if (lane >= b.k) break;
+        if (valid && ((bits >> t) & 1ull)) b.lev[(uint64_t)lane * n + v] = value;
+    }
+}
+
+struct LevelSums {
+    uint64_t nf = 0, mf = 0, reached = 0, mreach = 0;
+};
+
+// finalize one tile; returns the tile's push-list chunk count in tiles[t]
+__device__ __forceinline__ void finalize_tile(const gga_dev_graph &g, const gga_dev_msbfs &b, uint32_t t,
+                                              uint32_t level, LevelSums &ls, uint32_t *s_wave) {
+    const uint64_t v0 = (uint64_t)t * GGA_TILE;
+    uint32_t chunks = 0;
+    for (uint32_t i = 0; i < CITEMS; i++) {
+        const uint64_t vv = v0 + (uint64_t)i * CBLOCK + threadIdx.x;
+        const bool valid = vv < g.n;
+        const uint32_t v = valid ? (uint32_t)vv : 0u;
+        uint64_t any = 0;
+        uint32_t pc = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < WORDS; j++) {
+            uint64_t nw = 0;
+            if (valid) {
+                const uint64_t nx = b.next[(uint64_t)v * WORDS + j];
+                const uint64_t sn = b.seen[(uint64_t)v * WORDS + j];
+                nw = nx & ~sn;
+                if (nx) b.next[(uint64_t)v * WORDS + j] = 0ull;
+                if (nw) b.seen[(uint64_t)v * WORDS + j] = sn | nw;
+                b.front[(uint64_t)v * WORDS + j] = nw;
+            }
+            any |= nw;
+            pc += (uint32_t)__popcll(nw);
+            write_levels(b, g.n, v, valid, j, nw, level + 1u);
+        }
+        if (valid && any) {
+            const uint32_t deg = g.row[v + 1] - g.row[v];
+            ls.nf += 1;
+            ls.mf += deg;
+            ls.reached += pc;
+            ls.mreach += (uint64_t)pc * deg;
+            chunks += g.cstart[v + 1] - g.cstart[v];
+        }
+    }
+    uint32_t excl;
+    const uint32_t tot = block_excl_scan(chunks, excl, s_wave);
+    if (threadIdx.x == 0) b.tiles[t] = tot;
+}
+
+// inf for every (vertex, lane) never reached
+__device__ __forceinline__ void fill_unreached(const gga_dev_graph &g, const gga_dev_msbfs &b) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t nn = ((uint64_t)g.n + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves loop
+    for (uint64_t vv = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; vv < nn; vv += stride) {
+        const bool valid = vv < g.n;
+        const uint32_t v = valid ? (uint32_t)vv : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < WORDS; j++) {
+            const uint64_t miss = valid ? (~b.seen[(uint64_t)v * WORDS + j] & lane_mask(b.k, j)) : 0ull;
+            write_levels(b, g.n, v, valid, j, miss, b.inf);
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t block_sum(uint64_t x, uint64_t *s_red) {
+    x = wave_sum64(x);
+    if (lane_id() == 0) s_red[threadIdx.x / GGA_WAVE] = x;
+    __syncthreads();
+    uint64_t s = 0;
+    for (uint32_t i = 0; i < WAVES; i++) s += s_red[i];
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ __launch_bounds__(BLOCK) void k_msbfs_clear(gga_dev_graph g, gga_dev_msbfs b) {
+    const uint64_t total = (uint64_t)g.n * WORDS;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        b.seen[i] = 0ull;
+        b.front[i] = 0ull;
+        b.next[i] = 0ull;
+    }
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < GGA_CTL_WORDS; i += blockDim.x)
+            if (i < GGA_CTL_BAR || i >= GGA_CTL_BARG(0) + GGA_BAR_GROUPS * GGA_CTL_LINE) b.ctl[i] = 0u;
+}
+
+// lane i starts at src[i]: level 0 there; reached / out-edge totals seeded
+__global__ __launch_bounds__(BLOCK) void k_msbfs_seed(gga_dev_graph g, gga_dev_msbfs b, const uint32_t *src) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.k) return;
+    const uint32_t x = src[i];
+    const uint64_t bit = 1ull << (i % 64u);
+    atomic_or_u64(&b.seen[(uint64_t)x * WORDS + i / 64u], bit);
+    atomic_or_u64(&b.front[(uint64_t)x * WORDS + i / 64u], bit);
+    b.lev[(uint64_t)i * g.n + x] = 0u;
+    uint64_t *tot = reinterpret_cast<uint64_t *>(b.ctl + GGA_CTL_CNT);
+    atomic_add_u64(&tot[0], 1ull);
+    atomic_add_u64(&tot[1], (uint64_t)(g.row[x + 1] - g.row[x]));
+}
+
+__global__ __launch_bounds__(CBLOCK) void k_msbfs_list_count(gga_dev_graph g, gga_dev_msbfs b) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    const ListOut o = bfs_list(g, b);
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
+        compact_count_tile(o, g.n, t, FrontMarked{b.front}, s_wave);
+}
+__global__ __launch_bounds__(CBLOCK) void k_msbfs_list_scan(gga_dev_graph g, gga_dev_msbfs b) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    compact_scan(bfs_list(g, b), g.n, s_wave);
+}
+__global__ __launch_bounds__(CBLOCK) void k_msbfs_list_write(gga_dev_graph g, gga_dev_msbfs b) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    const ListOut o = bfs_list(g, b);
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
+        compact_write_tile(o, g.n, t, FrontMarked{b.front}, s_wave);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_msbfs(gga_dev_graph g, gga_dev_graph t, gga_dev_msbfs b,
+                                                 uint32_t max_levels, uint64_t spin_ticks) {
+    __shared__ uint32_t s_wave[WAVES];
+    __shared__ uint64_t s_red[WAVES];
+    __shared__ uint32_t s_state[3];
+    uint32_t gen = 0;
+    if (threadIdx.x == 0)
+        gen = __hip_atomic_load(&b.ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const ListOut o = bfs_list(g, b);
+    const uint32_t ntiles = GGA_TILES(g.n);
+    uint32_t level = 0;
+    for (; level < max_levels; level++) {
+        if (threadIdx.x == 0) {
+            s_state[0] = b.ctl[GGA_CTL_DIR];
+            s_state[1] = b.ctl[GGA_CTL_TAIL(0)];
+            s_state[2] = __hip_atomic_load(&b.ctl[GGA_CTL_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        const uint32_t dir = s_state[0], nq = s_state[1] < b.qcap ? s_state[1] : b.qcap, err = s_state[2];
+        __syncthreads();
+        if (err) break;
+        if (dir) pull_level(t, b);
+        else push_level(g, b, nq);
+        grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
+        LevelSums ls;
+        for (uint32_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) finalize_tile(g, b, tt, level, ls, s_wave);
+        const uint64_t nf = block_sum(ls.nf, s_red), mf = block_sum(ls.mf, s_red);
+        const uint64_t rs = block_sum(ls.reached, s_red), mr = block_sum(ls.mreach, s_red);
+        if (threadIdx.x == 0) {
+            b.part[4u * blockIdx.x + 0] = nf;
+            b.part[4u * blockIdx.x + 1] = mf;
+            b.part[4u * blockIdx.x + 2] = rs;
+            b.part[4u * blockIdx.x + 3] = mr;
+        }
+        grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
+        if (blockIdx.x == 0) {
+            uint64_t x[4] = {0, 0, 0, 0};
+            for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+                for (uint32_t k = 0; k < 4; k++) x[k] += b.part[4u * i + k];
+            for (uint32_t k = 0; k < 4; k++) x[k] = block_sum(x[k], s_red);
+            const bool pull = b.alpha && x[1] * (uint64_t)b.alpha > (uint64_t)g.m;
+            if (threadIdx.x == 0) {
+                uint64_t *tot = reinterpret_cast<uint64_t *>(b.ctl + GGA_CTL_CNT);
+                tot[0] += x[2];
+                tot[1] += x[3];
+                b.ctl[GGA_CTL_DONE] = x[0] == 0 ? 1u : 0u;
+                b.ctl[GGA_CTL_DIR] = pull ? 1u : 0u;
+                b.ctl[GGA_CTL_STEPS] = level + 1u;
+                if (pull) b.ctl[GGA_CTL_NPULL] += 1u;
+            }
+            if (!pull && x[0]) compact_scan(o, g.n, s_wave);
+        }
+        grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
+        if (threadIdx.x == 0) {
+            s_state[0] = b.ctl[GGA_CTL_DONE];
+            s_state[1] = b.ctl[GGA_CTL_DIR];
+        }
+        __syncthreads();
+        const uint32_t done = s_state[0], pull = s_state[1];
+        __syncthreads();
+        if (done) break;
+        if (!pull) {
+            for (uint32_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x)
+                compact_write_tile(o, g.n, tt, FrontMarked{b.front}, s_wave);
+            grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
+        }
+    }
+    fill_unreached(g, b);
+}
+
+// ------------------------------------------------------ transposed graph
+__global__ __launch_bounds__(BLOCK) void k_indeg(const uint32_t *col, uint32_t m, uint32_t *indeg) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (uint64_t)gridDim.x * blockDim.x)
+        atomic_add_u32(&indeg[col[e]], 1u);
+}
+
+// one thread per forward chunk; the in-edge order within a vertex is arbitrary
+__global__ __launch_bounds__(BLOCK) void k_transpose_fill(gga_dev_graph g, uint32_t *pos, uint32_t *tcol) {
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g.C;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = g.cown[c];
+        for (uint32_t e = g.cbeg[c]; e < g.cbeg[c + 1]; e++) tcol[atomic_add_u32(&pos[g.col[e]], 1u)] = u;
+    }
+}
+
+inline int launch_status() { return (int)hipGetLastError(); }
+
+inline uint32_t grid_for(uint64_t items, uint32_t cap) {
+    uint64_t b = (items + BLOCK - 1) / BLOCK;
+    if (b == 0) b = 1;
+    return (uint32_t)(b < cap ? b : cap);
+}
+
+inline uint32_t compact_grid(uint32_t n) {
+    const uint32_t ntiles = GGA_TILES(n);
+    return ntiles < 2048u ? (ntiles ? ntiles : 1u) : 2048u;
+}
+
+}  // namespace
+
+extern "C" int ggak_indeg(const uint32_t *col, uint32_t m, uint32_t *indeg, hipStream_t s) {
+    if (m) k_indeg<<<grid_for(m, 8192), BLOCK, 0, s>>>(col, m, indeg);
+    return launch_status();
+}
+
+extern "C" int ggak_transpose_fill(gga_dev_graph g, uint32_t *pos, uint32_t *tcol, hipStream_t s) {
+    k_transpose_fill<<<grid_for(g.C, 8192), BLOCK, 0, s>>>(g, pos, tcol);
+    return launch_status();
+}
+
+extern "C" uint32_t ggak_msbfs_grid(int device) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_msbfs), BLOCK, 0) !=
+            hipSuccess ||
+        per <= 0)
+        return 0;
+    return (uint32_t)cus;  // one workgroup per CU for the cooperative launch
+}
+
+extern "C" int ggak_msbfs_init(gga_dev_graph g, gga_dev_msbfs b, const uint32_t *d_src, hipStream_t s) {
+    if (b.k == 0 || b.k > GGA_MSBFS_MAX) return (int)hipErrorInvalidValue;
+    k_msbfs_clear<<<grid_for((uint64_t)g.n * WORDS, 8192), BLOCK, 0, s>>>(g, b);
+    k_msbfs_seed<<<(b.k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(g, b, d_src);
+    const uint32_t cg = compact_grid(g.n);
+    k_msbfs_list_count<<<cg, CBLOCK, 0, s>>>(g, b);
+    k_msbfs_list_scan<<<1, CBLOCK, 0, s>>>(g, b);
+    k_msbfs_list_write<<<cg, CBLOCK, 0, s>>>(g, b);
+    return launch_status();
+}
+
+extern "C" int ggak_msbfs_run(gga_dev_graph g, gga_dev_graph t, gga_dev_msbfs b, uint32_t max_levels,
+                              uint32_t grid, uint32_t spin_limit_ms, hipStream_t s) {
+    if (grid == 0) return (int)hipErrorInvalidValue;
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;
+    uint64_t spin_ticks = (uint64_t)spin_limit_ms * (uint64_t)khz;
+    void *args[] = {&g, &t, &b, &max_levels, &spin_ticks};
+    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_msbfs), dim3(grid), dim3(BLOCK),
+                                              args, 0, s);
+    if (e != hipSuccess) return (int)e;
+    return launch_status();
+}

@@ -172,9 +172,64 @@ __global__ __launch_bounds__(256) void k_split_fill(const uint32_t *__restrict__
     if (v == (uint64_t)n - 1) cbeg[cs + nch] = m;
 }
 
+// ---- exclusive scan of a uint32 array (out must not alias in), same three
+// passes as the chunk-count scan
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce_arr(const uint32_t *__restrict__ in,
+                                                                  uint32_t len, uint32_t *tile_sum) {
+    __shared__ uint32_t s_wave[SCAN_THREADS / GGA_WAVE];
+    const uint64_t tile0 = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < SCAN_ITEMS; i++) {
+        const uint64_t v = tile0 + (uint64_t)i * SCAN_THREADS + threadIdx.x;
+        if (v < len) s += in[v];
+    }
+    uint32_t excl;
+    const uint32_t tot = block_excl_scan(s, excl, s_wave);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down_arr(const uint32_t *__restrict__ in,
+                                                                uint32_t len,
+                                                                const uint32_t *__restrict__ tile_off,
+                                                                uint32_t ntiles,
+                                                                uint32_t *__restrict__ out) {
+    __shared__ uint32_t s_wave[SCAN_THREADS / GGA_WAVE];
+    const uint64_t v0 = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t c[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < SCAN_ITEMS; i++) {
+        const uint64_t v = v0 + i;
+        c[i] = v < len ? in[v] : 0u;
+        s += c[i];
+    }
+    uint32_t excl;
+    block_excl_scan(s, excl, s_wave);
+    uint32_t off = tile_off[blockIdx.x] + excl;
+#pragma unroll
+    for (uint32_t i = 0; i < SCAN_ITEMS; i++) {
+        const uint64_t v = v0 + i;
+        if (v < len) out[v] = off;
+        off += c[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[len] = tile_off[ntiles];
+}
+
 inline int launch_status() { return (int)hipGetLastError(); }
 
 }  // namespace
+
+extern "C" int ggak_scan_u32(const uint32_t *in, uint32_t len, uint32_t *out, uint32_t *d_tmp, hipStream_t s) {
+    const uint32_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+    if (ntiles) k_scan_reduce_arr<<<ntiles, SCAN_THREADS, 0, s>>>(in, len, d_tmp);
+    k_split_scan_top<<<1, 1024, 0, s>>>(d_tmp, ntiles);
+    if (ntiles) k_scan_down_arr<<<ntiles, SCAN_THREADS, 0, s>>>(in, len, d_tmp, ntiles, out);
+    else {
+        hipError_t e = hipMemcpyAsync(out, d_tmp, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return (int)e;
+    }
+    return launch_status();
+}
 
 extern "C" int ggak_degree_max(const uint32_t *row, uint32_t n, uint64_t *d_key, hipStream_t s) {
     if (n == 0) return 0;

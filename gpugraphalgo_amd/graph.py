@@ -239,7 +239,7 @@ class DeviceGraph:
         return {"nerr": r.nerr, "untight": r.untight, "src_ok": bool(r.src_ok),
                 "reached": r.reached}
 
-    def multi_sp(self, srcs, bfs_lanes=0, mode: str = "persistent", inf: int = INF,
+    def multi_sp(self, srcs, bfs_lanes=0, mode: str = "auto", inf: int = INF,
                  count: bool = False, keep_on_device: bool = False, **kw):
         """gga_multi_sp: up to 32 sources in one traversal.  bfs_lanes: bit i set
         -> source i is a BFS (hop counts).  Returns (k x n array or None, stats)."""
@@ -250,6 +250,18 @@ class DeviceGraph:
         out = None if keep_on_device else np.empty((k, self.n), dtype=_u32)
         check(load().gga_multi_sp(self._h, _ptr(srcs), k, int(bfs_lanes), _ptr(out), ctypes.byref(o),
                                   ctypes.byref(st)), "gga_multi_sp")
+        return out, _stats_dict(st)
+
+    def multi_bfs(self, srcs, inf: int = INF, keep_on_device: bool = False, **kw):
+        """gga_multi_bfs: up to 256 BFS in one launch (bitmask lanes).
+        Returns (k x n levels or None, stats)."""
+        srcs = np.ascontiguousarray(srcs, dtype=_u32)
+        k = int(srcs.size)
+        o = _opts("persistent", inf, False, kw)
+        st = _lib.Stats()
+        out = None if keep_on_device else np.empty((k, self.n), dtype=_u32)
+        check(load().gga_multi_bfs(self._h, _ptr(srcs), k, _ptr(out), ctypes.byref(o), ctypes.byref(st)),
+              "gga_multi_bfs")
         return out, _stats_dict(st)
 
 

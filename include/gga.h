@@ -165,6 +165,8 @@ typedef struct gga_opts {
     uint32_t dense_min;   /* WORKLIST / PERSISTENT: a step whose list holds >= dense_min chunks
                              records the next frontier as step stamps + compaction instead of
                              queue pushes; 0 -> max(4096, n / 64); 0xFFFFFFFF -> never */
+    uint32_t pull_alpha;  /* gga_multi_bfs: a level runs bottom-up (pull over in-edges) when the
+                             frontier's out-edges x pull_alpha > m; 0 -> 14; 0xFFFFFFFF -> never */
 } gga_opts;
 /* Fills defaults (all zero = defaults). */
 void gga_opts_init(gga_opts *o);
@@ -182,6 +184,7 @@ typedef struct gga_stats {
     uint64_t relaxations;     /* R: successful atomicMin (opts.count) */
     uint64_t m_reach;         /* sum of out-degrees of reached vertices */
     uint64_t launches;        /* relax-kernel launches */
+    uint32_t pull_levels;     /* gga_multi_bfs: levels run bottom-up */
 } gga_stats;
 
 /* BFS levels from src (replaces the na_bfs path: dist init bfs_sssp.cu:302-321,
@@ -266,10 +269,22 @@ int gga_ms_stats(const gga_ms *e, gga_stats *st);
 /* k <= 32 sources in one traversal (config: batched multi-source SSSP / BFS):
  * out = k x n row-major (row i = source srcs[i]) or NULL to keep the results on
  * the device; lanes in unit_lanes are BFS.  o->mode GGA_MODE_WORKLIST or
- * GGA_MODE_PERSISTENT (AUTO -> PERSISTENT).  st: iterations = steps, time_ms =
+ * GGA_MODE_PERSISTENT (AUTO -> WORKLIST).  st: iterations = steps, time_ms =
  * device time start -> convergence, reached / m_reach summed over sources. */
 int gga_multi_sp(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t unit_lanes, uint32_t *out,
                  const gga_opts *o, gga_stats *st);
+
+/* Multi-source BFS with bitmask lanes: k <= 256 sources in one cooperative
+ * launch (per vertex 4 x 64-bit visited / frontier / next words; levels pushed
+ * top-down from small frontiers and pulled bottom-up over in-edges when the
+ * frontier is large, see opts.pull_alpha).  out = k x n uint32 levels
+ * row-major (row i = source srcs[i]), unreached = o->inf; NULL keeps them on
+ * the device.  Builds and caches the transposed graph on first use.
+ * st: iterations = levels, reached (clamped to uint32) / m_reach summed over
+ * sources, time_ms = device time init -> all levels written. */
+#define GGA_MSBFS_LANES 256
+int gga_multi_bfs(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *out, const gga_opts *o,
+                  gga_stats *st);
 
 #ifdef __cplusplus
 }

@@ -37,17 +37,22 @@ def nonisolated(g, k, seed):
     return np.random.default_rng(seed).choice(cand, size=k, replace=False).astype(np.uint32)
 
 
+# next-list building: default switch, compaction every step, pushes every step
+DENSE = {"auto": 0, "flags": 1, "queue": 0xFFFFFFFF}
+
+
+@pytest.mark.parametrize("rep", list(DENSE))
 @pytest.mark.parametrize("mode", ["worklist", "persistent"])
 @pytest.mark.parametrize("T", [1, 16, 256])
-def test_multi_sp_rmat_each_lane_equals_oracle(gpu, mode, T):
+def test_multi_sp_rmat_each_lane_equals_oracle(gpu, mode, T, rep):
     g = gga.gen_rmat(14, 16, seed=13, wlo=1, whi=100)
     srcs = nonisolated(g, 32, T)
     srcs[5] = srcs[4]                    # duplicate source in two lanes
     bfs_lanes = 0xA5A5A5A5
     with gga.DeviceGraph(g) as dg:
         dg.split(mda=T)
-        announce(f"multi_sp rmat14 T={T} mode={mode}")
-        out, st = dg.multi_sp(srcs, bfs_lanes=bfs_lanes, mode=mode, count=True)
+        announce(f"multi_sp rmat14 T={T} mode={mode} lists={rep}")
+        out, st = dg.multi_sp(srcs, bfs_lanes=bfs_lanes, mode=mode, count=True, dense_min=DENSE[rep])
         reached = 0
         mreach = 0
         for i, s in enumerate(srcs):
@@ -86,7 +91,8 @@ def test_multi_sp_hipgraph_replay_and_ref_inf(gpu):
         dg.split(mda=4)
         for rep in range(3):
             announce(f"multi_sp grid60x70 hipgraph replay {rep}")
-            out, st = dg.multi_sp(srcs, bfs_lanes=0xFFFF0000, mode="worklist", use_graph=1)
+            out, st = dg.multi_sp(srcs, bfs_lanes=0xFFFF0000, mode="worklist", use_graph=1,
+                                  dense_min=list(DENSE.values())[rep])
             for i, s in enumerate(srcs):
                 np.testing.assert_array_equal(out[i], want_for(g, s, i >= 16), err_msg=f"lane {i}")
         announce("multi_sp grid60x70 inf=1e9")
@@ -126,7 +132,8 @@ def test_engine_lanes_recycled_while_others_run(gpu):
                         bfs.add(l)
                 if start:
                     eng.start(start, bfs)
-                live = eng.run(max_steps=2, mode="worklist" if steps % 2 else "persistent")
+                live = eng.run(max_steps=2, mode="worklist" if steps % 2 else "persistent",
+                               dense_min=list(DENSE.values())[steps % 3])
                 steps += 1
                 done = [l for l in lane_q if not (live >> l) & 1]
                 if done:
@@ -158,3 +165,53 @@ def test_engine_errors(gpu):
             assert eng.run() == 0
             np.testing.assert_array_equal(eng.fetch([0])[0], want_for(g, 1, False))
         dg.split(mda=4)                      # engines of the old chunk table are stale
+
+
+@pytest.mark.parametrize("alpha", [0, 14, 1])
+def test_multi_bfs_256_lanes_equal_oracle(gpu, alpha):
+    """256 bitmask lanes, push-only (alpha 0xFFFFFFFF), the default switch, and
+    pull from the first large level on; duplicate sources, k not a multiple of 64."""
+    g = gga.gen_rmat(13, 16, seed=13, wlo=1, whi=100)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=32)
+        for k in (256, 77, 1):
+            srcs = nonisolated(g, k, k)
+            if k > 3:
+                srcs[3] = srcs[2]
+            pa = 0xFFFFFFFF if alpha == 0 else alpha
+            announce(f"multi_bfs rmat13 k={k} pull_alpha={pa}")
+            out, st = dg.multi_bfs(srcs, pull_alpha=pa)
+            reached = 0
+            mreach = 0
+            for i, s in enumerate(srcs):
+                want = want_for(g, s, True)
+                np.testing.assert_array_equal(out[i], want, err_msg=f"lane {i} src {s}")
+                reached += int((want != INF).sum())
+                mreach += oracle.m_reach(g.row_ptr, want)
+            assert st["reached"] == reached and st["m_reach"] == mreach
+            if alpha == 0:
+                assert st["pull_levels"] == 0
+
+
+def test_multi_bfs_directed_grid_and_isolated(gpu):
+    """Directed graph (pull must use in-edges), an isolated source, REF_INF."""
+    rp = np.array([0, 2, 3, 3, 4, 4], dtype=np.uint32)     # 0->1,0->2  1->3  3->1  4 isolated
+    dst = np.array([1, 2, 3, 1], dtype=np.uint32)
+    g = gga.csr_from_arrays(rp, dst, None)
+    with gga.DeviceGraph(g) as dg:
+        for pa in (0xFFFFFFFF, 1):
+            out, _ = dg.multi_bfs(np.array([0, 4, 3, 2], dtype=np.uint32), inf=gga.REF_INF, pull_alpha=pa)
+            for i, s in enumerate([0, 4, 3, 2]):
+                want = want_for(g, s, True)
+                want = np.where(want == INF, gga.REF_INF, want).astype(np.uint32)
+                np.testing.assert_array_equal(out[i], want)
+    g = gga.gen_grid(40, 50, 0.01, seed=11, wlo=1, whi=1000)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=2)
+        srcs = np.arange(0, 2000, 8, dtype=np.uint32)
+        announce("multi_bfs grid40x50 k=250")
+        out, st = dg.multi_bfs(srcs)
+        for i, s in enumerate(srcs):
+            np.testing.assert_array_equal(out[i], want_for(g, s, True), err_msg=f"lane {i}")
+        with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+            dg.multi_bfs(np.zeros(257, dtype=np.uint32))
