@@ -4,8 +4,9 @@ The product is libgga.so: a C11 host layer plus hand-written gfx950 HIP
 kernels behind the C-ABI of include/gga.h.  This package is its Python view
 (ctypes); it holds no traversal logic of its own.
 """
+from .serve import Server, latency_summary, poisson_trace
 from .graph import (INF, REF_INF, CsrGraph, DeviceGraph, MultiSourceEngine, csr_from_arrays, device_count,
                     gen_grid, gen_rmat, read_gr, verify, write_dist, write_gr, write_result_row)
 
-__all__ = ["CsrGraph", "DeviceGraph", "MultiSourceEngine", "csr_from_arrays", "device_count", "gen_grid", "gen_rmat",
+__all__ = ["Server", "latency_summary", "poisson_trace", "CsrGraph", "DeviceGraph", "MultiSourceEngine", "csr_from_arrays", "device_count", "gen_grid", "gen_rmat",
            "read_gr", "verify", "write_dist", "write_gr", "write_result_row", "INF", "REF_INF"]

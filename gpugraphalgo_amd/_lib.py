@@ -20,6 +20,7 @@ P = ctypes.POINTER
 vp = ctypes.c_void_p
 
 GGA_OK = 0
+GGA_EBUDGET = -9
 ERRORS = {
     -1: "GGA_EINVAL", -2: "GGA_EIO", -3: "GGA_EFORMAT", -4: "GGA_ERANGE", -5: "GGA_ENOMEM",
     -6: "GGA_EHIP", -7: "GGA_ENOSPLIT", -8: "GGA_ETIMEDOUT", -9: "GGA_EBUDGET", -10: "GGA_ENODEV",
@@ -46,7 +47,7 @@ class SplitStats(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("mode", i32), ("inf", u32), ("max_iters", u32), ("block", u32),
                 ("sync_every", u32), ("use_graph", u32), ("count", u32), ("grid", u32),
-                ("dense_min", u32), ("pull_alpha", u32)]
+                ("dense_min", u32), ("pull_alpha", u32), ("ms_layout", u32)]
 
 
 class Stats(ctypes.Structure):
@@ -55,6 +56,35 @@ class Stats(ctypes.Structure):
                 ("ctime_ms", ctypes.c_float), ("mode", i32),
                 ("edges_inspected", u64), ("chunks_expanded", u64), ("relaxations", u64),
                 ("m_reach", u64), ("launches", u64), ("pull_levels", u32)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget", u64), ("steps_per_round", u32), ("lanes", u32), ("use_graph", u32),
+                ("dense_min", u32), ("host_results", u32), ("dry_run", u32), ("dry_steps", u32),
+                ("layout", u32)]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("graph", u32), ("src", u32), ("bfs", u32), ("out_on_device", u32),
+                ("arrival", ctypes.c_double), ("out", ctypes.c_void_p)]
+
+
+class QueryResult(ctypes.Structure):
+    _fields_ = [("id", u64), ("graph", u32), ("src", u32), ("bfs", u32), ("lane", u32),
+                ("status", i32), ("pad", u32), ("arrival", ctypes.c_double),
+                ("admitted", ctypes.c_double), ("done", ctypes.c_double), ("checksum", u64)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("rejected", u64), ("rounds", u64),
+                ("evictions", u64), ("engines_created", u64), ("bytes_in_use", u64),
+                ("peak_bytes", u64), ("engines_resident", u32), ("pad", u32)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("mda", u32), ("grid", u32), ("dense_min", u32), ("layout", u32), ("seed_mda", u32),
+                ("ntried", u32), ("seed_ms", ctypes.c_float), ("best_ms", ctypes.c_float),
+                ("layout_ms", ctypes.c_float * 2), ("tried_mda", u32 * 16), ("tried_ms", ctypes.c_float * 16)]
 
 
 class VerifyResult(ctypes.Structure):
@@ -100,18 +130,39 @@ SIGNATURES = [
     ("gga_graph_dist_device", vp, [vp]),
     ("gga_verify", ctypes.c_int, [u32, vp, vp, vp, u32, vp, u32, P(VerifyResult)]),
     ("gga_graph_verify", ctypes.c_int, [vp, u32, ctypes.c_int, u32, P(VerifyResult)]),
+    ("gga_graph_checksum", ctypes.c_int, [vp, P(u64)]),
     ("gga_write_dist", ctypes.c_int, [ctypes.c_char_p, vp, u32]),
     ("gga_write_result_row", ctypes.c_int, [ctypes.c_char_p, P(ResultRow)]),
     ("gga_graph_name", None, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
-    ("gga_ms_create", ctypes.c_int, [vp, u32, P(vp)]),
+    ("gga_ms_create", ctypes.c_int, [vp, u32, u32, P(vp)]),
     ("gga_ms_free", ctypes.c_int, [vp]),
     ("gga_ms_start", ctypes.c_int, [vp, u32, vp, u32]),
     ("gga_ms_run", ctypes.c_int, [vp, u32, P(Opts), P(u32)]),
     ("gga_ms_done", u32, [vp]),
+    ("gga_ms_launch", ctypes.c_int, [vp, u32, P(Opts)]),
+    ("gga_ms_collect", ctypes.c_int, [vp, P(u32)]),
+    ("gga_ms_ready", ctypes.c_int, [vp]),
+    ("gga_ms_checksum", ctypes.c_int, [vp, u32, vp]),
+    ("gga_ms_bytes", u64, [u32, u32]),
+    ("gga_checksum", u64, [vp, u32]),
     ("gga_ms_fetch", ctypes.c_int, [vp, u32, vp, ctypes.c_int]),
     ("gga_ms_stats", ctypes.c_int, [vp, P(Stats)]),
     ("gga_multi_sp", ctypes.c_int, [vp, vp, u32, u32, vp, P(Opts), P(Stats)]),
     ("gga_multi_bfs", ctypes.c_int, [vp, vp, u32, vp, P(Opts), P(Stats)]),
+    ("gga_tune", ctypes.c_int, [vp, u32, u32, P(TuneResult)]),
+    ("gga_tune_layout", u32, [vp]),
+    ("gga_server_create", ctypes.c_int, [P(ServerOpts), P(vp)]),
+    ("gga_server_free", ctypes.c_int, [vp]),
+    ("gga_server_add_graph", ctypes.c_int, [vp, vp, P(u32)]),
+    ("gga_server_add_graph_dry", ctypes.c_int, [vp, u32, u32, P(u32)]),
+    ("gga_server_now", ctypes.c_double, [vp]),
+    ("gga_server_submit", ctypes.c_int, [vp, P(Query), P(u64)]),
+    ("gga_server_pump", ctypes.c_int, [vp, P(u32)]),
+    ("gga_server_poll", ctypes.c_int, [vp, P(QueryResult), u32, P(u32)]),
+    ("gga_server_drain", ctypes.c_int, [vp]),
+    ("gga_server_replay", ctypes.c_int, [vp, P(Query), u32, P(QueryResult)]),
+    ("gga_server_replay_plain", ctypes.c_int, [vp, P(Query), u32, i32, P(QueryResult)]),
+    ("gga_server_stats", ctypes.c_int, [vp, P(ServerCounters)]),
 ]
 
 _LIB = None

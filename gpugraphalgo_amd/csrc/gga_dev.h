@@ -51,7 +51,11 @@ typedef struct gga_dev_work {
 /* Multi-source engine: GGA_MS_LANES independent traversals share every step.
  *   D[n * 32]   distances, vertex-major: the 32 lanes of a vertex are one
  *               128-byte line, so relaxing an edge for every lane touches one
- *               line of dist instead of 32
+ *               line of dist instead of 32 -- the layout for graphs whose
+ *               lanes reach the same vertices in the same steps (small-world
+ *               graphs); or lane-major (each lane's n distances contiguous),
+ *               for graphs whose lanes' frontiers rarely overlap (meshes),
+ *               where a vertex line would be fetched for one or two lanes
  *   M[2][n]     lane masks: M[s & 1][v] = lanes whose distance at v dropped in
  *               step s - 1 (step s expands exactly those lanes from v); a
  *               vertex is listed for step s iff its mask is non-zero, and the
@@ -72,7 +76,7 @@ typedef struct gga_dev_ms {
     uint64_t *cnt;       /* GGA_CNT_SLOTS x 4 work counters */
     uint32_t inf;
     uint32_t qcap;
-    uint32_t unit_lanes; /* lanes that relax with weight 1 (BFS levels) */
+    uint32_t lane_major; /* 1: D[lane * n + v] (each lane contiguous) instead of D[v * 32 + lane] */
     uint32_t dense_min;  /* a step whose list holds >= dense_min chunks rebuilds the next list
                             by compaction of the masks; smaller ones push */
 } gga_dev_ms;
@@ -134,6 +138,9 @@ typedef struct gga_dev_msbfs {
 
 /* multi-source engine (kernels/ms.hip): lane mask of a ring slot's list */
 #define GGA_CTL_LANES(s)  (7u * GGA_CTL_LINE + 8u + (s))
+/* multi-source engine: lanes relaxing with weight 1; kept on the device (not a
+ * kernel argument) so that steps replayed from a captured graph see restarts */
+#define GGA_CTL_UNIT      (7u * GGA_CTL_LINE + 12u)
 
 #define GGA_DEVERR_SPIN   1u   /* barrier spin limit hit */
 #define GGA_DEVERR_QFULL  2u   /* worklist overflow (cannot happen with de-duplication) */

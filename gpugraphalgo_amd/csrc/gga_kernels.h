@@ -54,15 +54,18 @@ int ggak_persistent(gga_dev_graph g, gga_dev_work w, int bfs, int count, uint32_
                     uint32_t max_steps, uint32_t spin_limit_ms, hipStream_t s);
 /* reached count -> ctl[RED] (u64), m_reach -> ctl[RED + 2] (u64). */
 int ggak_reach(gga_dev_graph g, gga_dev_work w, hipStream_t s);
+/* fingerprint of dist (gga_checksum) -> ctl[RED + 4] (u64). */
+int ggak_checksum(gga_dev_graph g, gga_dev_work w, hipStream_t s);
 /* nerr -> ctl[RED] (u64), untight -> ctl[RED + 2] (u64); d_tight: n bytes scratch. */
 int ggak_verify(gga_dev_graph g, gga_dev_work w, uint32_t src, int bfs, uint8_t *d_tight,
                 hipStream_t s);
 
 /* ---- multi-source engine (ms.hip), 32 lanes; `step` is the engine's step number */
 int ggak_ms_clear(gga_dev_ms ms, hipStream_t s);
-/* Restart `lanes` at src.v[lane] and list them for step `step`. */
-int ggak_ms_start(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, gga_ms_sources src, uint32_t step,
-                  hipStream_t s);
+/* Restart `lanes` at src.v[lane] (unit: which of them count hops) and list
+ * them for step `step`. */
+int ggak_ms_start(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, uint32_t unit, gga_ms_sources src,
+                  uint32_t step, hipStream_t s);
 /* One step: relax + list rebuild; ctl[TAIL((step+1)%3)] / ctl[LANES((step+1)%3)] give
  * the next list length and its live lanes. */
 int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int count, uint32_t grid, hipStream_t s);
@@ -76,6 +79,8 @@ int ggak_ms_harvest(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, gga_ms_outpu
 /* Reached (vertex, lane) pairs of `lanes` and their out-degree sum -> cnt word 3
  * partials; ggak_ms_fold then leaves them in ctl[RED], ctl[RED + 2] (uint64). */
 int ggak_ms_reach(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, hipStream_t s);
+/* sums[lane] += fingerprint of lane's distances (sums zeroed first; 32 uint64). */
+int ggak_ms_checksum(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, uint64_t *sums, hipStream_t s);
 /* Striped work counters -> ctl[CNT] (3 x uint64), reach partials -> ctl[RED]. */
 int ggak_ms_fold(gga_dev_ms ms, hipStream_t s);
 

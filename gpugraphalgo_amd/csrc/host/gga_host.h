@@ -22,6 +22,11 @@ typedef struct gga_graph_view_t {
 int gga_graph_view(gga_graph *g, gga_graph_view_t *v);
 /* The graph's cached engine for gga_multi_sp (freed with the graph or a re-split). */
 gga_ms **gga_graph_ms_slot(gga_graph *g);
+/* Tuned defaults (tune.c) that traversals use when their opts leave them 0. */
+void gga_graph_set_tuned(gga_graph *g, uint32_t grid, uint32_t dense_min, uint32_t layout);
+uint32_t gga_graph_tuned_layout(const gga_graph *g);
+uint32_t gga_graph_maxdeg(const gga_graph *g);
+uint32_t gga_graph_default_grid(gga_graph *g, int bfs);
 /* State of gga_multi_bfs (msbfs.c), owned by the graph. */
 void **gga_graph_msbfs_slot(gga_graph *g);
 void gga_msbfs_release(void *state);

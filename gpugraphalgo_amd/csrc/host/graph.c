@@ -40,6 +40,7 @@ struct gga_graph {
     uint32_t p_grid[2][2];
     uint32_t split_gen;       /* bumped by every split; engines built on an older table are stale */
     gga_ms *ms_cache;         /* engine of gga_multi_sp */
+    uint32_t tuned_grid, tuned_dense, tuned_layout;   /* gga_tune results (0: defaults) */
     void *msbfs;              /* transposed graph + state of gga_multi_bfs (msbfs.c) */
     hipEvent_t ev[4];
     hipEvent_t *kev;          /* per-launch timing events */
@@ -451,6 +452,16 @@ int gga_graph_view(gga_graph *g, gga_graph_view_t *v)
 }
 
 gga_ms **gga_graph_ms_slot(gga_graph *g) { return &g->ms_cache; }
+
+void gga_graph_set_tuned(gga_graph *g, uint32_t grid, uint32_t dense_min, uint32_t layout)
+{
+    g->tuned_grid = grid;
+    g->tuned_dense = dense_min;
+    g->tuned_layout = layout;
+}
+
+uint32_t gga_graph_tuned_layout(const gga_graph *g) { return g->tuned_layout; }
+uint32_t gga_graph_maxdeg(const gga_graph *g) { return g ? g->maxdeg : 0; }
 void **gga_graph_msbfs_slot(gga_graph *g) { return &g->msbfs; }
 
 /* -------------------------------------------------------------- traversal */
@@ -483,6 +494,8 @@ static uint32_t wl_grid(gga_graph *g, int persistent, int bfs, int count)
     if (*slot == 0) *slot = ggak_wl_grid(g->device, persistent, bfs, count);
     return *slot;
 }
+
+uint32_t gga_graph_default_grid(gga_graph *g, int bfs) { return wl_grid(g, 0, bfs ? 1 : 0, 0); }
 
 static int choose_mode(const gga_graph *g)
 {
@@ -523,7 +536,8 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
     }
     gga_dev_graph dg = dev_graph(g);
     gga_dev_work dw = dev_work(g, inf);
-    dw.dense_min = o.dense_min ? o.dense_min : (g->n / 64u > 4096u ? g->n / 64u : 4096u);
+    dw.dense_min = o.dense_min ? o.dense_min
+                               : g->tuned_dense ? g->tuned_dense : (g->n / 64u > 4096u ? g->n / 64u : 4096u);
     s.mode = mode;
     float ktime = 0.f;
 
@@ -555,7 +569,7 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
         }
         s.iterations = it;
     } else if (mode == GGA_MODE_WORKLIST) {
-        const uint32_t grid = o.grid ? o.grid : wl_grid(g, 0, bfs, count);
+        const uint32_t grid = o.grid ? o.grid : g->tuned_grid ? g->tuned_grid : wl_grid(g, 0, bfs, count);
         if (grid == 0) {
             rc = GGA_EHIP;
             gga_set_error("occupancy query failed");
@@ -687,6 +701,23 @@ fail:
 const uint32_t *gga_graph_dist_device(gga_graph *g)
 {
     return (g && g->have_result) ? g->d_dist : NULL;
+}
+
+int gga_graph_checksum(gga_graph *g, uint64_t *sum)
+{
+    int rc = GGA_OK;
+    if (!g || !sum) return GGA_EINVAL;
+    if (!g->have_result) return GGA_EINVAL;
+    HIP_TRY(hipSetDevice(g->device));
+    gga_dev_graph dg = dev_graph(g);
+    gga_dev_work dw = dev_work(g, GGA_INF);
+    K_TRY(ggak_checksum(dg, dw, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_RED + 4, 8, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    *sum = (uint64_t)g->h_pin[0] | ((uint64_t)g->h_pin[1] << 32);
+    return GGA_OK;
+fail:
+    return rc;
 }
 
 int gga_graph_verify(gga_graph *g, uint32_t src, int bfs, uint32_t inf, gga_verify_result *r)

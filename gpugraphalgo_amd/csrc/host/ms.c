@@ -30,7 +30,10 @@ struct gga_ms {
     int gx_count[6];
     uint32_t dense_default;
     uint64_t dev_bytes;
-    hipEvent_t ev[3];         /* run timing; [2] marks gga_multi_sp's start */
+    hipEvent_t ev[4];         /* [0],[1] around steps; [2] gga_multi_sp's start; [3] read-back done */
+    uint32_t pending;         /* steps launched, not yet collected */
+    uint32_t launched_lanes;  /* lanes running when the pending steps were launched */
+    uint64_t *d_sum, *h_sum;  /* 32 lane fingerprints */
     float ktime_ms;
     uint32_t iterations;
 };
@@ -93,13 +96,15 @@ int gga_ms_free(gga_ms *e)
     ms_release(e->d.cnt);
     ms_release(e->d_stage);
     if (e->h_pin) hipHostFree(e->h_pin);
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 4; i++)
         if (e->ev[i]) hipEventDestroy(e->ev[i]);
+    ms_release(e->d_sum);
+    if (e->h_sum) hipHostFree(e->h_sum);
     free(e);
     return GGA_OK;
 }
 
-int gga_ms_create(gga_graph *g, uint32_t inf, gga_ms **out)
+int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
 {
     int rc = GGA_OK;
     if (!g || !out) return GGA_EINVAL;
@@ -117,11 +122,15 @@ int gga_ms_create(gga_graph *g, uint32_t inf, gga_ms **out)
     e->ntiles = GGA_TILES(dg->n);
     e->d.inf = inf ? inf : GGA_INF;
     e->d.qcap = dg->C;
+    if (layout == GGA_MS_AUTO) layout = gga_tune_layout(g);
+    e->d.lane_major = layout == GGA_MS_LANE_MAJOR ? 1u : 0u;
     e->dense_default = e->n / 64u > 4096u ? e->n / 64u : 4096u;
     e->d.dense_min = e->dense_default;
     HIP_TRY(hipSetDevice(e->v.device));
-    for (int i = 0; i < 3; i++) HIP_TRY(hipEventCreate(&e->ev[i]));
+    for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&e->ev[i]));
     HIP_TRY(hipHostMalloc((void **)&e->h_pin, 4ull * GGA_CTL_WORDS, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&e->h_sum, 8ull * GGA_MS_LANES, hipHostMallocDefault));
+    if ((rc = ms_alloc(e, (void **)&e->d_sum, 8ull * GGA_MS_LANES))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.D, 4ull * GGA_MS_LANES * e->n))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.M0, 4ull * e->n))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.M1, 4ull * e->n))) goto fail;
@@ -177,8 +186,7 @@ int gga_ms_start(gga_ms *e, uint32_t lanes, const uint32_t *src, uint32_t unit_l
     if (!lanes) return GGA_OK;
     HIP_TRY(hipSetDevice(e->v.device));
     e->unit = (e->unit & ~lanes) | (unit_lanes & lanes);
-    e->d.unit_lanes = e->unit;
-    K_TRY(ggak_ms_start(e->v.dg, e->d, lanes, s, e->step % 6u, e->v.stream));
+    K_TRY(ggak_ms_start(e->v.dg, e->d, lanes, unit_lanes & lanes, s, e->step % 6u, e->v.stream));
     e->running |= lanes;
     e->done &= ~lanes;
     return GGA_OK;
@@ -252,6 +260,121 @@ static void ms_slot(const gga_ms *e, uint32_t *live, uint32_t *tail)
     *live = *tail ? e->h_pin[GGA_CTL_LANES(e->step % 3u)] : 0u;
 }
 
+int gga_ms_launch(gga_ms *e, uint32_t nsteps, const gga_opts *o)
+{
+    int rc = GGA_OK;
+    gga_opts oo;
+    if (!e) return GGA_EINVAL;
+    if (e->pending) {
+        gga_set_error("previous launch not collected");
+        return GGA_EINVAL;
+    }
+    if ((rc = ms_check(e))) return rc;
+    if (o) oo = *o;
+    else gga_opts_init(&oo);
+    const int count = oo.count ? 1 : 0;
+    e->d.dense_min = oo.dense_min ? oo.dense_min : e->dense_default;
+    const uint32_t grid = oo.grid ? oo.grid : ms_grid(e, 0, count);
+    if (grid == 0) {
+        gga_set_error("occupancy query failed");
+        return GGA_EHIP;
+    }
+    HIP_TRY(hipSetDevice(e->v.device));
+    HIP_TRY(hipEventRecord(e->ev[0], e->v.stream));
+    uint32_t k = 0;
+    if (oo.use_graph)
+        for (; k + 6u <= nsteps; k += 6u) {
+            if ((rc = ms_graph(e, (e->step + k) % 6u, count, grid))) return rc;
+            HIP_TRY(hipGraphLaunch(e->gx[(e->step + k) % 6u], e->v.stream));
+        }
+    for (; k < nsteps; k++) K_TRY(ggak_ms_step(e->v.dg, e->d, (e->step + k) % 6u, count, grid, e->v.stream));
+    HIP_TRY(hipEventRecord(e->ev[1], e->v.stream));
+    HIP_TRY(hipMemcpyAsync(e->h_pin, e->d.ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, e->v.stream));
+    HIP_TRY(hipEventRecord(e->ev[3], e->v.stream));
+    e->step += nsteps;
+    e->pending = nsteps ? nsteps : 1u;   /* a zero-step launch still carries a read-back */
+    e->launched_lanes = e->running;
+    return GGA_OK;
+fail:
+    return rc;
+}
+
+int gga_ms_collect(gga_ms *e, uint32_t *live_out)
+{
+    int rc = GGA_OK;
+    uint32_t live = 0, tail = 0;
+    float t = 0.f;
+    if (!e) return GGA_EINVAL;
+    if (!e->pending) {
+        gga_set_error("nothing launched");
+        return GGA_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(e->v.device));
+    HIP_TRY(hipEventSynchronize(e->ev[3]));
+    const uint32_t ran = e->pending;
+    e->pending = 0;
+    if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_QFULL) {
+        gga_set_error("multi-source list overflow (capacity %u)", e->d.qcap);
+        return GGA_ERANGE;
+    }
+    e->steps_seen = e->h_pin[GGA_CTL_STEPS];
+    ms_slot(e, &live, &tail);
+    HIP_TRY(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));
+    e->ktime_ms += t;
+    e->iterations += ran;
+    /* lanes started after the launch are not in this read-back */
+    e->done |= e->launched_lanes & ~live;
+    e->running &= live | ~e->launched_lanes;
+    if (live_out) *live_out = live | (e->running & ~e->launched_lanes);
+    return GGA_OK;
+fail:
+    return rc;
+}
+
+int gga_ms_ready(gga_ms *e)
+{
+    if (!e) return GGA_EINVAL;
+    if (!e->pending) return 1;
+    hipSetDevice(e->v.device);
+    const hipError_t r = hipEventQuery(e->ev[3]);
+    if (r == hipSuccess) return 1;
+    if (r == hipErrorNotReady) return 0;
+    gga_set_error("hipEventQuery: %s", hipGetErrorString(r));
+    return GGA_EHIP;
+}
+
+int gga_ms_checksum(gga_ms *e, uint32_t lanes, uint64_t *sums)
+{
+    int rc = GGA_OK;
+    if (!e || !sums) return GGA_EINVAL;
+    if (lanes & ~e->done) {
+        gga_set_error("lanes 0x%08x have not finished", lanes & ~e->done);
+        return GGA_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(e->v.device));
+    K_TRY(ggak_fill_u32((uint32_t *)e->d_sum, 0u, 2ull * GGA_MS_LANES, e->v.stream));
+    K_TRY(ggak_ms_checksum(e->v.dg, e->d, lanes, e->d_sum, e->v.stream));
+    HIP_TRY(hipMemcpyAsync(e->h_sum, e->d_sum, 8ull * GGA_MS_LANES, hipMemcpyDeviceToHost, e->v.stream));
+    HIP_TRY(hipStreamSynchronize(e->v.stream));
+    for (uint32_t i = 0; i < GGA_MS_LANES; i++) sums[i] = ((lanes >> i) & 1u) ? e->h_sum[i] : 0ull;
+    return GGA_OK;
+fail:
+    return rc;
+}
+
+uint64_t gga_ms_bytes(uint32_t n, uint32_t chunks)
+{
+    return 4ull * GGA_MS_LANES * n + 8ull * n + 8ull * chunks + 4ull * (GGA_TILES((uint64_t)n) + 1) +
+           4ull * GGA_CTL_WORDS + 32ull * GGA_CNT_SLOTS + 8ull * GGA_MS_LANES;
+}
+
+uint64_t gga_checksum(const uint32_t *dist, uint32_t n)
+{
+    uint64_t h = 0;
+    for (uint32_t v = 0; v < n; v++) h += gga_splitmix64(((uint64_t)v << 32) | dist[v]);
+    return h;
+}
+
 int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live_out)
 {
     int rc = GGA_OK;
@@ -288,33 +411,21 @@ int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live_
         HIP_TRY(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));
         e->ktime_ms += t;
     } else {
-        uint32_t grid = oo.grid ? oo.grid : ms_grid(e, 0, count);
-        if (grid == 0) {
-            gga_set_error("occupancy query failed");
-            return GGA_EHIP;
-        }
-        const int use_graph = oo.use_graph ? 1 : 0;
-        const uint32_t per = use_graph ? 6u : (oo.sync_every ? oo.sync_every : 8u);
+        const uint32_t per = oo.use_graph ? 6u : (oo.sync_every ? oo.sync_every : 8u);
         for (;;) {
             const uint32_t k = per < cap - ran ? per : cap - ran;
-            HIP_TRY(hipEventRecord(e->ev[0], e->v.stream));
-            if (use_graph && k == 6u) {
-                if ((rc = ms_graph(e, e->step % 6u, count, grid))) return rc;
-                HIP_TRY(hipGraphLaunch(e->gx[e->step % 6u], e->v.stream));
-            } else {
-                for (uint32_t i = 0; i < k; i++)
-                    K_TRY(ggak_ms_step(e->v.dg, e->d, (e->step + i) % 6u, count, grid, e->v.stream));
-            }
-            HIP_TRY(hipEventRecord(e->ev[1], e->v.stream));
-            e->step += k;
+            if ((rc = gga_ms_launch(e, k, &oo))) return rc;
+            if ((rc = gga_ms_collect(e, &live))) return rc;
             ran += k;
-            if ((rc = ms_readback(e))) return rc;
-            e->steps_seen = e->h_pin[GGA_CTL_STEPS];
-            ms_slot(e, &live, &tail);
-            HIP_TRY(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));
-            e->ktime_ms += t;
+            tail = e->h_pin[GGA_CTL_TAIL(e->step % 3u)];
             if (tail == 0 || ran >= cap) break;
         }
+        if (live_out) *live_out = live;
+        if (!max_steps && tail) {
+            gga_set_error("multi-source: %u steps without convergence", ran);
+            return GGA_ETIMEDOUT;
+        }
+        return GGA_OK;
     }
     e->iterations += ran;
     e->done |= e->running & ~live;
@@ -402,11 +513,13 @@ int gga_multi_sp(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t unit_l
     if (oo.mode != GGA_MODE_PERSISTENT) oo.mode = GGA_MODE_WORKLIST;
     const uint32_t inf = oo.inf ? oo.inf : GGA_INF;
     gga_ms **slot = gga_graph_ms_slot(g);
-    if (*slot && ((*slot)->d.inf != inf || (*slot)->running)) {
+    const uint32_t want_lm = (oo.ms_layout ? oo.ms_layout : gga_tune_layout(g)) == GGA_MS_LANE_MAJOR;
+    if (*slot && ((*slot)->d.inf != inf || (*slot)->running || (*slot)->d.lane_major != want_lm)) {
         gga_ms_free(*slot);
         *slot = NULL;
     }
-    if (!*slot && (rc = gga_ms_create(g, inf, slot))) return rc;
+    if (!*slot && (rc = gga_ms_create(g, inf, want_lm ? GGA_MS_LANE_MAJOR : GGA_MS_VERTEX_MAJOR, slot)))
+        return rc;
     gga_ms *e = *slot;
     const uint32_t lanes = k == 32 ? 0xFFFFFFFFu : ((1u << k) - 1u);
     uint32_t src[GGA_MS_LANES];

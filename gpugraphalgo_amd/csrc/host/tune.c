@@ -1,0 +1,150 @@
+/*
+ * tune.c -- the autotuner: split threshold, launch grid, push/compaction
+ * switch and multi-source layout, per graph, by measurement.
+ *
+ * Replaces the reference's two ways of picking the node-split threshold: the
+ * histogram guess getMaxAllowedOutDegree (graph.h:359-385, with the default
+ * 10 buckets of bfs_sssp.cu:166) and the hand sweep over NodeSplitLevel in
+ * mysc (mysc:20-51).  The histogram threshold is the tuner's first candidate
+ * (its seed); every candidate is timed on one query and the fastest kept.
+ * The choice never changes a result -- every threshold, grid and switch gives
+ * the same distances -- only the time.  Workgroups stay at 256 threads (four
+ * waves; the per-wave LDS staging is sized for it); the grid, i.e. the
+ * workgroups resident per CU, is tuned instead.
+ */
+#define _GNU_SOURCE
+#include <stdlib.h>
+#include <string.h>
+
+#include "gga_host.h"
+
+uint32_t gga_tune_layout(gga_graph *g)
+{
+    if (!g) return GGA_MS_VERTEX_MAJOR;
+    const uint32_t t = gga_graph_tuned_layout(g);
+    if (t) return t;
+    /* Without a measurement: vertex-major.  Lanes started at random sources
+     * meet at the same vertices in the same steps on small-world graphs and on
+     * meshes with a few random long edges alike (measured on RMAT and on grids
+     * with 0.5% shortcuts); only a measurement picks lane-major. */
+    return GGA_MS_VERTEX_MAJOR;
+}
+
+/* Best of two runs of one query (device time, ms). */
+static int time_query(gga_graph *g, uint32_t src, int bfs, const gga_opts *o, float *ms)
+{
+    float best = 0.f;
+    for (int rep = 0; rep < 2; rep++) {
+        gga_stats st;
+        const int rc = bfs ? gga_bfs(g, src, NULL, o, &st) : gga_sssp(g, src, NULL, o, &st);
+        if (rc) return rc;
+        if (rep == 0 || st.time_ms < best) best = st.time_ms;
+    }
+    *ms = best;
+    return GGA_OK;
+}
+
+static void record(gga_tune_result *r, uint32_t mda, float ms)
+{
+    if (r->ntried < GGA_TUNE_MAX) {
+        r->tried_mda[r->ntried] = mda;
+        r->tried_ms[r->ntried] = ms;
+    }
+    r->ntried++;
+}
+
+int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
+{
+    int rc = GGA_OK;
+    if (!g || !r) return GGA_EINVAL;
+    if (src >= gga_graph_n(g)) return GGA_EINVAL;
+    memset(r, 0, sizeof *r);
+    const int bfs = (flags & GGA_TUNE_BFS) ? 1 : 0;
+    gga_graph_set_tuned(g, 0, 0, 0);
+    gga_opts o;
+    gga_opts_init(&o);
+    o.mode = GGA_MODE_WORKLIST;
+    gga_split_stats st;
+
+    /* 1. split threshold: the histogram seed, then a sweep.  On graphs whose
+     * maximum degree is below the bucket count the histogram threshold is 0
+     * (the reference would divide by it); the unsplit graph seeds instead. */
+    const uint32_t maxdeg = gga_graph_maxdeg(g) ? gga_graph_maxdeg(g) : 1u;
+    rc = gga_graph_split(g, 0, -1, 10, &st);
+    if (rc == GGA_EINVAL) rc = gga_graph_split(g, maxdeg, 0, 0, &st);
+    else if (!rc) r->seed_mda = st.mda;
+    if (rc) return rc;
+    if ((rc = time_query(g, src, bfs, &o, &r->seed_ms))) return rc;
+    record(r, st.mda, r->seed_ms);
+    const uint32_t seed_t = st.mda;
+    uint32_t best_t = seed_t;
+    float best = r->seed_ms;
+    const uint32_t cand[] = {16u, 32u, 64u, 128u, 256u, 1024u, maxdeg};
+    for (size_t i = 0; i < sizeof cand / sizeof cand[0]; i++) {
+        const uint32_t t = cand[i] < maxdeg ? cand[i] : maxdeg;
+        int seen = t == seed_t;
+        for (size_t j = 0; j < i && !seen; j++) seen = (cand[j] < maxdeg ? cand[j] : maxdeg) == t;
+        if (seen) continue;
+        float ms;
+        if ((rc = gga_graph_split(g, t, 0, 0, &st))) return rc;
+        if ((rc = time_query(g, src, bfs, &o, &ms))) return rc;
+        record(r, t, ms);
+        if (ms < best) {
+            best = ms;
+            best_t = t;
+        }
+    }
+    if ((rc = gga_graph_split(g, best_t, 0, 0, &st))) return rc;
+    r->mda = best_t;
+
+    /* 2. grid (workgroups) and 3. the push / compaction switch */
+    const uint32_t g0 = gga_graph_default_grid(g, bfs);
+    r->grid = g0;
+    for (uint32_t div = 2; div <= 4 && g0 / div >= 64u; div *= 2) {
+        float ms;
+        o.grid = g0 / div;
+        if ((rc = time_query(g, src, bfs, &o, &ms))) return rc;
+        if (ms < best) {
+            best = ms;
+            r->grid = o.grid;
+        }
+    }
+    o.grid = r->grid;
+    const uint32_t n = gga_graph_n(g);
+    const uint32_t d0 = n / 64u > 4096u ? n / 64u : 4096u;
+    r->dense_min = d0;
+    const uint32_t dcand[] = {d0 / 4u, d0 * 4u};
+    for (size_t i = 0; i < 2; i++) {
+        float ms;
+        o.dense_min = dcand[i] ? dcand[i] : 1u;
+        if ((rc = time_query(g, src, bfs, &o, &ms))) return rc;
+        if (ms < best) {
+            best = ms;
+            r->dense_min = o.dense_min;
+        }
+    }
+    r->best_ms = best;
+
+    /* 4. multi-source layout: 32 lanes spread over the vertex range */
+    r->layout = gga_tune_layout(g);
+    if (flags & GGA_TUNE_LAYOUT) {
+        uint32_t srcs[GGA_MS_LANES];
+        for (uint32_t i = 0; i < GGA_MS_LANES; i++) srcs[i] = (uint32_t)((src + (uint64_t)i * (n / GGA_MS_LANES + 1)) % n);
+        float lt[2] = {0.f, 0.f};
+        for (int l = 0; l < 2; l++) {
+            gga_opts mo;
+            gga_opts_init(&mo);
+            mo.ms_layout = l == 0 ? GGA_MS_VERTEX_MAJOR : GGA_MS_LANE_MAJOR;
+            for (int rep = 0; rep < 2; rep++) {
+                gga_stats ms;
+                if ((rc = gga_multi_sp(g, srcs, GGA_MS_LANES, bfs ? 0xFFFFFFFFu : 0u, NULL, &mo, &ms))) return rc;
+                if (rep == 0 || ms.time_ms < lt[l]) lt[l] = ms.time_ms;
+            }
+        }
+        r->layout = lt[1] < lt[0] ? GGA_MS_LANE_MAJOR : GGA_MS_VERTEX_MAJOR;
+        r->layout_ms[0] = lt[0];
+        r->layout_ms[1] = lt[1];
+    }
+    gga_graph_set_tuned(g, r->grid, r->dense_min, (flags & GGA_TUNE_LAYOUT) ? r->layout : 0u);
+    return GGA_OK;
+}

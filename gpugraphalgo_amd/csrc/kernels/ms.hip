@@ -91,7 +91,9 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t x) {
 // ---------------------------------------------------------------- relax
 __device__ __forceinline__ bool ms_flag_step(const gga_dev_ms &ms, uint32_t nq) { return nq >= ms.dense_min; }
 
-template <bool COUNT>
+// LM: lane-major layout (compile-time, so the vertex-major loop keeps its
+// unconditional full-line loads)
+template <bool COUNT, bool LM>
 __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
                                          uint32_t nq, uint32_t *pbuf, Counters &cnt) {
     const bool push = !ms_flag_step(ms, nq);
@@ -109,7 +111,7 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
     const uint32_t nbatch = (nq + bsz - 1) / bsz;
     const uint32_t l = lane_id(), half = l >> 5, s = l & (LANES - 1u);
-    const bool unit = (ms.unit_lanes >> s) & 1u;
+    const bool unit = (ms.ctl[GGA_CTL_UNIT] >> s) & 1u;
     for (uint32_t b = wid; b < nbatch; b += waves) {
         const uint32_t idx = b * bsz + l;
         const bool valid = l < bsz && idx < nq;
@@ -141,17 +143,24 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                 mm[i] = wave_bcast(m, j);
                 v[i] = stream_load(&g.col[e]);
                 wt[i] = stream_load(&g.w[e]);
-                du[i] = D[(uint64_t)uu * LANES + s];
+                // vertex-major: the whole line comes in anyway, every lane loads;
+                // lane-major: only the lanes to relax gather their own word
+                if (LM) du[i] = ((mm[i] >> s) & 1u) ? D[(uint64_t)s * g.n + uu] : inf;
+                else du[i] = D[(uint64_t)uu * LANES + s];
             }
 #pragma unroll
-            for (uint32_t i = 0; i < MS_UNROLL; i++) dv[i] = D[(uint64_t)v[i] * LANES + s];
+            for (uint32_t i = 0; i < MS_UNROLL; i++) {
+                if (LM) dv[i] = ((mm[i] >> s) & 1u) ? D[(uint64_t)s * g.n + v[i]] : 0u;
+                else dv[i] = D[(uint64_t)v[i] * LANES + s];
+            }
 #pragma unroll
             for (uint32_t i = 0; i < MS_UNROLL; i++) {
                 const bool on = act[i] && ((mm[i] >> s) & 1u) && du[i] < inf;
                 const uint32_t a = du[i] + (unit ? 1u : wt[i]);
                 bool imp = false;
                 if (on && a >= du[i] && a < inf && a < dv[i]) {
-                    const uint32_t old = atomic_min_u32(&D[(uint64_t)v[i] * LANES + s], a);
+                    const uint32_t old =
+                        atomic_min_u32(&D[LM ? (uint64_t)s * g.n + v[i] : (uint64_t)v[i] * LANES + s], a);
                     imp = a < old;
                 }
                 const uint64_t bal = __ballot(imp);
@@ -232,12 +241,23 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reset(gga_dev_graph g, gga_dev_ms 
     }
 }
 
+// Lane-major form: blockIdx.y = lane, coalesced over vertices.
+__global__ __launch_bounds__(BLOCK) void k_ms_reset_lm(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes,
+                                                       gga_ms_sources src) {
+    const uint32_t s = blockIdx.y;
+    if (!((lanes >> s) & 1u)) return;
+    uint32_t *row = ms.D + (uint64_t)s * g.n;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x)
+        row[v] = v == src.v[s] ? 0u : ms.inf;
+}
+
 // Seeds the restarted lanes into the list of ring slot `step`: the source's
 // mask gets the lane bit; the first lane to reach an unlisted source appends
 // its chunks.
-__global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes,
+__global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, uint32_t unit,
                                                 gga_ms_sources src, uint32_t step) {
     const uint32_t s = threadIdx.x;
+    if (s == 0) ms.ctl[GGA_CTL_UNIT] = (ms.ctl[GGA_CTL_UNIT] & ~lanes) | (unit & lanes);
     if (s >= LANES || !((lanes >> s) & 1u)) return;
     const uint32_t x = src.v[s];
     uint32_t *mcur = ms_mcur(ms, step);
@@ -251,13 +271,13 @@ __global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, 
     for (uint32_t k = 0; k < nch; k++) queue_store(q, pos + k, cs + k, ms.qcap, ms.ctl);
 }
 
-template <bool COUNT>
+template <bool COUNT, bool LM>
 __global__ __launch_bounds__(BLOCK) void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
     const uint32_t nq = ms_len(ms, step);
     if (nq == 0) return;
     Counters cnt;
-    ms_relax<COUNT>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+    ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
     publish_counters<COUNT>(ms, cnt);
 }
 
@@ -294,7 +314,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_finish(gga_dev_graph g, gga_dev_ms
 
 // All steps in one cooperative launch: relax, then the three compaction
 // passes, separated by grid barriers.  Step numbers continue from `step0`.
-template <bool COUNT>
+template <bool COUNT, bool LM>
 __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0,
                                                          uint32_t max_steps, uint64_t spin_ticks) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
@@ -314,7 +334,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_de
         const uint32_t nq = s_state[0] < ms.qcap ? s_state[0] : ms.qcap, err = s_state[1];
         __syncthreads();
         if (nq == 0 || err) break;
-        ms_relax<COUNT>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+        ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
         grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
         if (ms_flag_step(ms, nq)) {
             const ListOut o = ms_list_out(g, ms, step);
@@ -352,6 +372,15 @@ __global__ __launch_bounds__(BLOCK) void k_ms_harvest(gga_dev_graph g, gga_dev_m
     }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_ms_harvest_lm(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes,
+                                                         gga_ms_outputs out) {
+    const uint32_t s = blockIdx.y;
+    if (!((lanes >> s) & 1u)) return;
+    const uint32_t *row = ms.D + (uint64_t)s * g.n;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x)
+        out.p[s][v] = row[v];
+}
+
 // Reached (vertex, lane) pairs of `lanes` and the out-degree sum over them
 // (the GTEPS numerator): per-workgroup partials in word 3 of cnt slots
 // [0, REACH) and [REACH, 2 REACH), folded by k_ms_fold.
@@ -386,6 +415,86 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reach(gga_dev_graph g, gga_dev_ms 
         }
         ms.cnt[4u * blockIdx.x + 3u] = a;
         ms.cnt[4u * (REACH + blockIdx.x) + 3u] = b;
+    }
+}
+
+// Lane-major reach: a 16 x 32 grid, one partial slot per workgroup.
+__global__ __launch_bounds__(BLOCK) void k_ms_reach_lm(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes) {
+    __shared__ uint64_t s_red[2][WAVES];
+    const uint32_t s = blockIdx.y;
+    uint64_t r = 0, mr = 0;
+    if ((lanes >> s) & 1u) {
+        const uint32_t *row = ms.D + (uint64_t)s * g.n;
+        for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+             v += (uint64_t)gridDim.x * blockDim.x)
+            if (row[v] < ms.inf) {
+                r += 1;
+                mr += g.row[v + 1] - g.row[v];
+            }
+    }
+    r = wave_sum64(r);
+    mr = wave_sum64(mr);
+    if (lane_id() == 0) {
+        s_red[0][threadIdx.x / GGA_WAVE] = r;
+        s_red[1][threadIdx.x / GGA_WAVE] = mr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0, b = 0;
+        for (uint32_t i = 0; i < WAVES; i++) {
+            a += s_red[0][i];
+            b += s_red[1][i];
+        }
+        const uint32_t slot = blockIdx.y * gridDim.x + blockIdx.x;
+        ms.cnt[4u * slot + 3u] = a;
+        ms.cnt[4u * (REACH + slot) + 3u] = b;
+    }
+}
+
+// Per-lane fingerprint of the distances: sum over v of mix64(v << 32 | D[v][lane])
+// (wrapping; order-independent), the value gga_checksum computes on the host.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ms_checksum(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes,
+                                                       uint64_t *sums) {
+    __shared__ uint64_t s_part[BLOCK];
+    const uint32_t s = threadIdx.x & (LANES - 1u);
+    const bool mine = (lanes >> s) & 1u;
+    uint64_t h = 0;
+    const uint64_t total = (uint64_t)g.n * LANES;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        if (mine) h += mix64(((i / LANES) << 32) | ms.D[i]);
+    s_part[threadIdx.x] = h;
+    __syncthreads();
+    if (threadIdx.x < LANES && mine) {
+        uint64_t t = 0;
+        for (uint32_t k = threadIdx.x; k < BLOCK; k += LANES) t += s_part[k];
+        atomic_add_u64(&sums[threadIdx.x], t);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ms_checksum_lm(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes,
+                                                          uint64_t *sums) {
+    __shared__ uint64_t s_red[WAVES];
+    const uint32_t s = blockIdx.y;
+    if (!((lanes >> s) & 1u)) return;
+    const uint32_t *row = ms.D + (uint64_t)s * g.n;
+    uint64_t h = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x)
+        h += mix64((v << 32) | row[v]);
+    h = wave_sum64(h);
+    if (lane_id() == 0) s_red[threadIdx.x / GGA_WAVE] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (uint32_t i = 0; i < WAVES; i++) t += s_red[i];
+        atomic_add_u64(&sums[s], t);
     }
 }
 
@@ -440,21 +549,40 @@ extern "C" int ggak_ms_clear(gga_dev_ms ms, hipStream_t s) {
     return launch_status();
 }
 
-extern "C" int ggak_ms_start(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, gga_ms_sources src, uint32_t step,
-                             hipStream_t s) {
+extern "C" int ggak_ms_start(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, uint32_t unit, gga_ms_sources src,
+                             uint32_t step, hipStream_t s) {
     if (lanes == 0) return 0;
     for (uint32_t i = 0; i < LANES; i++)
         if (((lanes >> i) & 1u) && src.v[i] >= g.n) return (int)hipErrorInvalidValue;
-    k_ms_reset<<<grid_for((uint64_t)g.n * LANES, 8192), BLOCK, 0, s>>>(g, ms, lanes, src);
-    k_ms_seed<<<1, 64, 0, s>>>(g, ms, lanes, src, step);
+    if (ms.lane_major) k_ms_reset_lm<<<dim3(grid_for(g.n, 256), LANES), BLOCK, 0, s>>>(g, ms, lanes, src);
+    else k_ms_reset<<<grid_for((uint64_t)g.n * LANES, 8192), BLOCK, 0, s>>>(g, ms, lanes, src);
+    k_ms_seed<<<1, 64, 0, s>>>(g, ms, lanes, unit, src, step);
     return launch_status();
+}
+
+template <bool COUNT, bool LM>
+static const void *step_fn() {
+    return reinterpret_cast<const void *>(&k_ms_step<COUNT, LM>);
+}
+template <bool COUNT, bool LM>
+static const void *persistent_fn() {
+    return reinterpret_cast<const void *>(&k_ms_persistent<COUNT, LM>);
+}
+static const void *pick(int persistent, int count, uint32_t lm) {
+    static const void *const st[4] = {step_fn<false, false>(), step_fn<false, true>(), step_fn<true, false>(),
+                                      step_fn<true, true>()};
+    static const void *const pk[4] = {persistent_fn<false, false>(), persistent_fn<false, true>(),
+                                      persistent_fn<true, false>(), persistent_fn<true, true>()};
+    const int i = (count ? 2 : 0) | (lm ? 1 : 0);
+    return persistent ? pk[i] : st[i];
 }
 
 extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int count, uint32_t grid,
                             hipStream_t s) {
     if (grid == 0) return (int)hipErrorInvalidValue;
-    if (count) k_ms_step<true><<<grid, BLOCK, 0, s>>>(g, ms, step);
-    else k_ms_step<false><<<grid, BLOCK, 0, s>>>(g, ms, step);
+    void *args[] = {&g, &ms, &step};
+    hipError_t e = hipLaunchKernel(pick(0, count, ms.lane_major), dim3(grid), dim3(BLOCK), args, 0, s);
+    if (e != hipSuccess) return (int)e;
     const uint32_t cg = compact_grid(g.n);
     k_ms_count<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
@@ -466,11 +594,9 @@ extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int c
 extern "C" uint32_t ggak_ms_grid(int device, int persistent, int count) {
     int cus = 0, per = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    const void *fn = persistent ? (count ? reinterpret_cast<const void *>(&k_ms_persistent<true>)
-                                         : reinterpret_cast<const void *>(&k_ms_persistent<false>))
-                                : (count ? reinterpret_cast<const void *>(&k_ms_step<true>)
-                                         : reinterpret_cast<const void *>(&k_ms_step<false>));
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, 0) != hipSuccess || per <= 0) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pick(persistent, count, 0), BLOCK, 0) != hipSuccess ||
+        per <= 0)
+        return 0;
     // cooperative launch: stay well under the co-residency measured for these
     // kernels (6 per CU); 2 per CU gives the relax twice the waves of 1 per CU
     if (persistent) per = per < 2 ? per : 2;
@@ -486,9 +612,7 @@ extern "C" int ggak_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0
         khz = 100000;
     uint64_t spin_ticks = (uint64_t)spin_limit_ms * (uint64_t)khz;
     void *args[] = {&g, &ms, &step0, &max_steps, &spin_ticks};
-    const void *fn = count ? reinterpret_cast<const void *>(&k_ms_persistent<true>)
-                           : reinterpret_cast<const void *>(&k_ms_persistent<false>);
-    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(BLOCK), args, 0, s);
+    hipError_t e = hipLaunchCooperativeKernel(pick(1, count, ms.lane_major), dim3(grid), dim3(BLOCK), args, 0, s);
     if (e != hipSuccess) return (int)e;
     return launch_status();
 }
@@ -497,13 +621,22 @@ extern "C" int ggak_ms_harvest(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, g
                                hipStream_t s) {
     if (lanes == 0) return 0;
     const uint32_t grid = (uint32_t)(((uint64_t)g.n + HV - 1) / HV);
-    k_ms_harvest<<<grid, BLOCK, 0, s>>>(g, ms, lanes, out);
+    if (ms.lane_major) k_ms_harvest_lm<<<dim3(grid_for(g.n, 256), LANES), BLOCK, 0, s>>>(g, ms, lanes, out);
+    else k_ms_harvest<<<grid, BLOCK, 0, s>>>(g, ms, lanes, out);
     return launch_status();
 }
 
 extern "C" int ggak_ms_reach(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, hipStream_t s) {
     // REACH partial slots are always all written (idle workgroups store 0)
-    k_ms_reach<<<REACH, BLOCK, 0, s>>>(g, ms, lanes);
+    if (ms.lane_major) k_ms_reach_lm<<<dim3(REACH / LANES, LANES), BLOCK, 0, s>>>(g, ms, lanes);
+    else k_ms_reach<<<REACH, BLOCK, 0, s>>>(g, ms, lanes);
+    return launch_status();
+}
+
+extern "C" int ggak_ms_checksum(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, uint64_t *sums, hipStream_t s) {
+    if (lanes == 0) return 0;
+    if (ms.lane_major) k_ms_checksum_lm<<<dim3(grid_for(g.n, 64), LANES), BLOCK, 0, s>>>(g, ms, lanes, sums);
+    else k_ms_checksum<<<grid_for((uint64_t)g.n * LANES, 1024), BLOCK, 0, s>>>(g, ms, lanes, sums);
     return launch_status();
 }
 

@@ -418,6 +418,34 @@ __global__ __launch_bounds__(BLOCK) void k_reach_fold(gga_dev_work w) {
     }
 }
 
+// Fingerprint of dist: sum over v of mix64(v << 32 | dist[v]) (the host's
+// gga_checksum); partials in word 3 of cnt slots [0, REACH_BLOCKS), folded
+// into ctl[RED + 4] (uint64).
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_checksum(gga_dev_graph g, gga_dev_work w) {
+    __shared__ uint64_t s_red[WAVES];
+    uint64_t h = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        h += mix64((v << 32) | w.dist[v]);
+    h = block_sum64(h, s_red);
+    if (threadIdx.x == 0) w.cnt[4u * blockIdx.x + 3u] = h;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_checksum_fold(gga_dev_work w) {
+    __shared__ uint64_t s_red[WAVES];
+    uint64_t h = 0;
+    for (uint32_t i = threadIdx.x; i < REACH_BLOCKS; i += blockDim.x) h += w.cnt[4u * i + 3u];
+    h = block_sum64(h, s_red);
+    if (threadIdx.x == 0) reinterpret_cast<uint64_t *>(w.ctl + GGA_CTL_RED)[2] = h;
+}
+
 // verifysolution (myutils.h:1-18) plus tightness, over chunks.
 template <bool BFS>
 __global__ __launch_bounds__(BLOCK) void k_verify_edges(gga_dev_graph g, gga_dev_work w,
@@ -573,6 +601,12 @@ extern "C" int ggak_reach(gga_dev_graph g, gga_dev_work w, hipStream_t s) {
     // REACH_BLOCKS partial slots are always all written (idle workgroups store 0)
     k_reach<<<REACH_BLOCKS, BLOCK, 0, s>>>(g, w);
     k_reach_fold<<<1, BLOCK, 0, s>>>(w);
+    return launch_status();
+}
+
+extern "C" int ggak_checksum(gga_dev_graph g, gga_dev_work w, hipStream_t s) {
+    k_checksum<<<REACH_BLOCKS, BLOCK, 0, s>>>(g, w);
+    k_checksum_fold<<<1, BLOCK, 0, s>>>(w);
     return launch_status();
 }
 

@@ -265,6 +265,23 @@ class DeviceGraph:
         return out, _stats_dict(st)
 
 
+    def tune(self, src: int, bfs: bool = False, layout: bool = False) -> dict:
+        """gga_tune: times split thresholds (seeded by the histogram heuristic),
+        grid and push/compaction switch on queries from src; keeps the best."""
+        r = _lib.TuneResult()
+        flags = (1 if bfs else 0) | (2 if layout else 0)
+        check(load().gga_tune(self._h, src, flags, ctypes.byref(r)), "gga_tune")
+        out = {f: getattr(r, f) for f, _ in r._fields_ if f not in ("tried_mda", "tried_ms", "layout_ms")}
+        k = min(r.ntried, 16)
+        out["tried"] = {int(r.tried_mda[i]): float(r.tried_ms[i]) for i in range(k)}
+        out["layout_ms"] = (float(r.layout_ms[0]), float(r.layout_ms[1]))
+        self.split_stats = None
+        return out
+
+    def tuned_layout(self) -> str:
+        return {1: "vertex", 2: "lane"}[int(load().gga_tune_layout(self._h))]
+
+
 def _opts(mode, inf: int, count: bool, kw: dict) -> _lib.Opts:
     o = _lib.Opts()
     o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
@@ -281,11 +298,13 @@ class MultiSourceEngine:
 
     LANES = 32
 
-    def __init__(self, dg: DeviceGraph, inf: int = INF):
+    LAYOUTS = {"auto": 0, "vertex": 1, "lane": 2}
+
+    def __init__(self, dg: DeviceGraph, inf: int = INF, layout: str = "auto"):
         self.dg = dg
         self.n = dg.n
         self._h = ctypes.c_void_p()
-        check(load().gga_ms_create(dg._h, inf, ctypes.byref(self._h)), "gga_ms_create")
+        check(load().gga_ms_create(dg._h, inf, self.LAYOUTS[layout], ctypes.byref(self._h)), "gga_ms_create")
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

@@ -167,6 +167,7 @@ typedef struct gga_opts {
                              queue pushes; 0 -> max(4096, n / 64); 0xFFFFFFFF -> never */
     uint32_t pull_alpha;  /* gga_multi_bfs: a level runs bottom-up (pull over in-edges) when the
                              frontier's out-edges x pull_alpha > m; 0 -> 14; 0xFFFFFFFF -> never */
+    uint32_t ms_layout;   /* gga_multi_sp: GGA_MS_* engine layout (0 -> GGA_MS_AUTO) */
 } gga_opts;
 /* Fills defaults (all zero = defaults). */
 void gga_opts_init(gga_opts *o);
@@ -214,6 +215,9 @@ int gga_verify(uint32_t n, const uint32_t *row_ptr, const uint32_t *dst, const u
  * a host copy is wasteful). */
 int gga_graph_verify(gga_graph *g, uint32_t src, int bfs, uint32_t inf, gga_verify_result *r);
 
+/* gga_checksum of the last result, computed on the device. */
+int gga_graph_checksum(gga_graph *g, uint64_t *sum);
+
 /* Distance file: one "%u\n" per original vertex (Ary2File, myutils.h:51-62). */
 int gga_write_dist(const char *path, const uint32_t *dist, uint32_t n);
 
@@ -244,8 +248,16 @@ void gga_graph_name(const char *path, char *out, size_t cap);
 typedef struct gga_ms gga_ms;
 /* Engine on a resident graph (split with the defaults first if it is not).
  * Device memory: 136 B per vertex + 8 B per chunk + the per-tile scratch.
- * inf: unreached value (0 -> GGA_INF).  Re-splitting the graph invalidates it. */
-int gga_ms_create(gga_graph *g, uint32_t inf, gga_ms **out);
+ * inf: unreached value (0 -> GGA_INF).  layout: GGA_MS_VERTEX_MAJOR (the 32
+ * lanes of a vertex in one 128-byte line: lanes that meet at the same
+ * vertices share every gather -- small-world graphs), GGA_MS_LANE_MAJOR (each
+ * lane's distances contiguous: lanes whose frontiers rarely overlap, e.g.
+ * meshes, gather only their own 4 bytes), or GGA_MS_AUTO (the tuner's choice
+ * for the graph, gga_tune_layout).  Re-splitting the graph invalidates it. */
+#define GGA_MS_AUTO          0
+#define GGA_MS_VERTEX_MAJOR  1
+#define GGA_MS_LANE_MAJOR    2
+int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out);
 int gga_ms_free(gga_ms *e);
 /* Restarts the lanes in `lanes` (bit i = lane i) at src[i] (src holds 32
  * entries; only those of `lanes` are read).  Lanes in unit_lanes count hops
@@ -257,8 +269,25 @@ int gga_ms_start(gga_ms *e, uint32_t lanes, const uint32_t *src, uint32_t unit_l
  * cooperative launch).  o may be NULL.  *live = lanes with work left; a lane
  * started and not live has finished. */
 int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live);
+/* Asynchronous form of a run of nsteps steps, for callers that drive several
+ * engines (graphs) at once: launch enqueues the steps and a read-back on the
+ * graph's stream and returns; collect waits for them and reports *live as
+ * gga_ms_run does.  Every launch is followed by one collect. */
+int gga_ms_launch(gga_ms *e, uint32_t nsteps, const gga_opts *o);
+int gga_ms_collect(gga_ms *e, uint32_t *live);
+/* 1 when a launch has completed (collect will not wait) or none is pending,
+ * 0 while it runs, < 0 on error. */
+int gga_ms_ready(gga_ms *e);
 /* Lanes started and finished since their last start (not yet restarted). */
 uint32_t gga_ms_done(const gga_ms *e);
+/* Fingerprints of finished lanes' distances: sums[i] = gga_checksum of lane
+ * i's n values for i in lanes (0 elsewhere; sums holds 32). */
+int gga_ms_checksum(gga_ms *e, uint32_t lanes, uint64_t *sums);
+/* Device bytes of an engine on a graph with n vertices and `chunks` chunks. */
+uint64_t gga_ms_bytes(uint32_t n, uint32_t chunks);
+/* sum over v of splitmix64(v << 32 | dist[v]) mod 2^64: an order-independent
+ * fingerprint used to compare results without moving them. */
+uint64_t gga_checksum(const uint32_t *dist, uint32_t n);
 /* Copies the distances of finished lanes: out[i] (n uint32) for each lane i in
  * `lanes`; host pointers, or device pointers when on_device. */
 int gga_ms_fetch(gga_ms *e, uint32_t lanes, uint32_t *const *out, int on_device);
@@ -285,6 +314,111 @@ int gga_multi_sp(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t unit_l
 #define GGA_MSBFS_LANES 256
 int gga_multi_bfs(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *out, const gga_opts *o,
                   gga_stats *st);
+
+/* ------------------------------------------------------------ autotuner
+ * Replaces the histogram guess getMaxAllowedOutDegree (graph.h:359-385) and
+ * the NodeSplitLevel sweep of mysc (mysc:20-51): the histogram threshold
+ * (10 buckets, bfs_sssp.cu:166) seeds the candidates {seed, 16, 32, 64, 128,
+ * 256, 1024, maxdeg}; each is timed on one query from src (best of 2 device
+ * times), the fastest split is kept, then the worklist grid (default, 1/2,
+ * 1/4) and the push / compaction switch (dense_min x 1/4, x 4) are timed the
+ * same way.  The graph keeps the choices: later queries whose opts leave grid
+ * and dense_min at 0 use them.  GGA_TUNE_LAYOUT also times both multi-source
+ * engine layouts on 32 spread sources.  Results never depend on any choice. */
+#define GGA_TUNE_BFS     1u   /* tune with BFS queries (default SSSP) */
+#define GGA_TUNE_LAYOUT  2u   /* also measure the multi-source layout */
+#define GGA_TUNE_MAX     16
+typedef struct gga_tune_result {
+    uint32_t mda;             /* chosen split threshold (the graph is left split with it) */
+    uint32_t grid;            /* chosen worklist grid (workgroups of 256 threads) */
+    uint32_t dense_min;       /* chosen push / compaction switch */
+    uint32_t layout;          /* GGA_MS_VERTEX_MAJOR or GGA_MS_LANE_MAJOR */
+    uint32_t seed_mda;        /* the histogram heuristic's threshold */
+    uint32_t ntried;          /* split thresholds timed */
+    float    seed_ms, best_ms;
+    float    layout_ms[2];    /* vertex-major, lane-major (GGA_TUNE_LAYOUT) */
+    uint32_t tried_mda[GGA_TUNE_MAX];
+    float    tried_ms[GGA_TUNE_MAX];
+} gga_tune_result;
+int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r);
+/* Engine layout for the graph: the measured one after gga_tune with
+ * GGA_TUNE_LAYOUT, else vertex-major. */
+uint32_t gga_tune_layout(gga_graph *g);
+
+/* ------------------------------------------------------------- serving
+ * No reference counterpart (one query per process in the reference).  A
+ * server keeps graphs resident in HBM and answers BFS/SSSP queries on them:
+ * each graph gets, when a query needs it, one 32-lane multi-source engine
+ * whose device memory (gga_ms_bytes) is admitted against hbm_budget, evicting
+ * idle engines least-recently-used first; a query takes one lane from
+ * admission to convergence, and lanes freed by finished queries are refilled
+ * from the waiting queries at the next round (continuous batching).  Every
+ * round launches `steps_per_round` steps on every engine with running lanes
+ * (one stream per graph, so graphs overlap on the device).  Results do not
+ * depend on the mix: each lane is the single-source computation. */
+typedef struct gga_server gga_server;
+typedef struct gga_server_opts {
+    uint64_t hbm_budget;      /* bytes of engine memory (0 -> 8 GiB) */
+    uint32_t steps_per_round; /* 0 -> 6 */
+    uint32_t lanes;           /* lanes used per engine, 1..32 (0 -> 32) */
+    uint32_t use_graph;       /* steps replayed from captured hipGraphs */
+    uint32_t dense_min;       /* engines' push / compaction switch (0 -> default) */
+    uint32_t host_results;    /* queries may ask for host copies: staging counted in the budget */
+    uint32_t dry_run;         /* no device: engines are only accounted and a query
+                                 completes dry_steps rounds after admission (tests of
+                                 admission and batching on hosts without a GPU) */
+    uint32_t dry_steps;       /* 0 -> 3 */
+    uint32_t layout;          /* GGA_MS_* layout of every engine (0 -> per graph, GGA_MS_AUTO) */
+} gga_server_opts;
+typedef struct gga_query {
+    uint32_t graph;           /* id from gga_server_add_graph */
+    uint32_t src;
+    uint32_t bfs;             /* 1: BFS levels, 0: SSSP */
+    uint32_t out_on_device;   /* out is a device pointer */
+    double   arrival;         /* server clock (s); <= 0: now.  Replays: offset from the start */
+    uint32_t *out;            /* n uint32 for the result, or NULL for its fingerprint only */
+} gga_query;
+typedef struct gga_query_result {
+    uint64_t id;
+    uint32_t graph, src, bfs, lane;
+    int32_t  status;          /* GGA_OK; GGA_EBUDGET if its engine exceeds the whole budget */
+    uint32_t pad;
+    double   arrival, admitted, done;   /* seconds (replays: from the replay's start) */
+    uint64_t checksum;        /* gga_checksum of the distances */
+} gga_query_result;
+typedef struct gga_server_counters {
+    uint64_t submitted, completed, rejected, rounds, evictions, engines_created;
+    uint64_t bytes_in_use, peak_bytes;
+    uint32_t engines_resident, pad;
+} gga_server_counters;
+
+int gga_server_create(const gga_server_opts *o, gga_server **out);
+int gga_server_free(gga_server *s);
+/* The graph stays owned by the caller and must outlive the server. */
+int gga_server_add_graph(gga_server *s, gga_graph *g, uint32_t *id);
+/* Dry-run servers: a graph known only by its size (n vertices, chunks). */
+int gga_server_add_graph_dry(gga_server *s, uint32_t n, uint32_t chunks, uint32_t *id);
+/* Seconds since the server was created. */
+double gga_server_now(const gga_server *s);
+/* Queues a query (thread-safe); *id identifies its result. */
+int gga_server_submit(gga_server *s, const gga_query *q, uint64_t *id);
+/* One round: admission, lane restarts, steps, result delivery.  *active =
+ * lanes still running. */
+int gga_server_pump(gga_server *s, uint32_t *active);
+/* Up to cap finished results, oldest first. */
+int gga_server_poll(gga_server *s, gga_query_result *res, uint32_t cap, uint32_t *count);
+/* Pumps until no query waits or runs. */
+int gga_server_drain(gga_server *s);
+/* Open-loop replay: trace[i] is submitted when the clock reaches its arrival
+ * offset (trace sorted by arrival), rounds run meanwhile; res[i] is trace[i]'s
+ * result. */
+int gga_server_replay(gga_server *s, const gga_query *trace, uint32_t nq, gga_query_result *res);
+/* The same trace answered one query at a time with gga_bfs / gga_sssp in
+ * `mode` (GGA_MODE_PERSISTENT: one launch per query) -- the baseline the
+ * batched server is measured against. */
+int gga_server_replay_plain(gga_server *s, const gga_query *trace, uint32_t nq, int32_t mode,
+                            gga_query_result *res);
+int gga_server_stats(const gga_server *s, gga_server_counters *st);
 
 #ifdef __cplusplus
 }

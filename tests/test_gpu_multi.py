@@ -41,18 +41,23 @@ def nonisolated(g, k, seed):
 DENSE = {"auto": 0, "flags": 1, "queue": 0xFFFFFFFF}
 
 
+LAYOUT = {"vertex": 1, "lane": 2}
+
+
+@pytest.mark.parametrize("layout", list(LAYOUT))
 @pytest.mark.parametrize("rep", list(DENSE))
 @pytest.mark.parametrize("mode", ["worklist", "persistent"])
 @pytest.mark.parametrize("T", [1, 16, 256])
-def test_multi_sp_rmat_each_lane_equals_oracle(gpu, mode, T, rep):
+def test_multi_sp_rmat_each_lane_equals_oracle(gpu, mode, T, rep, layout):
     g = gga.gen_rmat(14, 16, seed=13, wlo=1, whi=100)
     srcs = nonisolated(g, 32, T)
     srcs[5] = srcs[4]                    # duplicate source in two lanes
     bfs_lanes = 0xA5A5A5A5
     with gga.DeviceGraph(g) as dg:
         dg.split(mda=T)
-        announce(f"multi_sp rmat14 T={T} mode={mode} lists={rep}")
-        out, st = dg.multi_sp(srcs, bfs_lanes=bfs_lanes, mode=mode, count=True, dense_min=DENSE[rep])
+        announce(f"multi_sp rmat14 T={T} mode={mode} lists={rep} layout={layout}")
+        out, st = dg.multi_sp(srcs, bfs_lanes=bfs_lanes, mode=mode, count=True, dense_min=DENSE[rep],
+                              ms_layout=LAYOUT[layout])
         reached = 0
         mreach = 0
         for i, s in enumerate(srcs):
@@ -92,7 +97,7 @@ def test_multi_sp_hipgraph_replay_and_ref_inf(gpu):
         for rep in range(3):
             announce(f"multi_sp grid60x70 hipgraph replay {rep}")
             out, st = dg.multi_sp(srcs, bfs_lanes=0xFFFF0000, mode="worklist", use_graph=1,
-                                  dense_min=list(DENSE.values())[rep])
+                                  dense_min=list(DENSE.values())[rep], ms_layout=1 + rep % 2)
             for i, s in enumerate(srcs):
                 np.testing.assert_array_equal(out[i], want_for(g, s, i >= 16), err_msg=f"lane {i}")
         announce("multi_sp grid60x70 inf=1e9")
@@ -103,7 +108,8 @@ def test_multi_sp_hipgraph_replay_and_ref_inf(gpu):
             np.testing.assert_array_equal(out[i], want)
 
 
-def test_engine_lanes_recycled_while_others_run(gpu):
+@pytest.mark.parametrize("layout", list(LAYOUT))
+def test_engine_lanes_recycled_while_others_run(gpu, layout):
     """Continuous batching at the engine level: lanes finish at different
     steps, are harvested and restarted with new sources (BFS or SSSP) while the
     remaining lanes keep running; every result equals the single-source one."""
@@ -114,7 +120,7 @@ def test_engine_lanes_recycled_while_others_run(gpu):
     results = {}
     with gga.DeviceGraph(g) as dg:
         dg.split(mda=32)
-        with gga.MultiSourceEngine(dg) as eng:
+        with gga.MultiSourceEngine(dg, layout=layout) as eng:
             lane_q = {}
             pending = list(enumerate(queue))
             steps = 0
@@ -215,3 +221,24 @@ def test_multi_bfs_directed_grid_and_isolated(gpu):
             np.testing.assert_array_equal(out[i], want_for(g, s, True), err_msg=f"lane {i}")
         with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
             dg.multi_bfs(np.zeros(257, dtype=np.uint32))
+
+
+def test_autotuner_keeps_results_and_seeds_with_histogram(gpu):
+    """gga_tune: the histogram heuristic (10 buckets) is the seed, the chosen
+    threshold is one of the timed candidates, and every query after tuning
+    (single source and multi-source in the chosen layout) equals the oracle."""
+    for g, src in ((gga.gen_rmat(14, 16, seed=13, wlo=1, whi=100), 0),
+                   (gga.gen_grid(80, 90, 0.005, seed=11, wlo=1, whi=1000), 0)):
+        with gga.DeviceGraph(g) as dg:
+            announce(f"tune n={g.n}")
+            r = dg.tune(src, bfs=False, layout=True)
+            assert r["seed_mda"] == oracle.mda_histogram(g.degrees(), 10)
+            assert r["mda"] in r["tried"] and r["best_ms"] <= r["seed_ms"]
+            assert r["layout"] in (1, 2) and dg.tuned_layout() == {1: "vertex", 2: "lane"}[r["layout"]]
+            want = want_for(g, src, False)
+            np.testing.assert_array_equal(dg.sssp(src)[0], want)
+            np.testing.assert_array_equal(dg.bfs(src)[0], want_for(g, src, True))
+            srcs = nonisolated(g, 32, 1)
+            out, _ = dg.multi_sp(srcs, bfs_lanes=0xF0F0F0F0)
+            for i, s in enumerate(srcs):
+                np.testing.assert_array_equal(out[i], want_for(g, s, (0xF0F0F0F0 >> i) & 1))

@@ -1,0 +1,150 @@
+"""Serving loop policy on the CPU (dry-run servers: engines are only accounted,
+a query completes a fixed number of rounds after admission): admission by the
+device-memory budget, eviction of idle engines, rejection of queries whose
+engine exceeds the whole budget, continuous refilling of freed lanes, no
+head-of-line blocking across graphs, and the fingerprint the GPU results use.
+"""
+import numpy as np
+import pytest
+
+import gpugraphalgo_amd as gga
+from gpugraphalgo_amd import _lib
+from gpugraphalgo_amd.serve import Server, TRACE_DTYPE, checksum, poisson_trace
+
+
+def engine_bytes(n, chunks):
+    return int(_lib.load().gga_ms_bytes(n, chunks))
+
+
+def trace_of(rows):
+    t = np.zeros(len(rows), dtype=TRACE_DTYPE)
+    for i, (g, s, b, a) in enumerate(rows):
+        t[i] = (g, s, b, a)
+    return t
+
+
+def test_engine_bytes_formula():
+    n, c = 1000, 1500
+    ntiles = (n + 2047) // 2048
+    assert engine_bytes(n, c) == 128 * n + 8 * n + 8 * c + 4 * (ntiles + 1) + 4 * 44 * 32 + 32 * 1024 + 8 * 32
+
+
+def test_budget_never_exceeded_and_all_complete():
+    sizes = [(1000, 1200), (5000, 6000), (20000, 21000), (3000, 3100)]
+    budget = engine_bytes(20000, 21000) + engine_bytes(1000, 1200)   # fits the big one + one small
+    with Server(hbm_budget=budget, dry_run=True, dry_steps=2) as s:
+        ids = [s.add_graph_dry(n, c) for n, c in sizes]
+        rng = np.random.default_rng(0)
+        rows = [(int(g), int(rng.integers(0, sizes[g][0])), int(rng.integers(0, 2)), 0.0)
+                for g in rng.integers(0, len(ids), 200)]
+        res = s.replay(trace_of(rows))
+        st = s.stats()
+    assert (res["status"] == 0).all()
+    assert st["peak_bytes"] <= budget
+    assert st["evictions"] > 0
+    assert st["completed"] == 200
+    # a query is admitted no earlier than it arrived and completes after admission
+    assert (res["admitted"] >= res["arrival"]).all() and (res["done"] >= res["admitted"]).all()
+
+
+def test_engine_larger_than_budget_is_rejected():
+    with Server(hbm_budget=engine_bytes(1000, 1000), dry_run=True) as s:
+        small = s.add_graph_dry(1000, 1000)
+        big = s.add_graph_dry(100000, 100000)
+        res = s.replay(trace_of([(big, 5, 1, 0.0), (small, 3, 0, 0.0), (big, 7, 0, 0.0)]))
+    assert list(res["status"]) == [_lib.GGA_EBUDGET, 0, _lib.GGA_EBUDGET]
+
+
+def test_freed_lanes_are_refilled_continuously():
+    """4 lanes, 10 queries at once on one graph, 3 collected step batches
+    each: admission in waves of 4, each wave admitted in the round that frees
+    the lanes, FIFO order."""
+    with Server(lanes=4, dry_run=True, dry_steps=3) as s:
+        g = s.add_graph_dry(100, 100)
+        for i in range(10):
+            s.submit(g, i, i % 2 == 0)
+        rounds_seen = []
+        done = []
+        while True:
+            act = s.pump()
+            r = s.poll()
+            done += list(r["id"])
+            rounds_seen.append((act, len(done)))
+            if len(done) == 10:
+                break
+            assert len(rounds_seen) < 50
+        st = s.stats()
+    assert done == list(range(10))                 # FIFO within one graph
+    assert max(a for a, _ in rounds_seen) <= 4     # never more lanes than allowed
+    # round 0 launches; rounds 1-3 collect; the lanes freed in round 3 restart in round 3
+    assert [d for _, d in rounds_seen[:10]] == [0, 0, 0, 4, 4, 4, 8, 8, 8, 10]
+    assert st["engines_created"] == 1
+
+
+def test_no_head_of_line_blocking_across_graphs():
+    """Graph A's engine is full; a later query of graph B is admitted in the
+    same round instead of waiting behind A's queue."""
+    with Server(lanes=1, dry_run=True, dry_steps=5) as s:
+        a = s.add_graph_dry(100, 100)
+        b = s.add_graph_dry(100, 100)
+        s.submit(a, 1, True)
+        s.submit(a, 2, True)     # waits for A's only lane
+        s.submit(b, 3, False)    # must not wait behind it
+        s.pump()
+        assert s.stats()["engines_created"] == 2
+        s.drain()
+        r = s.poll()
+    by_id = {int(i): k for k, i in enumerate(r["id"])}
+    adm = {i: r["admitted"][k] for i, k in by_id.items()}
+    done = {i: r["done"][k] for i, k in by_id.items()}
+    assert adm[1] >= done[0]          # A's second query waited for A's lane
+    assert adm[2] == adm[0]           # B's query was admitted in the first round
+
+
+def test_queries_wait_for_busy_engines_then_run():
+    """Budget for one engine: B's query waits until A's lanes drain, then A is evicted."""
+    with Server(hbm_budget=engine_bytes(100, 100), lanes=2, dry_run=True, dry_steps=2) as s:
+        a = s.add_graph_dry(100, 100)
+        b = s.add_graph_dry(100, 100)
+        res = s.replay(trace_of([(a, 0, 1, 0.0), (a, 1, 1, 0.0), (b, 2, 0, 0.0), (a, 3, 0, 0.0)]))
+        st = s.stats()
+    assert (res["status"] == 0).all()
+    assert res["admitted"][2] >= max(res["done"][0], res["done"][1])
+    assert st["evictions"] >= 1 and st["peak_bytes"] <= engine_bytes(100, 100)
+
+
+def test_submit_validates_graph_and_source():
+    with Server(dry_run=True) as s:
+        g = s.add_graph_dry(10, 10)
+        with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+            s.submit(g + 1, 0, True)
+        with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+            s.submit(g, 10, True)
+
+
+def test_poisson_trace_shape():
+    t = poisson_trace([100, 200, 300], 1000, rate=500.0, seed=3)
+    assert t["arrival"][0] == 0.0 and (np.diff(t["arrival"]) >= 0).all()
+    assert 1.5 < t["arrival"][-1] < 2.5                     # ~1000 / 500 s
+    assert set(np.unique(t["graph"])) == {0, 1, 2}
+    assert 0.4 < t["bfs"].mean() < 0.6
+    assert (t["src"] < np.array([100, 200, 300])[t["graph"]]).all()
+
+
+def test_checksum_is_order_independent_fingerprint():
+    rng = np.random.default_rng(5)
+    d = rng.integers(0, 2**32, 1000, dtype=np.uint64).astype(np.uint32)
+
+    def mix(x):
+        x = (x + np.uint64(0x9E3779B97F4A7C15))
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+    with np.errstate(over="ignore"):
+        v = np.arange(d.size, dtype=np.uint64)
+        want = int(mix((v << np.uint64(32)) | d.astype(np.uint64)).sum(dtype=np.uint64))
+    assert checksum(d) == want
+    e = d.copy()
+    e[17] ^= 1
+    assert checksum(e) != checksum(d)
