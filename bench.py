@@ -105,6 +105,116 @@ def cpu_baseline(g, src, m_reach):
                        f"{os.cpu_count()} host cores")}, d
 
 
+def batched_leg(gga, np, quick=False):
+    """Config 4: 256 BFS (bitmask lanes, one cooperative launch) + 32 SSSP
+    (32-lane engine, hipGraph-replayed steps) on R-MAT s20 ef16 seed 13,
+    sources uniform among vertices with out-edges; a sample of lanes is
+    checked bit-exact against single-source runs of the same sources."""
+    g = gga.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=13, wlo=1, whi=100)
+    rng = np.random.default_rng(13)
+    cand = np.flatnonzero(g.degrees() > 0)
+    bsrc = rng.choice(cand, 256, replace=False).astype(np.uint32)
+    ssrc = rng.choice(cand, 32, replace=False).astype(np.uint32)
+    out = {"workload": "R-MAT s20 ef16 seed 13, 256 BFS + 32 SSSP sources (config 4)", "n": g.n, "m": g.m}
+    with gga.DeviceGraph(g) as dg:
+        tr = dg.tune(int(ssrc[0]), bfs=False, layout=not quick)
+        out["tuned"] = {"split_threshold": tr["mda"], "seed_threshold": tr["seed_mda"], "grid": tr["grid"],
+                        "dense_min": tr["dense_min"], "layout": {1: "vertex", 2: "lane"}[tr["layout"]]}
+        reps = 3 if quick else 5
+        for _ in range(2):
+            dg.multi_bfs(bsrc, keep_on_device=True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, sb = dg.multi_bfs(bsrc, keep_on_device=True)
+        wb = (time.perf_counter() - t0) / reps
+        for _ in range(2):
+            dg.multi_sp(ssrc, use_graph=1, keep_on_device=True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, ss = dg.multi_sp(ssrc, use_graph=1, keep_on_device=True)
+        ws = (time.perf_counter() - t0) / reps
+        # single-source baselines on the same sources (worklist mode, result kept on the device)
+        t0 = time.perf_counter()
+        mr1 = sum(dg.bfs(int(s), keep_on_device=True)[1]["m_reach"] for s in bsrc[:32])
+        w1b = (time.perf_counter() - t0) / 32
+        t0 = time.perf_counter()
+        mr2 = sum(dg.sssp(int(s), keep_on_device=True)[1]["m_reach"] for s in ssrc)
+        w1s = (time.perf_counter() - t0) / 32
+        lv, _ = dg.multi_bfs(bsrc)
+        ds, _ = dg.multi_sp(ssrc, use_graph=1)
+        for i in range(0, 256, 37):
+            assert np.array_equal(lv[i], dg.bfs(int(bsrc[i]))[0]), f"BFS lane {i}"
+        for i in range(0, 32, 5):
+            assert np.array_equal(ds[i], dg.sssp(int(ssrc[i]))[0]), f"SSSP lane {i}"
+    out["bfs256"] = {"ms": round(wb * 1e3, 3), "gteps": round(sb["m_reach"] / wb / 1e9, 2),
+                     "levels": sb["iterations"], "pull_levels": sb["pull_levels"],
+                     "single_source_ms_per_query": round(w1b * 1e3, 3),
+                     "speedup_vs_single": round(w1b * 256 / wb, 2)}
+    out["sssp32"] = {"ms": round(ws * 1e3, 3), "gteps": round(ss["m_reach"] / ws / 1e9, 2),
+                     "steps": ss["iterations"], "single_source_ms_per_query": round(w1s * 1e3, 3),
+                     "speedup_vs_single": round(w1s * 32 / ws, 2)}
+    out["checked_lanes_bit_exact"] = len(range(0, 256, 37)) + len(range(0, 32, 5))
+    del mr1, mr2
+    return out
+
+
+def serving_leg(gga, np, full=False, quick=False):
+    """Config 5 serving: resident R-MAT graphs (s16.. ef16, weights [1,100])
+    and 4-neighbour grids (weights [1,1000]), seed 17; BFS/SSSP 50/50, uniform
+    graph and source (among vertices with out-edges).  The server (32-lane
+    engines, HBM-budget admission, continuous batching) against the same
+    queries answered one persistent launch per query."""
+    from gpugraphalgo_amd.serve import Server, latency_summary, poisson_trace
+    scales = range(16, 24) if full else range(16, 21)
+    sides = (512, 1024, 2048, 4096) if full else (512, 1024)
+    gs = [("rmat%d" % s, gga.gen_rmat(s, 16, 0.57, 0.19, 0.19, seed=17, wlo=1, whi=100)) for s in scales]
+    gs += [("grid%d" % a, gga.gen_grid(a, a, 0.0, seed=17, wlo=1, whi=1000)) for a in sides]
+    dgs = []
+    for name, g in gs:
+        dg = gga.DeviceGraph(g)
+        if name.startswith("rmat"):
+            dg.split(mda=64)
+        else:
+            dg.split()
+        dgs.append(dg)
+    sizes = [g.n for _, g in gs]
+    degs = [np.flatnonzero(g.degrees() > 0).astype(np.uint32) for _, g in gs]
+    nq = 120 if quick else (2000 if full else 400)
+    nplain = 40 if quick else (200 if full else 80)
+    out = {"workload": ("config 5 resident set" if full else
+                        "reduced config 5 set: R-MAT s16-s20 + grids 512^2, 1024^2") +
+                       ", BFS/SSSP 50/50, uniform graph and source",
+           "graphs": [n for n, _ in gs], "resident_gb": round(sum(d.device_bytes for d in dgs) / 1e9, 3)}
+    with Server(hbm_budget=16 << 30, steps_per_round=6, use_graph=True) as srv:
+        for dg in dgs:
+            srv.add_graph(dg)
+        sat = poisson_trace(sizes, nq, rate=0, seed=21, degrees=degs)
+        srv.replay(sat[: min(48, nq)])                    # engines created, graphs captured
+        r = srv.replay(sat)
+        s = latency_summary(r)
+        p = srv.replay_plain(sat[:nplain])
+        sp = latency_summary(p)
+        assert np.array_equal(p["checksum"], r["checksum"][:nplain]), "served != one-query-per-launch"
+        out["saturating"] = {"queries": nq, "qps": round(s["qps"], 1), "p50_ms": round(s["p50_ms"], 2),
+                             "p99_ms": round(s["p99_ms"], 2)}
+        out["plain_one_launch_per_query"] = {"queries": nplain, "qps": round(sp["qps"], 1),
+                                             "p50_ms": round(sp["p50_ms"], 2), "p99_ms": round(sp["p99_ms"], 2)}
+        out["speedup_qps"] = round(s["qps"] / sp["qps"], 2) if sp["qps"] else None
+        out["results_bit_exact_vs_plain"] = nplain
+        rates = (0.5, 0.9) if not quick else (0.5,)
+        out["poisson"] = []
+        for f in rates:
+            rate = max(1.0, f * s["qps"])
+            tr = poisson_trace(sizes, nq, rate=rate, seed=31, degrees=degs)
+            rr = latency_summary(srv.replay(tr))
+            out["poisson"].append({"offered_qps": round(rate, 1), "achieved_qps": round(rr["qps"], 1),
+                                   "p50_ms": round(rr["p50_ms"], 2), "p99_ms": round(rr["p99_ms"], 2)})
+        out["server"] = {k: v for k, v in srv.stats().items() if k in ("rounds", "engines_created", "peak_bytes")}
+    for dg in dgs:
+        dg.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +229,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="profiling run: one T, no CPU leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the multi-source leg (config 4)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg (config 5)")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="serving leg on the full config 5 set (R-MAT s16-s23, grids up to 4096^2)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times), to scale rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count")
@@ -169,6 +283,16 @@ def main() -> int:
         assert np.array_equal(d_cpu, d_gpu), "GPU distances differ from the CPU reference"
         log(f"cpu: {cpu['value']:.4f} GTEPS; GPU result bit-exact")
 
+    batched = serving = None
+    if not (args.no_batched or args.quick):
+        log("batched multi-source leg (config 4)")
+        batched = batched_leg(gga, np)
+        log(f"batched: {json.dumps(batched)}")
+    if not (args.no_serve or args.quick):
+        log("serving leg (config 5%s)" % ("" if args.serve_full else ", reduced set"))
+        serving = serving_leg(gga, np, full=args.serve_full)
+        log(f"serving: {json.dumps(serving)}")
+
     traffic = None
     if os.path.exists(TRAFFIC) and r["mode"] == 1:
         from gpugraphalgo_amd.profiling import read_summary
@@ -212,6 +336,8 @@ def main() -> int:
                      "relaxations": r["relaxations"]},
         },
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line))
     return 0

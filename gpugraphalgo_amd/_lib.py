@@ -44,10 +44,20 @@ class SplitStats(ctypes.Structure):
                 ("max_chunk", u32), ("split_ms", ctypes.c_float)]
 
 
+class WgRecord(ctypes.Structure):
+    _fields_ = [("start", u64), ("ticks", u32), ("place", u32)]
+
+
+class ConvRow(ctypes.Structure):
+    _fields_ = [("step", u32), ("diff_count", u32), ("norm2", ctypes.c_double)]
+
+
 class Opts(ctypes.Structure):
     _fields_ = [("mode", i32), ("inf", u32), ("max_iters", u32), ("block", u32),
                 ("sync_every", u32), ("use_graph", u32), ("count", u32), ("grid", u32),
-                ("dense_min", u32), ("pull_alpha", u32), ("ms_layout", u32)]
+                ("dense_min", u32), ("pull_alpha", u32), ("ms_layout", u32),
+                ("timeline", ctypes.c_void_p), ("timeline_cap", u32), ("conv_cap", u32),
+                ("conv_final", ctypes.c_void_p), ("conv_rows", ctypes.c_void_p)]
 
 
 class Stats(ctypes.Structure):
@@ -55,7 +65,8 @@ class Stats(ctypes.Structure):
                 ("ktime_ms", ctypes.c_float), ("sktime_ms", ctypes.c_float),
                 ("ctime_ms", ctypes.c_float), ("mode", i32),
                 ("edges_inspected", u64), ("chunks_expanded", u64), ("relaxations", u64),
-                ("m_reach", u64), ("launches", u64), ("pull_levels", u32)]
+                ("m_reach", u64), ("launches", u64), ("pull_levels", u32), ("timeline_records", u32),
+                ("timeline_grid", u32), ("conv_steps", u32)]
 
 
 class ServerOpts(ctypes.Structure):
@@ -131,6 +142,8 @@ SIGNATURES = [
     ("gga_verify", ctypes.c_int, [u32, vp, vp, vp, u32, vp, u32, P(VerifyResult)]),
     ("gga_graph_verify", ctypes.c_int, [vp, u32, ctypes.c_int, u32, P(VerifyResult)]),
     ("gga_graph_checksum", ctypes.c_int, [vp, P(u64)]),
+    ("gga_timeline_makespan", ctypes.c_double, [vp, u32, u32, u32]),
+    ("gga_write_conv", ctypes.c_int, [ctypes.c_char_p, i32, u32, ctypes.c_char_p, ctypes.c_char_p, vp, u32, u32]),
     ("gga_write_dist", ctypes.c_int, [ctypes.c_char_p, vp, u32]),
     ("gga_write_result_row", ctypes.c_int, [ctypes.c_char_p, P(ResultRow)]),
     ("gga_graph_name", None, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),

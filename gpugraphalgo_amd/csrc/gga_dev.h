@@ -48,6 +48,18 @@ typedef struct gga_dev_work {
     uint32_t pad;
 } gga_dev_work;
 
+/* Per-workgroup timeline record of an instrumented relax step: start on the
+ * 100 MHz device wall clock, duration in its ticks, placement = XCC id << 8 |
+ * the SE / SH / CU bits of HW_ID (bits 8..15). */
+#ifndef GGA_WG_RECORD_DEFINED
+#define GGA_WG_RECORD_DEFINED
+typedef struct gga_wg_record {
+    uint64_t start;
+    uint32_t ticks;
+    uint32_t place;
+} gga_wg_record;
+#endif
+
 /* Multi-source engine: GGA_MS_LANES independent traversals share every step.
  *   D[n * 32]   distances, vertex-major: the 32 lanes of a vertex are one
  *               128-byte line, so relaxing an edge for every lane touches one

@@ -41,8 +41,11 @@ int ggak_scan_u32(const uint32_t *in, uint32_t len, uint32_t *out, uint32_t *d_t
 int ggak_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t s);
 int ggak_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src, uint32_t epoch, hipStream_t s);
 int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int count, hipStream_t s);
+/* tl != NULL: instrumented (counting) variant; workgroup b writes tl[b]. */
 int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t phase, int bfs, int count,
-                 uint32_t grid, hipStream_t s);
+                 uint32_t grid, gga_wg_record *tl, hipStream_t s);
+/* rows[0] += #v with dist != fin, rows[1] += sum of (fin - dist)^2 in uint32. */
+int ggak_conv(const uint32_t *dist, const uint32_t *fin, uint32_t n, uint64_t *rows, hipStream_t s);
 /* After ggak_wl_step(phase): builds the next list from step stamps when that
  * step was a flag step (its list held >= w.dense_min chunks); else a no-op. */
 int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t phase, hipStream_t s);

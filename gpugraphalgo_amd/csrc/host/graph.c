@@ -10,6 +10,7 @@
  * split kernels build on the device (kernels/split.hip).
  */
 #define _GNU_SOURCE
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -507,6 +508,10 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
                     gga_stats *st)
 {
     int rc = GGA_OK;
+    gga_wg_record *d_tl = NULL;
+    uint32_t *d_fin = NULL;
+    uint64_t *d_conv = NULL;
+    uint64_t tl_bytes = 0, fin_bytes = 0, conv_bytes = 0;
     gga_opts o;
     gga_stats s;
     memset(&s, 0, sizeof s);
@@ -577,16 +582,32 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
         }
         uint32_t per = o.sync_every ? o.sync_every : 8u;
         if ((rc = ensure_kev(g, 3 * per))) goto fail;
+        /* instrumentation buffers (freed below, also on failure) */
+        const uint32_t tl_steps = o.timeline && o.timeline_cap >= grid ? o.timeline_cap / grid : 0u;
+        const uint32_t conv_n = o.conv_final && o.conv_rows ? o.conv_cap : 0u;
+        tl_bytes = 16ull * tl_steps * grid;
+        if (tl_steps && (rc = dmalloc(g, (void **)&d_tl, tl_bytes))) goto fail;
+        if (conv_n) {
+            fin_bytes = 4ull * g->n;
+            conv_bytes = 16ull * conv_n;
+            if ((rc = dmalloc(g, (void **)&d_fin, fin_bytes))) goto fail;
+            if ((rc = dmalloc(g, (void **)&d_conv, conv_bytes))) goto fail;
+            HIP_TRY(hipMemcpyAsync(d_fin, o.conv_final, 4ull * g->n, hipMemcpyHostToDevice, g->stream));
+            K_TRY(ggak_fill_u32((uint32_t *)d_conv, 0u, 4ull * conv_n, g->stream));
+        }
         uint32_t step = 0;
         for (;;) {
             /* `per` steps back to back: every kernel reads its list length on
              * the device, so steps after convergence are empty no-ops */
             for (uint32_t k = 0; k < per; k++) {
+                const uint32_t abs_step = step + k;
+                gga_wg_record *tl = abs_step < tl_steps ? d_tl + (uint64_t)abs_step * grid : NULL;
                 HIP_TRY(hipEventRecord(g->kev[3 * k], g->stream));
-                K_TRY(ggak_wl_step(dg, dw, (step + k) % 6u, bfs, count, grid, g->stream));
+                K_TRY(ggak_wl_step(dg, dw, abs_step % 6u, bfs, count, grid, tl, g->stream));
                 HIP_TRY(hipEventRecord(g->kev[3 * k + 1], g->stream));
-                K_TRY(ggak_compact(dg, dw, (step + k) % 6u, g->stream));
+                K_TRY(ggak_compact(dg, dw, abs_step % 6u, g->stream));
                 HIP_TRY(hipEventRecord(g->kev[3 * k + 2], g->stream));
+                if (abs_step < conv_n) K_TRY(ggak_conv(g->d_dist, d_fin, g->n, d_conv + 2ull * abs_step, g->stream));
             }
             step += per;
             HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_TAIL(step % 3u), 4, hipMemcpyDeviceToHost,
@@ -606,6 +627,28 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
                 gga_set_error("worklist mode: %u steps without convergence", step);
                 goto fail;
             }
+        }
+        if (tl_steps) {
+            const uint32_t ns = step < tl_steps ? step : tl_steps;
+            HIP_TRY(hipMemcpy(o.timeline, d_tl, 16ull * ns * grid, hipMemcpyDeviceToHost));
+            s.timeline_records = ns * grid;
+            s.timeline_grid = grid;
+        }
+        if (conv_n) {
+            uint64_t *h = (uint64_t *)malloc(16ull * conv_n);
+            if (!h) {
+                rc = GGA_ENOMEM;
+                goto fail;
+            }
+            HIP_TRY(hipMemcpy(h, d_conv, 16ull * conv_n, hipMemcpyDeviceToHost));
+            const uint32_t ns = step < conv_n ? step : conv_n;
+            for (uint32_t k = 0; k < ns; k++) {
+                o.conv_rows[k].step = k + 1u;   /* after k + 1 relax steps (bfs_sssp.cu:377, 423) */
+                o.conv_rows[k].diff_count = (uint32_t)h[2 * k];
+                o.conv_rows[k].norm2 = sqrt((double)h[2 * k + 1]);
+            }
+            free(h);
+            s.conv_steps = ns;
         }
     } else { /* GGA_MODE_PERSISTENT */
         const uint32_t maxg = wl_grid(g, 1, bfs, count);
@@ -666,10 +709,16 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
     g->epoch += s.iterations + 2u;
     g->have_result = 1;
     if (out) HIP_TRY(hipMemcpy(out, g->d_dist, 4ull * g->n, hipMemcpyDeviceToHost));   /* bfs_sssp.cu:489 */
+    dfree(g, (void **)&d_tl, tl_bytes);
+    dfree(g, (void **)&d_fin, fin_bytes);
+    dfree(g, (void **)&d_conv, conv_bytes);
     if (st) *st = s;
     return GGA_OK;
 fail:
     hipStreamSynchronize(g->stream);
+    dfree(g, (void **)&d_tl, tl_bytes);
+    dfree(g, (void **)&d_fin, fin_bytes);
+    dfree(g, (void **)&d_conv, conv_bytes);
     g->epoch += max_iters < 0x10000000u ? max_iters + 2u : 0x10000000u;
     g->have_result = 0;
     if (st) *st = s;

@@ -285,16 +285,41 @@ __device__ __forceinline__ uint32_t step_len(const gga_dev_work &w, uint32_t ste
     return t < w.qcap ? t : w.qcap;  // > qcap only with QFULL set
 }
 
-template <bool BFS, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+// Placement of the running wave: XCC id << 8 | SE / SH / CU bits of HW_ID.
+__device__ __forceinline__ uint32_t hw_place() {
+    uint32_t hw, xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(hw));
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    return ((xcc & 0xFu) << 8) | ((hw >> 8) & 0xFFu);
+}
+
+// TL: instrumented variant (the reference's my_drelax_metric, cudaKernels.h:2-70,
+// which stores each block's clock64 span): each workgroup writes its start,
+// duration and placement to tl[blockIdx.x].
+template <bool BFS, bool COUNT, bool TL>
+__global__ __launch_bounds__(BLOCK) void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step,
+                                                   gga_wg_record *tl) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
+    uint64_t t0 = 0;
+    if (TL && threadIdx.x == 0) t0 = wall_clock64();
     const uint32_t nq = step_len(w, step);
     const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
     if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
-    if (nq == 0) return;
-    Counters cnt;
-    worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
-    publish_counters<COUNT>(w, cnt);
+    if (nq != 0) {
+        Counters cnt;
+        worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+        publish_counters<COUNT>(w, cnt);
+    }
+    if (TL) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            gga_wg_record r;
+            r.start = t0;
+            r.ticks = (uint32_t)(wall_clock64() - t0);
+            r.place = hw_place();
+            tl[blockIdx.x] = r;
+        }
+    }
 }
 
 // The three compaction passes run after k_wl_step(step); each is a no-op
@@ -535,7 +560,7 @@ extern "C" int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int coun
 // Kernel variants by (bfs, count).
 template <bool BFS, bool COUNT>
 static const void *wl_fn() {
-    return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT>);
+    return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, false>);
 }
 template <bool BFS, bool COUNT>
 static const void *persistent_fn() {
@@ -551,10 +576,42 @@ static const void *pick_kernel(int persistent, int bfs, int count) {
 }
 
 extern "C" int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, int bfs, int count,
-                            uint32_t grid, hipStream_t s) {
+                            uint32_t grid, gga_wg_record *tl, hipStream_t s) {
     if (grid == 0) return (int)hipErrorInvalidValue;
-    void *args[] = {&g, &w, &step};
-    return (int)hipLaunchKernel(pick_kernel(0, bfs, count), dim3(grid), dim3(BLOCK), args, 0, s);
+    void *args[] = {&g, &w, &step, &tl};
+    // the instrumented variant always counts (its timeline goes with the work counters)
+    const void *fn = tl ? (bfs ? reinterpret_cast<const void *>(&k_wl_step<true, true, true>)
+                               : reinterpret_cast<const void *>(&k_wl_step<false, true, true>))
+                        : pick_kernel(0, bfs, count);
+    return (int)hipLaunchKernel(fn, dim3(grid), dim3(BLOCK), args, 0, s);
+}
+
+// Convergence of one step against the final distances (printConv_rtime,
+// convergence.h:100-140): diff = vertices whose distance differs; sq = sum of
+// (final - cur)^2 evaluated in uint32 arithmetic as the reference does with
+// its unsigned foru values (so an unreached vertex contributes
+// (final - inf)^2 mod 2^32).  rows[0] += diff, rows[1] += sq.
+__global__ __launch_bounds__(BLOCK) void k_conv(const uint32_t *dist, const uint32_t *fin, uint32_t n,
+                                                uint64_t *rows) {
+    __shared__ uint64_t s_red[WAVES];
+    uint64_t diff = 0, sq = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t fs = fin[v], fi = dist[v];
+        const uint32_t d = fs - fi;
+        diff += fs != fi;
+        sq += (uint32_t)(d * d);
+    }
+    diff = block_sum64(diff, s_red);
+    sq = block_sum64(sq, s_red);
+    if (threadIdx.x == 0) {
+        if (diff) atomic_add_u64(&rows[0], diff);
+        if (sq) atomic_add_u64(&rows[1], sq);
+    }
+}
+
+extern "C" int ggak_conv(const uint32_t *dist, const uint32_t *fin, uint32_t n, uint64_t *rows, hipStream_t s) {
+    k_conv<<<grid_for(n, 1024), BLOCK, 0, s>>>(dist, fin, n, rows);
+    return launch_status();
 }
 
 extern "C" int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t step, hipStream_t s) {

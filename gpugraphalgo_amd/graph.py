@@ -227,6 +227,28 @@ class DeviceGraph:
              keep_on_device: bool = False, **kw):
         return self._run(load().gga_sssp, src, mode, inf, count, keep_on_device, **kw)
 
+    def trace(self, src: int, bfs: bool = False, timeline_cap: int = 0, conv_final=None,
+              conv_cap: int = 4096, inf: int = INF, **kw):
+        """A worklist-mode query with the instrumentation on: per-workgroup
+        timeline records (start, ticks, placement) for the first
+        timeline_cap // grid steps, and, given the final distances, the
+        per-step convergence rows.  Returns (dist, stats, timeline, conv)."""
+        tl = np.zeros(timeline_cap, dtype=[("start", np.uint64), ("ticks", np.uint32), ("place", np.uint32)])
+        rows = np.zeros(conv_cap if conv_final is not None else 0,
+                        dtype=[("step", np.uint32), ("diff_count", np.uint32), ("norm2", np.float64)])
+        fin = None if conv_final is None else np.ascontiguousarray(conv_final, dtype=_u32)
+        kw = dict(kw)
+        if timeline_cap:
+            kw["timeline"] = tl.ctypes.data
+            kw["timeline_cap"] = timeline_cap
+        if fin is not None:
+            kw["conv_final"] = fin.ctypes.data
+            kw["conv_rows"] = rows.ctypes.data
+            kw["conv_cap"] = conv_cap
+        fn = load().gga_bfs if bfs else load().gga_sssp
+        d, st = self._run(fn, src, "worklist", inf, True, False, **kw)
+        return d, st, tl[: st["timeline_records"]], rows[: st["conv_steps"]]
+
     def copy_dist(self) -> np.ndarray:
         out = np.empty(self.n, dtype=_u32)
         check(load().gga_graph_copy_dist(self._h, _ptr(out)), "copy_dist")
@@ -362,6 +384,20 @@ class MultiSourceEngine:
         return _stats_dict(st)
 
 
-__all__ = ["CsrGraph", "DeviceGraph", "MultiSourceEngine", "csr_from_arrays", "read_gr", "write_gr", "gen_rmat",
+def timeline_makespan(tl: np.ndarray, grid: int, slots: int) -> float:
+    """gga_timeline_makespan: list-scheduling estimate (ms) from timeline records."""
+    tl = np.ascontiguousarray(tl)
+    return float(load().gga_timeline_makespan(tl.ctypes.data_as(ctypes.c_void_p), tl.size, grid, slots))
+
+
+def write_conv(path: str, rows: np.ndarray, level: int, maxdeg: int, graph: str, algo: str,
+               max_dist: int) -> None:
+    """gga_write_conv: the conv.vi block of printConv_rtime_init/_rtime/_final."""
+    rows = np.ascontiguousarray(rows)
+    check(load().gga_write_conv(os.fsencode(path), level, maxdeg, graph.encode(), algo.encode(),
+                                rows.ctypes.data_as(ctypes.c_void_p), rows.size, max_dist), "write_conv")
+
+
+__all__ = ["timeline_makespan", "write_conv", "CsrGraph", "DeviceGraph", "MultiSourceEngine", "csr_from_arrays", "read_gr", "write_gr", "gen_rmat",
            "gen_grid", "verify", "write_dist", "write_result_row", "device_count", "INF",
            "REF_INF"]

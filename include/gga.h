@@ -153,6 +153,28 @@ int gga_graph_chunks(gga_graph *g, uint32_t *chunk_begin, uint32_t *owner);
 #define GGA_MODE_PERSISTENT  2  /* chunk worklist in one cooperative launch with a grid barrier per step */
 #define GGA_MODE_AUTO        3  /* library choice from the degree distribution */
 
+/* Instrumentation (the reference's metrics.h / my_drelax_metric per-block
+ * clock spans, and convergence.h's per-iteration distance to the final
+ * result).  One record per workgroup per relax step: start on the 100 MHz
+ * device wall clock, duration in ticks, placement = XCC id << 8 | SE/SH/CU
+ * bits of the HW_ID register. */
+#ifndef GGA_WG_RECORD_DEFINED
+#define GGA_WG_RECORD_DEFINED
+typedef struct gga_wg_record {
+    uint64_t start;
+    uint32_t ticks;
+    uint32_t place;
+} gga_wg_record;
+#endif
+/* One row per relax step: vertices whose distance still differs from the
+ * final one, and the L2 norm of the difference computed as printConv_rtime
+ * does (convergence.h:100-140: squares of unsigned differences, mod 2^32). */
+typedef struct gga_conv_row {
+    uint32_t step;
+    uint32_t diff_count;
+    double   norm2;
+} gga_conv_row;
+
 typedef struct gga_opts {
     int32_t  mode;        /* GGA_MODE_* (default GGA_MODE_AUTO) */
     uint32_t inf;         /* unreached value: GGA_INF (0 -> default) or GGA_REF_INF */
@@ -168,6 +190,12 @@ typedef struct gga_opts {
     uint32_t pull_alpha;  /* gga_multi_bfs: a level runs bottom-up (pull over in-edges) when the
                              frontier's out-edges x pull_alpha > m; 0 -> 14; 0xFFFFFFFF -> never */
     uint32_t ms_layout;   /* gga_multi_sp: GGA_MS_* engine layout (0 -> GGA_MS_AUTO) */
+    /* WORKLIST instrumentation (both off when NULL) */
+    gga_wg_record *timeline;    /* host buffer: grid records per step, steps in order */
+    uint32_t timeline_cap;      /* records it holds; steps beyond it run uninstrumented */
+    uint32_t conv_cap;          /* rows conv_rows holds */
+    const uint32_t *conv_final; /* host: the final distances (n), e.g. from a previous run */
+    gga_conv_row *conv_rows;    /* host: one row per relax step */
 } gga_opts;
 /* Fills defaults (all zero = defaults). */
 void gga_opts_init(gga_opts *o);
@@ -186,6 +214,9 @@ typedef struct gga_stats {
     uint64_t m_reach;         /* sum of out-degrees of reached vertices */
     uint64_t launches;        /* relax-kernel launches */
     uint32_t pull_levels;     /* gga_multi_bfs: levels run bottom-up */
+    uint32_t timeline_records;/* records written to opts.timeline */
+    uint32_t timeline_grid;   /* workgroups per step in it */
+    uint32_t conv_steps;      /* rows written to opts.conv_rows */
 } gga_stats;
 
 /* BFS levels from src (replaces the na_bfs path: dist init bfs_sssp.cu:302-321,
@@ -214,6 +245,17 @@ int gga_verify(uint32_t n, const uint32_t *row_ptr, const uint32_t *dst, const u
 /* Same check on the device against the graph's last result (for sizes where
  * a host copy is wasteful). */
 int gga_graph_verify(gga_graph *g, uint32_t src, int bfs, uint32_t inf, gga_verify_result *r);
+
+/* List-scheduling estimate of the relax time from a timeline (combineClocks,
+ * metrics.h:70-92): each step's workgroup durations, in launch order, go to
+ * the least-loaded of `slots` execution slots (CUs x resident workgroups);
+ * returns the sum over steps of the busiest slot, in ms (100 MHz ticks). */
+double gga_timeline_makespan(const gga_wg_record *r, uint32_t nrec, uint32_t grid, uint32_t slots);
+/* Appends the convergence block of printConv_rtime_init / _rtime / _final
+ * (convergence.h:81-140) to path: header, "Iteration:<step> = <norm2>(<diff>)"
+ * per row, "Total Iteration : <n>     Max Dist : <max_dist>". */
+int gga_write_conv(const char *path, int32_t level, uint32_t maxdeg, const char *graph, const char *algo,
+                   const gga_conv_row *rows, uint32_t nrows, uint32_t max_dist);
 
 /* gga_checksum of the last result, computed on the device. */
 int gga_graph_checksum(gga_graph *g, uint64_t *sum);

@@ -190,3 +190,29 @@ def test_device_verify_and_errors(gpu):
             dg.sssp(g.n)
         with pytest.raises(_lib.GgaError, match="GGA_ERANGE"):
             dg.split(level=-1, buckets=1 << 20)  # more buckets than the kernel holds
+
+
+def test_instrumented_worklist_timeline_and_convergence(gpu):
+    """The instrumented relax gives the same distances; it records one entry
+    per workgroup per step with plausible placement, and the convergence rows
+    end at zero difference with a non-increasing difference count (distances
+    only decrease toward the final ones)."""
+    g = gga.gen_rmat(14, 16, seed=3, wlo=1, whi=100)
+    src = g.maxdeg()[1]
+    want = oracle.dijkstra(g.row_ptr, g.dst, g.w, src)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=32)
+        announce("instrumented sssp rmat14")
+        d, st, tl, conv = dg.trace(src, timeline_cap=1 << 20, conv_final=want)
+        np.testing.assert_array_equal(d, want)
+        grid = st["timeline_grid"]
+        assert grid > 0 and tl.size == st["timeline_records"] and tl.size % grid == 0
+        assert tl.size // grid >= st["iterations"]
+        assert (tl["ticks"][: grid * st["iterations"]] > 0).all()
+        assert ((tl["place"] >> 8) < 8).all()                  # XCC ids of MI355X
+        assert len(set((tl["place"] >> 8).tolist())) > 1        # workgroups spread over XCDs
+        assert conv.size >= st["iterations"] and conv["diff_count"][-1] == 0 and conv["norm2"][-1] == 0
+        assert (np.diff(conv["diff_count"].astype(np.int64)) <= 0).all()
+        assert conv["diff_count"][0] > 0
+        ms = gga.timeline_makespan(tl, grid, 256)
+        assert 0 < ms < 1e3
