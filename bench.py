@@ -117,7 +117,7 @@ def batched_leg(gga, np, quick=False):
     ssrc = rng.choice(cand, 32, replace=False).astype(np.uint32)
     out = {"workload": "R-MAT s20 ef16 seed 13, 256 BFS + 32 SSSP sources (config 4)", "n": g.n, "m": g.m}
     with gga.DeviceGraph(g) as dg:
-        tr = dg.tune(int(ssrc[0]), bfs=False, layout=not quick)
+        tr = dg.tune(int(ssrc[0]), bfs=False, layout=not quick, multi=True)
         out["tuned"] = {"split_threshold": tr["mda"], "seed_threshold": tr["seed_mda"], "grid": tr["grid"],
                         "dense_min": tr["dense_min"], "layout": {1: "vertex", 2: "lane"}[tr["layout"]]}
         reps = 3 if quick else 5

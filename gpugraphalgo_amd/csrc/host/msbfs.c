@@ -4,7 +4,7 @@
  * bfs_sssp.cu:203); each lane equals gga_bfs from its source.
  *
  * The graph keeps, once built: the transposed CSR (in-edges) with its own
- * chunk table (same threshold as the forward split), the per-vertex bitmaps
+ * chunk table (threshold min(forward threshold, 32)), the per-vertex bitmaps
  * and the level buffer of the largest k seen.
  */
 #define _GNU_SOURCE
@@ -93,7 +93,7 @@ void gga_msbfs_release(void *state)
     free(s);
 }
 
-/* Transposed CSR + chunk table with threshold T (one-time). */
+/* Transposed CSR + chunk table with threshold min(T, 32) (one-time). */
 static int build_transpose(msbfs_state *s, const gga_graph_view_t *v)
 {
     int rc = GGA_OK;
@@ -102,7 +102,9 @@ static int build_transpose(msbfs_state *s, const gga_graph_view_t *v)
     const uint32_t n = g->n;
     uint32_t tmp_words = ggak_split_tmp_words(n + 1);
     if (tmp_words < GGA_MAXCHUNK_SLOTS) tmp_words = GGA_MAXCHUNK_SLOTS;
-    s->tT = g->mda ? g->mda : 1u;
+    /* a pull lane walks its in-chunk serially: short chunks keep the lanes of
+     * a wave balanced whatever the forward threshold */
+    s->tT = g->mda && g->mda < 32u ? g->mda : 32u;
     if ((rc = xalloc((void **)&indeg, 4ull * n))) goto fail;
     if ((rc = xalloc((void **)&pos, 4ull * (n + 1ull)))) goto fail;
     if ((rc = xalloc((void **)&tmp, 4ull * tmp_words))) goto fail;

@@ -30,13 +30,26 @@ uint32_t gga_tune_layout(gga_graph *g)
     return GGA_MS_VERTEX_MAJOR;
 }
 
-/* Best of two runs of one query (device time, ms). */
-static int time_query(gga_graph *g, uint32_t src, int bfs, const gga_opts *o, float *ms)
+/* 32 sources spread over the vertex range, starting at src. */
+static void spread_sources(uint32_t n, uint32_t src, uint32_t *srcs)
+{
+    for (uint32_t i = 0; i < GGA_MS_LANES; i++) srcs[i] = (uint32_t)((src + (uint64_t)i * (n / GGA_MS_LANES + 1)) % n);
+}
+
+/* Best of two runs (device time, ms) of the tuning objective: one query from
+ * src, or with GGA_TUNE_MULTI one 32-lane multi-source run. */
+static int time_query(gga_graph *g, uint32_t src, int bfs, uint32_t flags, const gga_opts *o, float *ms)
 {
     float best = 0.f;
+    uint32_t srcs[GGA_MS_LANES];
+    if (flags & GGA_TUNE_MULTI) spread_sources(gga_graph_n(g), src, srcs);
     for (int rep = 0; rep < 2; rep++) {
         gga_stats st;
-        const int rc = bfs ? gga_bfs(g, src, NULL, o, &st) : gga_sssp(g, src, NULL, o, &st);
+        int rc;
+        if (flags & GGA_TUNE_MULTI)
+            rc = gga_multi_sp(g, srcs, GGA_MS_LANES, bfs ? 0xFFFFFFFFu : 0u, NULL, o, &st);
+        else
+            rc = bfs ? gga_bfs(g, src, NULL, o, &st) : gga_sssp(g, src, NULL, o, &st);
         if (rc) return rc;
         if (rep == 0 || st.time_ms < best) best = st.time_ms;
     }
@@ -74,7 +87,7 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
     if (rc == GGA_EINVAL) rc = gga_graph_split(g, maxdeg, 0, 0, &st);
     else if (!rc) r->seed_mda = st.mda;
     if (rc) return rc;
-    if ((rc = time_query(g, src, bfs, &o, &r->seed_ms))) return rc;
+    if ((rc = time_query(g, src, bfs, flags, &o, &r->seed_ms))) return rc;
     record(r, st.mda, r->seed_ms);
     const uint32_t seed_t = st.mda;
     uint32_t best_t = seed_t;
@@ -87,7 +100,7 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
         if (seen) continue;
         float ms;
         if ((rc = gga_graph_split(g, t, 0, 0, &st))) return rc;
-        if ((rc = time_query(g, src, bfs, &o, &ms))) return rc;
+        if ((rc = time_query(g, src, bfs, flags, &o, &ms))) return rc;
         record(r, t, ms);
         if (ms < best) {
             best = ms;
@@ -97,27 +110,33 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
     if ((rc = gga_graph_split(g, best_t, 0, 0, &st))) return rc;
     r->mda = best_t;
 
-    /* 2. grid (workgroups) and 3. the push / compaction switch */
+    /* 2. grid (workgroups) and 3. the push / compaction switch (single-source
+     * objective; the multi-source engine sizes its own grid) */
+    const uint32_t n = gga_graph_n(g);
+    if (flags & GGA_TUNE_MULTI) {
+        r->best_ms = best;
+        r->layout = gga_tune_layout(g);
+        goto layout;
+    }
     const uint32_t g0 = gga_graph_default_grid(g, bfs);
     r->grid = g0;
     for (uint32_t div = 2; div <= 4 && g0 / div >= 64u; div *= 2) {
         float ms;
         o.grid = g0 / div;
-        if ((rc = time_query(g, src, bfs, &o, &ms))) return rc;
+        if ((rc = time_query(g, src, bfs, flags, &o, &ms))) return rc;
         if (ms < best) {
             best = ms;
             r->grid = o.grid;
         }
     }
     o.grid = r->grid;
-    const uint32_t n = gga_graph_n(g);
     const uint32_t d0 = n / 64u > 4096u ? n / 64u : 4096u;
     r->dense_min = d0;
     const uint32_t dcand[] = {d0 / 4u, d0 * 4u};
     for (size_t i = 0; i < 2; i++) {
         float ms;
         o.dense_min = dcand[i] ? dcand[i] : 1u;
-        if ((rc = time_query(g, src, bfs, &o, &ms))) return rc;
+        if ((rc = time_query(g, src, bfs, flags, &o, &ms))) return rc;
         if (ms < best) {
             best = ms;
             r->dense_min = o.dense_min;
@@ -125,11 +144,12 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
     }
     r->best_ms = best;
 
-    /* 4. multi-source layout: 32 lanes spread over the vertex range */
     r->layout = gga_tune_layout(g);
+layout:
+    /* 4. multi-source layout: 32 lanes spread over the vertex range */
     if (flags & GGA_TUNE_LAYOUT) {
         uint32_t srcs[GGA_MS_LANES];
-        for (uint32_t i = 0; i < GGA_MS_LANES; i++) srcs[i] = (uint32_t)((src + (uint64_t)i * (n / GGA_MS_LANES + 1)) % n);
+        spread_sources(n, src, srcs);
         float lt[2] = {0.f, 0.f};
         for (int l = 0; l < 2; l++) {
             gga_opts mo;

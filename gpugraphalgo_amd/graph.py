@@ -287,11 +287,12 @@ class DeviceGraph:
         return out, _stats_dict(st)
 
 
-    def tune(self, src: int, bfs: bool = False, layout: bool = False) -> dict:
+    def tune(self, src: int, bfs: bool = False, layout: bool = False, multi: bool = False) -> dict:
         """gga_tune: times split thresholds (seeded by the histogram heuristic),
-        grid and push/compaction switch on queries from src; keeps the best."""
+        grid and push/compaction switch on queries from src (multi: 32-lane
+        multi-source runs instead); keeps the best."""
         r = _lib.TuneResult()
-        flags = (1 if bfs else 0) | (2 if layout else 0)
+        flags = (1 if bfs else 0) | (2 if layout else 0) | (4 if multi else 0)
         check(load().gga_tune(self._h, src, flags, ctypes.byref(r)), "gga_tune")
         out = {f: getattr(r, f) for f, _ in r._fields_ if f not in ("tried_mda", "tried_ms", "layout_ms")}
         k = min(r.ntried, 16)

@@ -369,6 +369,8 @@ int gga_multi_bfs(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *out,
  * engine layouts on 32 spread sources.  Results never depend on any choice. */
 #define GGA_TUNE_BFS     1u   /* tune with BFS queries (default SSSP) */
 #define GGA_TUNE_LAYOUT  2u   /* also measure the multi-source layout */
+#define GGA_TUNE_MULTI   4u   /* objective: a 32-lane gga_multi_sp run from 32 spread sources
+                                 (split threshold only; grid and switch are left default) */
 #define GGA_TUNE_MAX     16
 typedef struct gga_tune_result {
     uint32_t mda;             /* chosen split threshold (the graph is left split with it) */
