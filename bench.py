@@ -158,6 +158,36 @@ def batched_leg(gga, np, quick=False):
     return out
 
 
+def grid_leg(gga, np, quick=False):
+    """Config 3: 4096 x 4096 grid, 4-neighbour undirected, + 0.5% random
+    shortcut edges (seed 11), weights [1, 1000], source 0 (corner); BFS and
+    SSSP in each launch mode.  Checked on the device (fixpoint + tightness +
+    dist[src] = 0, which fixes the shortest-path distances)."""
+    side = 1024 if quick else 4096
+    g = gga.gen_grid(side, side, 0.005, seed=11, wlo=1, whi=1000)
+    out = {"workload": f"{side}x{side} grid + 0.5% shortcuts, weights [1,1000], source 0 (config 3)",
+           "n": g.n, "m": g.m}
+    with gga.DeviceGraph(g) as dg:
+        dg.split()
+        for algo in ("bfs", "sssp"):
+            run = dg.bfs if algo == "bfs" else dg.sssp
+            res = {}
+            for mode in ("auto", "worklist", "persistent"):
+                run(0, mode=mode, keep_on_device=True)
+                reps = 3
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    _, st = run(0, mode=mode, keep_on_device=True)
+                wall = (time.perf_counter() - t0) / reps
+                ver = dg.verify(0, bfs=(algo == "bfs"))
+                assert ver["nerr"] == 0 and ver["untight"] == 0 and ver["src_ok"], (algo, mode, ver)
+                res[mode] = {"ms": round(wall * 1e3, 3), "gteps": round(st["m_reach"] / wall / 1e9, 3),
+                             "steps": st["iterations"],
+                             "mode_used": {0: "topo", 1: "worklist", 2: "persistent"}.get(st["mode"], st["mode"])}
+            out[algo] = res
+    return out
+
+
 def serving_leg(gga, np, full=False, quick=False):
     """Config 5 serving: resident R-MAT graphs (s16.. ef16, weights [1,100])
     and 4-neighbour grids (weights [1,1000]), seed 17; BFS/SSSP 50/50, uniform
@@ -229,6 +259,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="profiling run: one T, no CPU leg")
+    ap.add_argument("--no-grid", action="store_true", help="skip the grid leg (config 3)")
     ap.add_argument("--no-batched", action="store_true", help="skip the multi-source leg (config 4)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg (config 5)")
     ap.add_argument("--serve-full", action="store_true",
@@ -283,7 +314,11 @@ def main() -> int:
         assert np.array_equal(d_cpu, d_gpu), "GPU distances differ from the CPU reference"
         log(f"cpu: {cpu['value']:.4f} GTEPS; GPU result bit-exact")
 
-    batched = serving = None
+    batched = serving = grid = None
+    if not (args.no_grid or args.quick):
+        log("grid leg (config 3)")
+        grid = grid_leg(gga, np)
+        log(f"grid: {json.dumps(grid)}")
     if not (args.no_batched or args.quick):
         log("batched multi-source leg (config 4)")
         batched = batched_leg(gga, np)
@@ -336,6 +371,7 @@ def main() -> int:
                      "relaxations": r["relaxations"]},
         },
         "cpu_baseline": cpu,
+        "grid": grid,
         "batched": batched,
         "serving": serving,
     }

@@ -55,7 +55,7 @@ class ConvRow(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("mode", i32), ("inf", u32), ("max_iters", u32), ("block", u32),
                 ("sync_every", u32), ("use_graph", u32), ("count", u32), ("grid", u32),
-                ("dense_min", u32), ("pull_alpha", u32), ("ms_layout", u32),
+                ("dense_min", u32), ("pull_alpha", u32), ("ms_layout", u32), ("delta", u32),
                 ("timeline", ctypes.c_void_p), ("timeline_cap", u32), ("conv_cap", u32),
                 ("conv_final", ctypes.c_void_p), ("conv_rows", ctypes.c_void_p)]
 
@@ -93,7 +93,8 @@ class ServerCounters(ctypes.Structure):
 
 
 class TuneResult(ctypes.Structure):
-    _fields_ = [("mda", u32), ("grid", u32), ("dense_min", u32), ("layout", u32), ("seed_mda", u32),
+    _fields_ = [("mda", u32), ("grid", u32), ("dense_min", u32), ("layout", u32), ("mode", i32), ("delta", u32),
+                ("seed_mda", u32),
                 ("ntried", u32), ("seed_ms", ctypes.c_float), ("best_ms", ctypes.c_float),
                 ("layout_ms", ctypes.c_float * 2), ("tried_mda", u32 * 16), ("tried_ms", ctypes.c_float * 16)]
 

@@ -42,10 +42,12 @@ typedef struct gga_dev_work {
     uint32_t *ctl;
     uint64_t *cnt;       /* GGA_CNT_SLOTS x 4 work counters, striped over many addresses */
     uint32_t *tiles;     /* list compaction: per-tile counts, then offsets (GGA_TILES(n) + 1) */
+    uint32_t *last;      /* near-far: distance of each vertex at its last expansion (NULL: off) */
+    uint32_t *tmin;      /* near-far: per-tile minimum deferred distance (GGA_TILES(n)) */
     uint32_t inf;
     uint32_t qcap;       /* capacity of q0 / q1 in entries (>= C) */
     uint32_t dense_min;  /* a step whose list holds >= dense_min chunks is a flag step */
-    uint32_t pad;
+    uint32_t delta;      /* near-far bucket width (0: plain Bellman-Ford worklist) */
 } gga_dev_work;
 
 /* Per-workgroup timeline record of an instrumented relax step: start on the
@@ -147,6 +149,14 @@ typedef struct gga_dev_msbfs {
 #define GGA_BAR_GROUPS    32u                           /* grid barrier: first-level groups */
 #define GGA_CTL_BARG(i)   ((12u + (i)) * GGA_CTL_LINE)  /* grid barrier: arrivals within group i */
 #define GGA_CTL_WORDS     ((12u + GGA_BAR_GROUPS) * GGA_CTL_LINE)
+
+/* near-far: threshold of a ring slot's list (only vertices with dist < THR
+ * are listed); FAR = deferred vertices remain when a refill found none under
+ * the next threshold; RTHR = the threshold the current refill lists under */
+#define GGA_CTL_THR(s)    (6u * GGA_CTL_LINE + 4u + (s))
+#define GGA_CTL_FAR       (6u * GGA_CTL_LINE + 8u)
+#define GGA_CTL_RTHR      (6u * GGA_CTL_LINE + 9u)
+#define GGA_CTL_RSTEP     (6u * GGA_CTL_LINE + 10u)   /* step number (ITER) of the last refill */
 
 /* multi-source engine (kernels/ms.hip): lane mask of a ring slot's list */
 #define GGA_CTL_LANES(s)  (7u * GGA_CTL_LINE + 8u + (s))

@@ -49,6 +49,9 @@ int ggak_conv(const uint32_t *dist, const uint32_t *fin, uint32_t n, uint64_t *r
 /* After ggak_wl_step(phase): builds the next list from step stamps when that
  * step was a flag step (its list held >= w.dense_min chunks); else a no-op. */
 int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t phase, hipStream_t s);
+/* Near-far (w.last != NULL): after ggak_compact(phase), when the next list is
+ * empty, lists the deferred vertices under the next threshold. */
+int ggak_refill(gga_dev_graph g, gga_dev_work w, uint32_t phase, hipStream_t s);
 /* Occupancy-bound grid (workgroups) for the worklist / persistent kernels. */
 uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count);
 /* One cooperative launch running worklist steps until the list is empty or

@@ -25,6 +25,9 @@ gga_ms **gga_graph_ms_slot(gga_graph *g);
 /* Tuned defaults (tune.c) that traversals use when their opts leave them 0. */
 void gga_graph_set_tuned(gga_graph *g, uint32_t grid, uint32_t dense_min, uint32_t layout);
 uint32_t gga_graph_tuned_layout(const gga_graph *g);
+/* mode < 0: no tuned mode; delta 0xFFFFFFFF: near-far tuned off, 0: untuned. */
+void gga_graph_set_tuned_sssp(gga_graph *g, int32_t mode, uint32_t delta);
+uint32_t gga_graph_wmean(const gga_graph *g);
 uint32_t gga_graph_maxdeg(const gga_graph *g);
 uint32_t gga_graph_default_grid(gga_graph *g, int bfs);
 /* State of gga_multi_bfs (msbfs.c), owned by the graph. */

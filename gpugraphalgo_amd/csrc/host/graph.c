@@ -42,6 +42,11 @@ struct gga_graph {
     uint32_t split_gen;       /* bumped by every split; engines built on an older table are stale */
     gga_ms *ms_cache;         /* engine of gga_multi_sp */
     uint32_t tuned_grid, tuned_dense, tuned_layout;   /* gga_tune results (0: defaults) */
+    uint32_t tuned_delta;
+    int32_t tuned_mode;          /* 0: none; else GGA_MODE_* + 1 */
+    uint32_t last_steps[2];      /* [bfs] relax steps of the last query (AUTO's evidence) */
+    uint32_t wmean;              /* mean edge weight, rounded up (>= 1) */
+    uint32_t *d_last, *d_tmin;   /* near-far state (allocated on first use) */
     void *msbfs;              /* transposed graph + state of gga_multi_bfs (msbfs.c) */
     hipEvent_t ev[4];
     hipEvent_t *kev;          /* per-launch timing events */
@@ -133,7 +138,9 @@ static gga_dev_work dev_work(const gga_graph *g, uint32_t inf)
     w.inf = inf;
     w.qcap = g->qcap;
     w.dense_min = 0xFFFFFFFFu;
-    w.pad = 0;
+    w.last = NULL;
+    w.tmin = NULL;
+    w.delta = 0;
     return w;
 }
 
@@ -173,6 +180,8 @@ int gga_graph_free(gga_graph *g)
     dfree(g, (void **)&g->d_ctl, 4ull * GGA_CTL_WORDS);
     dfree(g, (void **)&g->d_cnt, 32ull * GGA_CNT_SLOTS);
     dfree(g, (void **)&g->d_tiles, 4ull * (GGA_TILES((uint64_t)g->n) + 1));
+    dfree(g, (void **)&g->d_last, 4ull * g->n);
+    dfree(g, (void **)&g->d_tmin, 4ull * GGA_TILES((uint64_t)g->n));
     if (g->h_pin) hipHostFree(g->h_pin);
     for (int i = 0; i < 4; i++)
         if (g->ev[i]) hipEventDestroy(g->ev[i]);
@@ -223,6 +232,15 @@ static int graph_create(uint32_t n, uint32_t m, const uint32_t *row_ptr, const u
     if (m) HIP_TRY(hipMemcpyAsync(g->d_col, dst, 4ull * m, hipMemcpyHostToDevice, g->stream));
     if (m && w) HIP_TRY(hipMemcpyAsync(g->d_w, w, 4ull * m, hipMemcpyHostToDevice, g->stream));
     else if (m) K_TRY(ggak_fill_u32(g->d_w, 1u, m, g->stream));
+    {
+        uint64_t ws = 0;
+        if (w)
+            for (uint32_t e = 0; e < m; e++) ws += w[e];
+        else
+            ws = m;
+        g->wmean = m ? (uint32_t)((ws + m - 1) / m) : 1u;
+        if (g->wmean == 0) g->wmean = 1;
+    }
     K_TRY(ggak_fill_u32(g->d_ctl, 0u, GGA_CTL_WORDS, g->stream));
     K_TRY(ggak_fill_u32(g->d_mark, 0xFFFFFFFFu, n, g->stream));
     /* max degree + first index on the device (graph.h:338-358) */
@@ -461,6 +479,14 @@ void gga_graph_set_tuned(gga_graph *g, uint32_t grid, uint32_t dense_min, uint32
     g->tuned_layout = layout;
 }
 
+void gga_graph_set_tuned_sssp(gga_graph *g, int32_t mode, uint32_t delta)
+{
+    g->tuned_mode = mode < 0 ? 0 : mode + 1;
+    g->tuned_delta = delta;
+}
+
+uint32_t gga_graph_wmean(const gga_graph *g) { return g ? g->wmean : 1u; }
+
 uint32_t gga_graph_tuned_layout(const gga_graph *g) { return g->tuned_layout; }
 uint32_t gga_graph_maxdeg(const gga_graph *g) { return g ? g->maxdeg : 0; }
 void **gga_graph_msbfs_slot(gga_graph *g) { return &g->msbfs; }
@@ -498,10 +524,35 @@ static uint32_t wl_grid(gga_graph *g, int persistent, int bfs, int count)
 
 uint32_t gga_graph_default_grid(gga_graph *g, int bfs) { return wl_grid(g, 0, bfs ? 1 : 0, 0); }
 
-static int choose_mode(const gga_graph *g)
+/* Mesh-like: maximum degree within a few times the mean.  Such graphs have
+ * long shortest paths and thin frontiers, so steps are many and small. */
+static int mesh_like(const gga_graph *g)
 {
-    (void)g;
-    return GGA_MODE_WORKLIST;
+    const uint64_t avg = g->n ? ((uint64_t)g->m + g->n - 1) / g->n : 0;
+    return g->maxdeg <= 4 * avg + 8;
+}
+
+/* AUTO: the tuned mode; else the worklist launches (full occupancy for a
+ * few big steps), switching to the persistent kernel once a query on this
+ * graph needed more than 256 steps (one launch and a grid barrier per step
+ * then beat per-step launches and host checks: measured 2.3x on a pure
+ * 1024^2 grid at ~2 400 steps, 0.8x on a shortcut 4096^2 grid at ~160). */
+static int choose_mode(const gga_graph *g, int bfs)
+{
+    if (g->tuned_mode) return g->tuned_mode - 1;
+    return g->last_steps[bfs ? 1 : 0] > 256u ? GGA_MODE_PERSISTENT : GGA_MODE_WORKLIST;
+}
+
+/* Near-far width when opts leave it 0: the tuned one; else, for SSSP on
+ * mesh-like graphs, 8 mean edge weights (bucket widths from 2 to 32 mean
+ * weights measured within 10% of each other on 1024^2 and 2048^2 grids);
+ * else off. */
+static uint32_t auto_delta(const gga_graph *g, int bfs)
+{
+    if (g->tuned_delta) return g->tuned_delta == 0xFFFFFFFFu ? 0u : g->tuned_delta;
+    if (bfs || !mesh_like(g)) return 0u;
+    const uint64_t d = 8ull * g->wmean;
+    return d < 0xFFFFFFF0ull ? (uint32_t)d : 0xFFFFFFF0u;
 }
 
 static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gga_opts *user,
@@ -528,9 +579,16 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
         if (rc) return rc;
     }
     const uint32_t inf = o.inf ? o.inf : GGA_INF;
-    const uint32_t max_iters = o.max_iters ? o.max_iters : (g->n + 1u);
     int mode = o.mode;
-    if (mode == GGA_MODE_AUTO || mode < 0 || mode > GGA_MODE_AUTO) mode = choose_mode(g);
+    if (mode == GGA_MODE_AUTO || mode < 0 || mode > GGA_MODE_AUTO) mode = choose_mode(g, bfs);
+    /* Bellman-Ford needs at most n + 1 steps (graph.h:122); near-far adds
+     * bucket changes and one empty step per jump over empty buckets, so its
+     * default cap is only a guard */
+    const uint32_t delta = o.delta == 0xFFFFFFFFu ? 0u : (o.delta ? o.delta : auto_delta(g, bfs));
+    const int near_far = !bfs && (mode == GGA_MODE_WORKLIST || mode == GGA_MODE_PERSISTENT) && delta != 0u;
+    uint32_t max_iters = o.max_iters ? o.max_iters : (g->n + 1u);
+    if (!o.max_iters && near_far)
+        max_iters = (uint64_t)64u * (g->n + 1u) < 0xFFFFFFF0ull ? 64u * (g->n + 1u) : 0xFFFFFFF0u;
     const int count = o.count ? 1 : 0;
     HIP_TRY(hipSetDevice(g->device));
     if ((rc = ensure_queues(g, g->C))) return rc;
@@ -543,6 +601,14 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
     gga_dev_work dw = dev_work(g, inf);
     dw.dense_min = o.dense_min ? o.dense_min
                                : g->tuned_dense ? g->tuned_dense : (g->n / 64u > 4096u ? g->n / 64u : 4096u);
+    /* near-far buckets: SSSP only (BFS levels are already one bucket per level) */
+    if (near_far) {
+        if (!g->d_last && (rc = dmalloc(g, (void **)&g->d_last, 4ull * g->n))) return rc;
+        if (!g->d_tmin && (rc = dmalloc(g, (void **)&g->d_tmin, 4ull * GGA_TILES((uint64_t)g->n)))) return rc;
+        dw.last = g->d_last;
+        dw.tmin = g->d_tmin;
+        dw.delta = delta;
+    }
     s.mode = mode;
     float ktime = 0.f;
 
@@ -606,12 +672,14 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
                 K_TRY(ggak_wl_step(dg, dw, abs_step % 6u, bfs, count, grid, tl, g->stream));
                 HIP_TRY(hipEventRecord(g->kev[3 * k + 1], g->stream));
                 K_TRY(ggak_compact(dg, dw, abs_step % 6u, g->stream));
+                K_TRY(ggak_refill(dg, dw, abs_step % 6u, g->stream));
                 HIP_TRY(hipEventRecord(g->kev[3 * k + 2], g->stream));
                 if (abs_step < conv_n) K_TRY(ggak_conv(g->d_dist, d_fin, g->n, d_conv + 2ull * abs_step, g->stream));
             }
             step += per;
             HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_TAIL(step % 3u), 4, hipMemcpyDeviceToHost,
                                    g->stream));
+            HIP_TRY(hipMemcpyAsync(g->h_pin + 1, g->d_ctl + GGA_CTL_FAR, 4, hipMemcpyDeviceToHost, g->stream));
             HIP_TRY(hipStreamSynchronize(g->stream));
             for (uint32_t k = 0; k < per; k++) {
                 float t = 0.f, tc = 0.f;
@@ -621,7 +689,7 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
                 s.ctime_ms += tc;
             }
             s.launches += per;
-            if (g->h_pin[0] == 0) break;
+            if (g->h_pin[0] == 0 && (!dw.last || g->h_pin[1] == 0)) break;
             if (step >= max_iters) {
                 rc = GGA_ETIMEDOUT;
                 gga_set_error("worklist mode: %u steps without convergence", step);
@@ -708,6 +776,7 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
     s.sktime_ms = 0.f;
     g->epoch += s.iterations + 2u;
     g->have_result = 1;
+    g->last_steps[bfs ? 1 : 0] = s.iterations;
     if (out) HIP_TRY(hipMemcpy(out, g->d_dist, 4ull * g->n, hipMemcpyDeviceToHost));   /* bfs_sssp.cu:489 */
     dfree(g, (void **)&d_tl, tl_bytes);
     dfree(g, (void **)&d_fin, fin_bytes);

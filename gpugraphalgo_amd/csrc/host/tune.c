@@ -72,8 +72,11 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
     if (!g || !r) return GGA_EINVAL;
     if (src >= gga_graph_n(g)) return GGA_EINVAL;
     memset(r, 0, sizeof *r);
+    r->mode = GGA_MODE_WORKLIST;
+    r->delta = 0xFFFFFFFFu;
     const int bfs = (flags & GGA_TUNE_BFS) ? 1 : 0;
     gga_graph_set_tuned(g, 0, 0, 0);
+    gga_graph_set_tuned_sssp(g, -1, 0);
     gga_opts o;
     gga_opts_init(&o);
     o.mode = GGA_MODE_WORKLIST;
@@ -111,7 +114,9 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
     r->mda = best_t;
 
     /* 2. grid (workgroups) and 3. the push / compaction switch (single-source
-     * objective; the multi-source engine sizes its own grid) */
+     * objective, worklist launches of plain Bellman-Ford; the multi-source
+     * engine sizes its own grid) */
+    o.delta = 0xFFFFFFFFu;
     const uint32_t n = gga_graph_n(g);
     if (flags & GGA_TUNE_MULTI) {
         r->best_ms = best;
@@ -141,6 +146,39 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
             best = ms;
             r->dense_min = o.dense_min;
         }
+    }
+    /* 4. launch mode and near-far width: the worklist launches and the
+     * persistent kernel, plain Bellman-Ford and buckets of 2, 8, 32 mean
+     * weights (SSSP); the two modes (BFS) */
+    {
+        const uint32_t wm = gga_graph_wmean(g);
+        const struct { int32_t mode; uint32_t delta; } cand2[] = {
+            {GGA_MODE_WORKLIST, 0xFFFFFFFFu},  {GGA_MODE_PERSISTENT, 0xFFFFFFFFu},
+            {GGA_MODE_PERSISTENT, 2u * wm},    {GGA_MODE_PERSISTENT, 8u * wm},
+            {GGA_MODE_PERSISTENT, 32u * wm},   {GGA_MODE_WORKLIST, 8u * wm}};
+        const size_t nc = bfs ? 2 : sizeof cand2 / sizeof cand2[0];
+        r->mode = GGA_MODE_WORKLIST;
+        r->delta = 0xFFFFFFFFu;
+        float mbest = best;
+        for (size_t i = 0; i < nc; i++) {
+            gga_opts o2 = o;
+            float ms;
+            o2.mode = cand2[i].mode;
+            o2.delta = cand2[i].delta;
+            if (o2.mode == GGA_MODE_PERSISTENT) o2.grid = 0;   /* one workgroup per CU */
+            if (i == 0) {
+                mbest = best;   /* the configuration timed last above */
+                continue;
+            }
+            if ((rc = time_query(g, src, bfs, flags, &o2, &ms))) return rc;
+            if (ms < mbest) {
+                mbest = ms;
+                r->mode = o2.mode;
+                r->delta = o2.delta;
+            }
+        }
+        best = mbest;
+        gga_graph_set_tuned_sssp(g, r->mode, bfs ? 0u : r->delta);
     }
     r->best_ms = best;
 

@@ -63,11 +63,16 @@ struct Counters {
 };
 
 // Expand one batch of chunks (this lane's chunk c when valid).
+// thr (near-far): a destination whose new distance is >= thr is lowered but
+// not listed -- it is deferred until the threshold passes it, and w.last
+// (when set) records the distance each owner was expanded at, so that a
+// deferred vertex is recognised by dist < last.  thr = 0xFFFFFFFF: plain
+// worklist (every improvement is listed).
 template <bool BFS, bool COUNT, int OUT>
 __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_dev_work &w,
                                              uint32_t c, bool valid, uint32_t it, uint32_t *pbuf,
                                              uint32_t &pcount, uint32_t *qout, uint32_t *tail,
-                                             bool &improved, Counters &cnt) {
+                                             bool &improved, Counters &cnt, uint32_t thr = 0xFFFFFFFFu) {
     const uint32_t l = lane_id();
     uint32_t *__restrict__ dist = w.dist;
     const uint32_t inf = w.inf;
@@ -75,8 +80,10 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
     if (valid) {
         eb = stream_load(&g.cbeg[c]);
         const uint32_t ee = stream_load(&g.cbeg[c + 1]);
-        du = dist[stream_load(&g.cown[c])];
+        const uint32_t own = stream_load(&g.cown[c]);
+        du = dist[own];
         n = du < inf ? ee - eb : 0u;
+        if (w.last && du < inf) w.last[own] = du;
     }
     const uint32_t incl = wave_incl_scan(n);
     const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
@@ -114,7 +121,7 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
             if (act[u] && alt >= ds[u] && alt < inf && alt < cur[u]) {
                 if (OUT == OUT_FLAG) {
                     (void)atomic_min_u32(&dist[v[u]], alt);  // result unused: no-return atomic
-                    w.mark[v[u]] = it;
+                    if (alt < thr) w.mark[v[u]] = it;
                     imp = true;
                 } else {
                     const uint32_t old = atomic_min_u32(&dist[v[u]], alt);
@@ -123,7 +130,7 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
             }
             if (COUNT) cnt.relax += (uint32_t)__popcll(__ballot(imp));
             if (OUT == OUT_QUEUE) {
-                bool want = imp;
+                bool want = imp && alt < thr;
                 if (!BFS && want)
                     want = (w.mark[v[u]] != it) && (atomic_xchg_u32(&w.mark[v[u]], it) != it);
                 const uint64_t mask = __ballot(want);
@@ -176,6 +183,8 @@ __global__ __launch_bounds__(BLOCK) void k_query_init(gga_dev_graph g, gga_dev_w
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t nth = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = tid; v < g.n; v += nth) w.dist[v] = (v == src) ? 0u : w.inf;
+    if (w.last)
+        for (uint64_t v = tid; v < g.n; v += nth) w.last[v] = w.inf;
     for (uint64_t i = tid; i < 4ull * GGA_CNT_SLOTS; i += nth) w.cnt[i] = 0;
     if (blockIdx.x == 0) {
         const uint32_t cs = g.cstart[src], nch = g.cstart[src + 1] - cs;
@@ -184,6 +193,12 @@ __global__ __launch_bounds__(BLOCK) void k_query_init(gga_dev_graph g, gga_dev_w
             w.ctl[GGA_CTL_TAIL(threadIdx.x)] = threadIdx.x == 0 ? nch : 0u;
             w.ctl[GGA_CTL_BC(threadIdx.x)] = 0;
             w.ctl[GGA_CTL_ITER(threadIdx.x)] = epoch;
+            // near-far: the first bucket is [0, delta); plain worklist: no bound
+            w.ctl[GGA_CTL_THR(threadIdx.x)] = w.last ? (w.delta < w.inf ? w.delta : w.inf) : 0xFFFFFFFFu;
+        }
+        if (threadIdx.x == 0) {
+            w.ctl[GGA_CTL_FAR] = 0;
+            w.ctl[GGA_CTL_RSTEP] = epoch - 1u;   // no refill yet (step numbers start at epoch)
         }
         if (threadIdx.x < GGA_BAR_GROUPS) w.ctl[GGA_CTL_BARG(threadIdx.x)] = 0;
         if (threadIdx.x == 0) {
@@ -237,6 +252,7 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     const uint32_t nbatch = (nq + bsz - 1) / bsz;
     const bool flags = flag_step(w, nq);
     const uint32_t l = lane_id();
+    const uint32_t thr = w.ctl[GGA_CTL_THR(step % 3u)];
     uint32_t pcount = 0;
     bool dummy = false;
     for (uint32_t b = wid; b < nbatch; b += waves) {
@@ -244,9 +260,9 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
         const bool valid = l < bsz && idx < nq;
         const uint32_t c = valid ? stream_load(&qin[idx]) : 0u;
         if (flags)
-            expand_batch<BFS, COUNT, OUT_FLAG>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt);
+            expand_batch<BFS, COUNT, OUT_FLAG>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt, thr);
         else
-            expand_batch<BFS, COUNT, OUT_QUEUE>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt);
+            expand_batch<BFS, COUNT, OUT_QUEUE>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt, thr);
     }
     if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
 }
@@ -276,6 +292,7 @@ __device__ __forceinline__ void step_bookkeeping(uint32_t *ctl, uint32_t step, u
                                                  uint32_t nq) {
     const uint32_t sout = (step + 1u) % 3u, sclr = (step + 2u) % 3u;
     ctl[GGA_CTL_ITER(sout)] = it + 1u;
+    ctl[GGA_CTL_THR(sout)] = ctl[GGA_CTL_THR(step % 3u)];
     ctl[GGA_CTL_TAIL(sclr)] = 0u;
     if (nq) ctl[GGA_CTL_STEPS] += 1u;
 }
@@ -349,13 +366,127 @@ __global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_de
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
 }
 
+// ---- near-far refill: after a step whose next list came out empty, list
+// the deferred vertices (dist < last) under the next threshold.
+struct DeferredUnder {
+    const uint32_t *dist, *last;
+    uint32_t t;
+    __device__ __forceinline__ bool operator()(uint32_t v) const {
+        const uint32_t d = dist[v];
+        return d < last[v] && d < t;
+    }
+};
+
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
+    const uint32_t s = a + b;
+    return s < a ? 0xFFFFFFFFu : s;
+}
+
+__device__ __forceinline__ bool refill_needed(const gga_dev_work &w, uint32_t step) {
+    return w.last && w.ctl[GGA_CTL_TAIL((step + 1u) % 3u)] == 0u;
+}
+
+// count: chunk ids per tile under thr + delta, and the tile's smallest deferred distance
+__device__ __forceinline__ void refill_count_body(const gga_dev_graph &g, const gga_dev_work &w, uint32_t step,
+                                                  uint32_t *s_wave, uint32_t *s_min) {
+    const ListOut o = list_out(g, w, step);
+    const DeferredUnder pred{w.dist, w.last, sat_add(w.ctl[GGA_CTL_THR((step + 1u) % 3u)], w.delta)};
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) {
+        compact_count_tile(o, g.n, t, pred, s_wave);
+        uint32_t m = 0xFFFFFFFFu;
+        const uint64_t v0 = (uint64_t)t * GGA_TILE;
+        for (uint32_t i = 0; i < CITEMS; i++) {
+            const uint64_t v = v0 + (uint64_t)i * BLOCK + threadIdx.x;
+            if (v < g.n) {
+                const uint32_t d = w.dist[v];
+                if (d < w.last[v] && d < m) m = d;
+            }
+        }
+#pragma unroll
+        for (uint32_t off = 32; off >= 1; off >>= 1) {
+            const uint32_t y = __shfl_xor(m, (int)off, GGA_WAVE);
+            m = y < m ? y : m;
+        }
+        if (lane_id() == 0) s_min[threadIdx.x / GGA_WAVE] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t x = s_min[0];
+            for (uint32_t i = 1; i < WAVES; i++) x = s_min[i] < x ? s_min[i] : x;
+            w.tmin[t] = x;
+        }
+        __syncthreads();
+    }
+}
+
+// scan (one workgroup): offsets and list length; the threshold of the new
+// list; when nothing lies under it but deferred vertices remain, the
+// threshold jumps to the smallest deferred distance and FAR asks for another
+// refill.
+__device__ __forceinline__ void refill_scan_body(const gga_dev_graph &g, const gga_dev_work &w, uint32_t step,
+                                                 uint32_t *s_wave, uint32_t *s_min) {
+    const uint32_t sout = (step + 1u) % 3u;
+    const uint32_t t1 = sat_add(w.ctl[GGA_CTL_THR(sout)], w.delta);
+    const ListOut o = list_out(g, w, step);
+    compact_scan(o, g.n, s_wave);
+    uint32_t m = 0xFFFFFFFFu;
+    for (uint32_t i = threadIdx.x; i < GGA_TILES(g.n); i += blockDim.x) m = w.tmin[i] < m ? w.tmin[i] : m;
+#pragma unroll
+    for (uint32_t off = 32; off >= 1; off >>= 1) {
+        const uint32_t y = __shfl_xor(m, (int)off, GGA_WAVE);
+        m = y < m ? y : m;
+    }
+    if (lane_id() == 0) s_min[threadIdx.x / GGA_WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t gmin = s_min[0];
+        for (uint32_t i = 1; i < WAVES; i++) gmin = s_min[i] < gmin ? s_min[i] : gmin;
+        const uint32_t total = *o.tail;
+        w.ctl[GGA_CTL_RTHR] = t1;
+        if (total) {
+            w.ctl[GGA_CTL_THR(sout)] = t1;
+            w.ctl[GGA_CTL_FAR] = 0u;
+            w.ctl[GGA_CTL_RSTEP] = w.ctl[GGA_CTL_ITER(step % 3u)];
+        } else if (gmin != 0xFFFFFFFFu) {
+            w.ctl[GGA_CTL_THR(sout)] = gmin;   // the next refill lists [gmin, gmin + delta)
+            w.ctl[GGA_CTL_FAR] = 1u;
+        } else {
+            w.ctl[GGA_CTL_FAR] = 0u;           // nothing deferred: converged
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void refill_write_body(const gga_dev_graph &g, const gga_dev_work &w, uint32_t step,
+                                                  uint32_t *s_wave) {
+    const ListOut o = list_out(g, w, step);
+    const DeferredUnder pred{w.dist, w.last, w.ctl[GGA_CTL_RTHR]};
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, pred, s_wave);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_refill_count(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+    __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
+    if (refill_needed(w, step)) refill_count_body(g, w, step, s_wave, s_min);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_refill_scan(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+    __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
+    if (refill_needed(w, step)) refill_scan_body(g, w, step, s_wave, s_min);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_refill_write(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+    __shared__ uint32_t s_wave[WAVES];
+    if (!w.last) return;
+    if (w.ctl[GGA_CTL_RSTEP] != w.ctl[GGA_CTL_ITER(step % 3u)]) return;  // no refill after this step
+    refill_write_body(g, w, step, s_wave);
+}
+
 // ------------------------------------------------------ persistent kernel
 template <bool BFS, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_work w,
                                                       uint32_t max_steps, uint64_t spin_ticks) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
     __shared__ uint32_t s_state[2];
-    __shared__ uint32_t s_wave[WAVES];
+    __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
     uint32_t gen = 0;
     if (threadIdx.x == 0)
         gen = __hip_atomic_load(&w.ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -385,6 +516,37 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
                 compact_write_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
         }
         grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
+        // near-far: an empty next list is refilled from the deferred vertices,
+        // as often as the threshold has to jump over empty buckets
+        while (w.last) {
+            if (threadIdx.x == 0) {
+                s_state[0] = __hip_atomic_load(&w.ctl[GGA_CTL_TAIL((step + 1u) % 3u)], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                s_state[1] = __hip_atomic_load(&w.ctl[GGA_CTL_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            const uint32_t tail = s_state[0], rerr = s_state[1];
+            __syncthreads();
+            if (tail || rerr) break;
+            refill_count_body(g, w, step, s_wave, s_min);
+            grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
+            if (blockIdx.x == 0) refill_scan_body(g, w, step, s_wave, s_min);
+            grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
+            if (threadIdx.x == 0) {
+                s_state[0] = __hip_atomic_load(&w.ctl[GGA_CTL_TAIL((step + 1u) % 3u)], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                s_state[1] = __hip_atomic_load(&w.ctl[GGA_CTL_FAR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            const uint32_t listed = s_state[0], far = s_state[1];
+            __syncthreads();
+            if (listed) {
+                refill_write_body(g, w, step, s_wave);
+                grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
+                break;
+            }
+            if (!far) break;   // converged: the next step finds an empty list
+        }
     }
     publish_counters<COUNT>(w, cnt);
 }
@@ -611,6 +773,16 @@ __global__ __launch_bounds__(BLOCK) void k_conv(const uint32_t *dist, const uint
 
 extern "C" int ggak_conv(const uint32_t *dist, const uint32_t *fin, uint32_t n, uint64_t *rows, hipStream_t s) {
     k_conv<<<grid_for(n, 1024), BLOCK, 0, s>>>(dist, fin, n, rows);
+    return launch_status();
+}
+
+extern "C" int ggak_refill(gga_dev_graph g, gga_dev_work w, uint32_t step, hipStream_t s) {
+    if (!w.last) return 0;
+    const uint32_t ntiles = GGA_TILES(g.n);
+    const uint32_t grid = ntiles < 2048u ? (ntiles ? ntiles : 1u) : 2048u;
+    k_refill_count<<<grid, BLOCK, 0, s>>>(g, w, step);
+    k_refill_scan<<<1, BLOCK, 0, s>>>(g, w, step);
+    k_refill_write<<<grid, BLOCK, 0, s>>>(g, w, step);
     return launch_status();
 }
 

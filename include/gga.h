@@ -190,6 +190,11 @@ typedef struct gga_opts {
     uint32_t pull_alpha;  /* gga_multi_bfs: a level runs bottom-up (pull over in-edges) when the
                              frontier's out-edges x pull_alpha > m; 0 -> 14; 0xFFFFFFFF -> never */
     uint32_t ms_layout;   /* gga_multi_sp: GGA_MS_* engine layout (0 -> GGA_MS_AUTO) */
+    uint32_t delta;       /* SSSP, WORKLIST / PERSISTENT: near-far bucket width -- an improved vertex is listed
+                             only while its distance is below the current threshold, the rest
+                             wait until the threshold passes them (Davidson et al. 2014), so
+                             fewer vertices are expanded more than once; 0 -> the tuned width
+                             (gga_tune) else off; 0xFFFFFFFF -> off */
     /* WORKLIST instrumentation (both off when NULL) */
     gga_wg_record *timeline;    /* host buffer: grid records per step, steps in order */
     uint32_t timeline_cap;      /* records it holds; steps beyond it run uninstrumented */
@@ -363,10 +368,12 @@ int gga_multi_bfs(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *out,
  * (10 buckets, bfs_sssp.cu:166) seeds the candidates {seed, 16, 32, 64, 128,
  * 256, 1024, maxdeg}; each is timed on one query from src (best of 2 device
  * times), the fastest split is kept, then the worklist grid (default, 1/2,
- * 1/4) and the push / compaction switch (dense_min x 1/4, x 4) are timed the
- * same way.  The graph keeps the choices: later queries whose opts leave grid
- * and dense_min at 0 use them.  GGA_TUNE_LAYOUT also times both multi-source
- * engine layouts on 32 spread sources.  Results never depend on any choice. */
+ * 1/4), the push / compaction switch (dense_min x 1/4, x 4) and the launch
+ * mode with the near-far width (worklist / persistent, off / 2 / 8 / 32 mean
+ * weights) are timed the same way.  The graph keeps the choices: later
+ * queries whose opts leave mode AUTO and grid, dense_min, delta at 0 use
+ * them.  GGA_TUNE_LAYOUT also times both multi-source engine layouts on 32
+ * spread sources.  Results never depend on any choice. */
 #define GGA_TUNE_BFS     1u   /* tune with BFS queries (default SSSP) */
 #define GGA_TUNE_LAYOUT  2u   /* also measure the multi-source layout */
 #define GGA_TUNE_MULTI   4u   /* objective: a 32-lane gga_multi_sp run from 32 spread sources
@@ -377,6 +384,8 @@ typedef struct gga_tune_result {
     uint32_t grid;            /* chosen worklist grid (workgroups of 256 threads) */
     uint32_t dense_min;       /* chosen push / compaction switch */
     uint32_t layout;          /* GGA_MS_VERTEX_MAJOR or GGA_MS_LANE_MAJOR */
+    int32_t  mode;            /* chosen GGA_MODE_WORKLIST / GGA_MODE_PERSISTENT (what AUTO then uses) */
+    uint32_t delta;           /* chosen near-far width (0xFFFFFFFF: plain Bellman-Ford) */
     uint32_t seed_mda;        /* the histogram heuristic's threshold */
     uint32_t ntried;          /* split thresholds timed */
     float    seed_ms, best_ms;

@@ -216,3 +216,69 @@ def test_instrumented_worklist_timeline_and_convergence(gpu):
         assert conv["diff_count"][0] > 0
         ms = gga.timeline_makespan(tl, grid, 256)
         assert 0 < ms < 1e3
+
+
+@pytest.mark.parametrize("delta", [1, 7, 50, 300, 100000])
+def test_near_far_sssp_equals_oracle(gpu, golden, delta):
+    """Near-far buckets of width delta: distances equal the oracle on the golden
+    vectors, R-MAT s16 (several thresholds and both list representations) and
+    a grid (whose shortest paths cross many buckets)."""
+    index, arrays, _ = golden
+    for e in (e for e in index if e["algo"] == "sssp"):
+        rp, dst, w = golden_graph(arrays, e["graph"])
+        g = gga.csr_from_arrays(rp, dst, w)
+        with gga.DeviceGraph(g) as dg:
+            for mda in (1, 3):
+                dg.split(mda=mda)
+                for rep in DENSE:
+                    d, _ = dg.sssp(e["src"], mode="worklist", inf=e["inf"], delta=delta, dense_min=DENSE[rep])
+                    np.testing.assert_array_equal(d, arrays[e["key"]], err_msg=f"{e['key']} mda={mda} {rep}")
+    g = gga.gen_rmat(16, 16, seed=7, wlo=1, whi=100)
+    src = g.maxdeg()[1]
+    want = oracle.dijkstra(g.row_ptr, g.dst, g.w, src)
+    with gga.DeviceGraph(g) as dg:
+        for T in (32, 1024):
+            dg.split(mda=T)
+            for rep in DENSE:
+                announce(f"near-far rmat16 delta={delta} T={T} lists={rep}")
+                d, st = dg.sssp(src, mode="worklist", count=True, delta=delta, dense_min=DENSE[rep])
+                np.testing.assert_array_equal(d, want)
+                assert st["m_reach"] == oracle.m_reach(g.row_ptr, want)
+    g = gga.gen_grid(120, 150, 0.0, seed=11, wlo=1, whi=1000)
+    want = oracle.dijkstra(g.row_ptr, g.dst, g.w, 0)
+    with gga.DeviceGraph(g) as dg:
+        dg.split()
+        announce(f"near-far grid120x150 delta={delta}")
+        np.testing.assert_array_equal(dg.sssp(0, mode="worklist", delta=delta)[0], want)
+        np.testing.assert_array_equal(dg.sssp(7, mode="worklist", delta=delta, sync_every=1)[0],
+                                      oracle.dijkstra(g.row_ptr, g.dst, g.w, 7))
+
+
+@pytest.mark.slow
+def test_full_size_config3_grid_sssp_bfs_equal_oracle(gpu):
+    """Config 3 at full size (4096^2 grid + 0.5% shortcuts): the default
+    (AUTO: persistent kernel, near-far SSSP) and worklist modes against the
+    oracle's Dijkstra / queue BFS."""
+    g = gga.gen_grid(4096, 4096, 0.005, seed=11, wlo=1, whi=1000)
+    want_s = oracle.dijkstra(g.row_ptr, g.dst, g.w, 0)
+    want_b = oracle.bfs_queue(g.row_ptr, g.dst, 0)
+    with gga.DeviceGraph(g) as dg:
+        dg.split()
+        for mode in ("auto", "worklist"):
+            announce(f"config3 full size sssp mode={mode}")
+            np.testing.assert_array_equal(dg.sssp(0, mode=mode)[0], want_s)
+            announce(f"config3 full size bfs mode={mode}")
+            np.testing.assert_array_equal(dg.bfs(0, mode=mode)[0], want_b)
+
+
+@pytest.mark.slow
+def test_full_size_config2_rmat22_sssp_equals_oracle(gpu):
+    g = gga.gen_rmat(22, 16, seed=7, wlo=1, whi=100)
+    src = g.maxdeg()[1]
+    want = oracle.dijkstra(g.row_ptr, g.dst, g.w, src)
+    with gga.DeviceGraph(g) as dg:
+        for T in (32, 64, 128, 256, 1024):
+            dg.split(mda=T)
+            announce(f"config2 full size sssp T={T}")
+            np.testing.assert_array_equal(dg.sssp(src)[0], want)
+        np.testing.assert_array_equal(dg.bfs(src)[0], oracle.bfs_queue(g.row_ptr, g.dst, src))
