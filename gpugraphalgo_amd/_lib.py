@@ -150,6 +150,8 @@ SIGNATURES = [
     ("gga_graph_name", None, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
     ("gga_ms_create", ctypes.c_int, [vp, u32, u32, P(vp)]),
     ("gga_ms_free", ctypes.c_int, [vp]),
+    ("gga_ms_set_delta", ctypes.c_int, [vp, u32]),
+    ("gga_ms_graph_bytes", u64, [vp]),
     ("gga_ms_start", ctypes.c_int, [vp, u32, vp, u32]),
     ("gga_ms_run", ctypes.c_int, [vp, u32, P(Opts), P(u32)]),
     ("gga_ms_done", u32, [vp]),

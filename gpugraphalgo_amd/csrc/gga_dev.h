@@ -76,6 +76,7 @@ typedef struct gga_wg_record {
  *               finish pass of step s clears the masks it consumed
  *   q0, q1      chunk lists (step s reads q[s & 1])
  *   tiles[GGA_TILES(n) + 1]  compaction counts / offsets */
+#define GGA_MS_RBLOCKS 512u   /* workgroups of the near-far refill (rows of lpart) */
 #ifndef GGA_MS_LANES
 #define GGA_MS_LANES 32   /* same value as include/gga.h */
 #endif
@@ -93,6 +94,11 @@ typedef struct gga_dev_ms {
     uint32_t lane_major; /* 1: D[lane * n + v] (each lane contiguous) instead of D[v * 32 + lane] */
     uint32_t dense_min;  /* a step whose list holds >= dense_min chunks rebuilds the next list
                             by compaction of the masks; smaller ones push */
+    /* near-far per lane (NULL / 0: plain Bellman-Ford lanes) */
+    uint32_t *L;         /* distance of (vertex, lane) at its last expansion, D's layout */
+    uint32_t *lpart;     /* refill: per-workgroup per-lane minimum deferred distance */
+    uint32_t delta;      /* bucket width */
+    uint32_t pad2;
 } gga_dev_ms;
 typedef struct gga_ms_sources { uint32_t v[GGA_MS_LANES]; } gga_ms_sources;
 typedef struct gga_ms_outputs { uint32_t *p[GGA_MS_LANES]; } gga_ms_outputs;
@@ -137,7 +143,11 @@ typedef struct gga_dev_msbfs {
 /* ctl word offsets (uint32 index); hot atomics sit on their own 128-byte line */
 #define GGA_CTL_LINE      32u
 #define GGA_CTL_TAIL(s)   ((s) * GGA_CTL_LINE)          /* s = 0..2: list length of a ring slot */
-#define GGA_CTL_BC(s)     ((3u + (s)) * GGA_CTL_LINE)   /* batch counter of a ring slot */
+/* multi-source near-far: line 3 = per-lane thresholds, line 4 = lane sets */
+#define GGA_CTL_LTHR(s)   (3u * GGA_CTL_LINE + (s))     /* lane s lists distances < LTHR(s) */
+#define GGA_CTL_RUN       (4u * GGA_CTL_LINE)           /* lanes started and not converged */
+#define GGA_CTL_RLISTED   (4u * GGA_CTL_LINE + 1u)      /* lanes a refill listed vertices for */
+#define GGA_CTL_RNEED     (4u * GGA_CTL_LINE + 2u)      /* a refill listed new vertices: rebuild */
 #define GGA_CTL_ITER(s)   (6u * GGA_CTL_LINE + (s))     /* step number of a ring slot */
 #define GGA_CTL_CHANGED   (7u * GGA_CTL_LINE)           /* topology-mode `changed` flag */
 #define GGA_CTL_STEPS     (7u * GGA_CTL_LINE + 1u)      /* steps that had a non-empty list */

@@ -28,6 +28,8 @@ uint32_t gga_graph_tuned_layout(const gga_graph *g);
 /* mode < 0: no tuned mode; delta 0xFFFFFFFF: near-far tuned off, 0: untuned. */
 void gga_graph_set_tuned_sssp(gga_graph *g, int32_t mode, uint32_t delta);
 uint32_t gga_graph_wmean(const gga_graph *g);
+/* Near-far width a query with opts.delta == 0 would use (0: off). */
+uint32_t gga_graph_auto_delta(const gga_graph *g, int bfs);
 uint32_t gga_graph_maxdeg(const gga_graph *g);
 uint32_t gga_graph_default_grid(gga_graph *g, int bfs);
 /* State of gga_multi_bfs (msbfs.c), owned by the graph. */

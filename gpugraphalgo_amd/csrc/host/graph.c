@@ -555,6 +555,8 @@ static uint32_t auto_delta(const gga_graph *g, int bfs)
     return d < 0xFFFFFFF0ull ? (uint32_t)d : 0xFFFFFFF0u;
 }
 
+uint32_t gga_graph_auto_delta(const gga_graph *g, int bfs) { return g ? auto_delta(g, bfs) : 0u; }
+
 static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gga_opts *user,
                     gga_stats *st)
 {

@@ -99,6 +99,8 @@ int gga_ms_free(gga_ms *e)
     for (int i = 0; i < 4; i++)
         if (e->ev[i]) hipEventDestroy(e->ev[i]);
     ms_release(e->d_sum);
+    ms_release(e->d.L);
+    ms_release(e->d.lpart);
     if (e->h_sum) hipHostFree(e->h_sum);
     free(e);
     return GGA_OK;
@@ -144,11 +146,56 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
     K_TRY(ggak_fill_u32(e->d.M1, 0u, e->n, e->v.stream));
     K_TRY(ggak_ms_clear(e->d, e->v.stream));
     HIP_TRY(hipStreamSynchronize(e->v.stream));
+    if ((rc = gga_ms_set_delta(e, gga_graph_auto_delta(g, 0)))) goto fail;
     *out = e;
     return GGA_OK;
 fail:
     gga_ms_free(e);
     return rc;
+}
+
+int gga_ms_set_delta(gga_ms *e, uint32_t delta)
+{
+    int rc = GGA_OK;
+    if (!e) return GGA_EINVAL;
+    if (e->running) {
+        gga_set_error("near-far width changes only while no lane runs");
+        return GGA_EINVAL;
+    }
+    if (delta == 0xFFFFFFFFu) delta = 0;
+    HIP_TRY(hipSetDevice(e->v.device));
+    if (delta && !e->d.L) {
+        if ((rc = ms_alloc(e, (void **)&e->d.L, 4ull * GGA_MS_LANES * e->n))) return rc;
+        if ((rc = ms_alloc(e, (void **)&e->d.lpart, 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS))) return rc;
+        K_TRY(ggak_fill_u32(e->d.L, e->d.inf, (uint64_t)GGA_MS_LANES * e->n, e->v.stream));
+        HIP_TRY(hipStreamSynchronize(e->v.stream));
+    } else if (!delta && e->d.L) {
+        HIP_TRY(hipStreamSynchronize(e->v.stream));
+        ms_release(e->d.L);
+        ms_release(e->d.lpart);
+        e->dev_bytes -= 4ull * GGA_MS_LANES * e->n + 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
+        e->d.L = NULL;
+        e->d.lpart = NULL;
+    }
+    e->d.delta = delta;
+    /* captured steps hold the old engine arguments */
+    for (int i = 0; i < 6; i++)
+        if (e->gx[i]) {
+            hipGraphExecDestroy(e->gx[i]);
+            e->gx[i] = NULL;
+        }
+    return GGA_OK;
+fail:
+    return rc;
+}
+
+uint64_t gga_ms_graph_bytes(gga_graph *g)
+{
+    gga_graph_view_t v;
+    if (!g || gga_graph_view(g, &v)) return 0;
+    uint64_t b = gga_ms_bytes(v.dg.n, v.dg.C);
+    if (gga_graph_auto_delta(g, 0)) b += 4ull * GGA_MS_LANES * v.dg.n + 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
+    return b;
 }
 
 static int ms_check(gga_ms *e)
@@ -253,11 +300,14 @@ fail:
     return rc;
 }
 
-/* List length and live lanes of the slot the next step reads. */
+/* List length and live lanes of the slot the next step reads.  Near-far
+ * lanes are live while the device keeps them running (they may wait for a
+ * refill with nothing listed). */
 static void ms_slot(const gga_ms *e, uint32_t *live, uint32_t *tail)
 {
     *tail = e->h_pin[GGA_CTL_TAIL(e->step % 3u)];
-    *live = *tail ? e->h_pin[GGA_CTL_LANES(e->step % 3u)] : 0u;
+    if (e->d.L) *live = e->h_pin[GGA_CTL_RUN];
+    else *live = *tail ? e->h_pin[GGA_CTL_LANES(e->step % 3u)] : 0u;
 }
 
 int gga_ms_launch(gga_ms *e, uint32_t nsteps, const gga_opts *o)
@@ -386,6 +436,10 @@ int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live_
     else gga_opts_init(&oo);
     const int count = oo.count ? 1 : 0;
     const int persistent = oo.mode == GGA_MODE_PERSISTENT;
+    if (persistent && e->d.L) {
+        gga_set_error("near-far engines run in step mode (gga_ms_set_delta(e, 0) for the cooperative launch)");
+        return GGA_EINVAL;
+    }
     const uint32_t cap = max_steps ? max_steps : e->n + 1u;
     e->d.dense_min = oo.dense_min ? oo.dense_min : e->dense_default;
     uint32_t live = 0, tail = 0, ran = 0;
@@ -418,10 +472,10 @@ int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live_
             if ((rc = gga_ms_collect(e, &live))) return rc;
             ran += k;
             tail = e->h_pin[GGA_CTL_TAIL(e->step % 3u)];
-            if (tail == 0 || ran >= cap) break;
+            if ((tail == 0 && live == 0) || ran >= cap) break;
         }
         if (live_out) *live_out = live;
-        if (!max_steps && tail) {
+        if (!max_steps && (tail || live)) {
             gga_set_error("multi-source: %u steps without convergence", ran);
             return GGA_ETIMEDOUT;
         }
@@ -521,6 +575,13 @@ int gga_multi_sp(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t unit_l
     if (!*slot && (rc = gga_ms_create(g, inf, want_lm ? GGA_MS_LANE_MAJOR : GGA_MS_VERTEX_MAJOR, slot)))
         return rc;
     gga_ms *e = *slot;
+    {
+        /* near-far width: opts, else the graph's; the cooperative launch runs
+         * plain Bellman-Ford lanes */
+        uint32_t want = oo.delta == 0xFFFFFFFFu ? 0u : (oo.delta ? oo.delta : gga_graph_auto_delta(g, 0));
+        if (oo.mode == GGA_MODE_PERSISTENT) want = 0;
+        if (want != e->d.delta && (rc = gga_ms_set_delta(e, want))) return rc;
+    }
     const uint32_t lanes = k == 32 ? 0xFFFFFFFFu : ((1u << k) - 1u);
     uint32_t src[GGA_MS_LANES];
     memset(src, 0, sizeof src);

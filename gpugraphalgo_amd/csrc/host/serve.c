@@ -123,7 +123,7 @@ int gga_server_free(gga_server *s)
     return GGA_OK;
 }
 
-static int add_graph(gga_server *s, gga_graph *g, uint32_t n, uint32_t C, uint32_t *id)
+static int add_graph(gga_server *s, gga_graph *g, uint32_t n, uint32_t C, uint64_t bytes, uint32_t *id)
 {
     uint64_t cap = s->cap_gr;
     int rc = grow((void **)&s->gr, &cap, s->ngr + 1ull, sizeof(srv_graph));
@@ -134,7 +134,7 @@ static int add_graph(gga_server *s, gga_graph *g, uint32_t n, uint32_t C, uint32
     x->g = g;
     x->n = n;
     x->C = C;
-    x->engine_bytes = gga_ms_bytes(n, C);
+    x->engine_bytes = bytes ? bytes : gga_ms_bytes(n, C);
     if (s->o.host_results) x->engine_bytes += 4ull * GGA_MS_LANES * n;   /* fetch staging */
     if (id) *id = s->ngr;
     s->ngr++;
@@ -147,13 +147,13 @@ int gga_server_add_graph(gga_server *s, gga_graph *g, uint32_t *id)
     gga_graph_view_t v;
     int rc = gga_graph_view(g, &v);   /* splits with the defaults if needed */
     if (rc) return rc;
-    return add_graph(s, g, v.dg.n, v.dg.C, id);
+    return add_graph(s, g, v.dg.n, v.dg.C, gga_ms_graph_bytes(g), id);
 }
 
 int gga_server_add_graph_dry(gga_server *s, uint32_t n, uint32_t chunks, uint32_t *id)
 {
     if (!s || !s->o.dry_run || n == 0) return GGA_EINVAL;
-    return add_graph(s, NULL, n, chunks, id);
+    return add_graph(s, NULL, n, chunks, 0, id);
 }
 
 int gga_server_submit(gga_server *s, const gga_query *q, uint64_t *id)

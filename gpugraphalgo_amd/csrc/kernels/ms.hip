@@ -61,6 +61,16 @@ __device__ __forceinline__ uint32_t *ms_mcur(const gga_dev_ms &ms, uint32_t step
 __device__ __forceinline__ uint32_t *ms_mnxt(const gga_dev_ms &ms, uint32_t step) {
     return (step & 1u) ? ms.M0 : ms.M1;
 }
+// index of (vertex v, lane s) in D / L for the engine's layout
+__device__ __forceinline__ uint64_t didx(const gga_dev_ms &ms, uint32_t n, uint32_t v, uint32_t s) {
+    return ms.lane_major ? (uint64_t)s * n + v : (uint64_t)v * LANES + s;
+}
+
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
+    const uint32_t r = a + b;
+    return r < a ? 0xFFFFFFFFu : r;
+}
+
 __device__ __forceinline__ uint32_t ms_len(const gga_dev_ms &ms, uint32_t step) {
     const uint32_t t = ms.ctl[GGA_CTL_TAIL(step % 3u)];
     return t < ms.qcap ? t : ms.qcap;
@@ -112,6 +122,9 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     const uint32_t nbatch = (nq + bsz - 1) / bsz;
     const uint32_t l = lane_id(), half = l >> 5, s = l & (LANES - 1u);
     const bool unit = (ms.ctl[GGA_CTL_UNIT] >> s) & 1u;
+    // near-far: an improvement at or beyond the lane's threshold lowers the
+    // distance but is deferred (no mask bit) until a refill reaches it
+    const uint32_t thr_s = ms.L ? ms.ctl[GGA_CTL_LTHR(s)] : 0xFFFFFFFFu;
     for (uint32_t b = wid; b < nbatch; b += waves) {
         const uint32_t idx = b * bsz + l;
         const bool valid = l < bsz && idx < nq;
@@ -163,14 +176,14 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                         atomic_min_u32(&D[LM ? (uint64_t)s * g.n + v[i] : (uint64_t)v[i] * LANES + s], a);
                     imp = a < old;
                 }
-                const uint64_t bal = __ballot(imp);
+                const uint64_t bal = __ballot(imp && a < thr_s);
                 const uint32_t hm = (uint32_t)(bal >> (32u * half));
                 bool first = false;
                 if (s == 0 && hm)
                     first = __hip_atomic_fetch_or(&mnxt[v[i]], hm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
                 if (COUNT) {
                     cnt.edges += (uint32_t)__popcll(__ballot(on));
-                    cnt.relax += (uint32_t)__popcll(bal);
+                    cnt.relax += (uint32_t)__popcll(__ballot(imp));
                 }
                 if (push) {
                     // the improver that found the next mask empty lists the vertex
@@ -209,7 +222,25 @@ __device__ __forceinline__ void ms_finish(const gga_dev_graph &g, const gga_dev_
     const uint32_t *mnxt = ms_mnxt(ms, step);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint64_t i = tid; i < nin; i += stride) mcur[g.cown[qin[i]]] = 0u;
+    // the thread holding an owner's first chunk retires the owner: records the
+    // distances its lanes were expanded at (near-far) and clears its mask.  A
+    // lane whose distance dropped during the step has its bit in the next mask
+    // (the old distance was under the threshold, so the new one is), so the
+    // current distance is the expanded one for every lane not re-listed.
+    for (uint64_t i = tid; i < nin; i += stride) {
+        const uint32_t c = qin[i], u = g.cown[c];
+        if (c != g.cstart[u]) continue;
+        if (ms.L) {
+            uint32_t m = mcur[u];
+            while (m) {
+                const uint32_t s = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                const uint64_t k = didx(ms, g.n, u, s);
+                ms.L[k] = ms.D[k];
+            }
+        }
+        mcur[u] = 0u;
+    }
     uint32_t x = 0;
     for (uint64_t i = tid; i < nout; i += stride) x |= mnxt[g.cown[qout[i]]];
     x = wave_or(x);
@@ -237,7 +268,10 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reset(gga_dev_graph g, gga_dev_ms 
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t s = (uint32_t)(i & (LANES - 1u)), v = (uint32_t)(i / LANES);
-        if ((lanes >> s) & 1u) ms.D[i] = v == src.v[s] ? 0u : ms.inf;
+        if ((lanes >> s) & 1u) {
+            ms.D[i] = v == src.v[s] ? 0u : ms.inf;
+            if (ms.L) ms.L[i] = ms.inf;
+        }
     }
 }
 
@@ -247,8 +281,11 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reset_lm(gga_dev_graph g, gga_dev_
     const uint32_t s = blockIdx.y;
     if (!((lanes >> s) & 1u)) return;
     uint32_t *row = ms.D + (uint64_t)s * g.n;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x)
+    uint32_t *lrow = ms.L ? ms.L + (uint64_t)s * g.n : nullptr;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x) {
         row[v] = v == src.v[s] ? 0u : ms.inf;
+        if (lrow) lrow[v] = ms.inf;
+    }
 }
 
 // Seeds the restarted lanes into the list of ring slot `step`: the source's
@@ -257,7 +294,12 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reset_lm(gga_dev_graph g, gga_dev_
 __global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, uint32_t lanes, uint32_t unit,
                                                 gga_ms_sources src, uint32_t step) {
     const uint32_t s = threadIdx.x;
-    if (s == 0) ms.ctl[GGA_CTL_UNIT] = (ms.ctl[GGA_CTL_UNIT] & ~lanes) | (unit & lanes);
+    if (s == 0) {
+        ms.ctl[GGA_CTL_UNIT] = (ms.ctl[GGA_CTL_UNIT] & ~lanes) | (unit & lanes);
+        ms.ctl[GGA_CTL_RUN] |= lanes;
+    }
+    if (s < LANES && ((lanes >> s) & 1u))   // BFS lanes are one bucket per level already
+        ms.ctl[GGA_CTL_LTHR(s)] = (ms.L && ms.delta && !((unit >> s) & 1u)) ? ms.delta : 0xFFFFFFFFu;
     if (s >= LANES || !((lanes >> s) & 1u)) return;
     const uint32_t x = src.v[s];
     uint32_t *mcur = ms_mcur(ms, step);
@@ -274,6 +316,8 @@ __global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, 
 template <bool COUNT, bool LM>
 __global__ __launch_bounds__(BLOCK) void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
+    // the previous step's refill rebuild has run (stream order): re-arm it
+    if (blockIdx.x == 0 && threadIdx.x == 0) ms.ctl[GGA_CTL_RNEED] = 0u;
     const uint32_t nq = ms_len(ms, step);
     if (nq == 0) return;
     Counters cnt;
@@ -310,6 +354,116 @@ __global__ __launch_bounds__(CBLOCK) void k_ms_write(gga_dev_graph g, gga_dev_ms
 __global__ __launch_bounds__(BLOCK) void k_ms_finish(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_or[WAVES];
     ms_finish(g, ms, step, s_or);
+}
+
+// ---- near-far refill, after the finish pass of step `step`: lanes still
+// running but without listed work (candidates) list their deferred
+// (vertex, lane) pairs -- distance below the distance last expanded at -- up
+// to threshold + delta.  Per-lane minima of the deferred distances beyond it
+// go to lpart (one row of 32 per workgroup) for the reduction.
+constexpr uint32_t RBLOCKS = GGA_MS_RBLOCKS;
+__global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_t1[LANES], s_min[LANES], s_listed, s_need;
+    if (!ms.L) return;
+    const uint32_t cand = ms.ctl[GGA_CTL_RUN] & ~ms.ctl[GGA_CTL_LANES((step + 1u) % 3u)];
+    if (!cand) return;
+    uint32_t *mnxt = ms_mnxt(ms, step);
+    if (threadIdx.x < LANES) {
+        s_t1[threadIdx.x] = sat_add(ms.ctl[GGA_CTL_LTHR(threadIdx.x)], ms.delta);
+        s_min[threadIdx.x] = 0xFFFFFFFFu;
+    }
+    if (threadIdx.x == 0) {
+        s_listed = 0;
+        s_need = 0;
+    }
+    __syncthreads();
+    uint32_t listed = 0, need = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t m = cand, bits = 0;
+        while (m) {
+            const uint32_t s = (uint32_t)__builtin_ctz(m);
+            m &= m - 1u;
+            const uint64_t k = didx(ms, g.n, (uint32_t)v, s);
+            const uint32_t d = ms.D[k];
+            if (d < ms.L[k]) {
+                if (d < s_t1[s]) bits |= 1u << s;
+                else atomicMin(&s_min[s], d);
+            }
+        }
+        if (bits) {
+            listed |= bits;
+            need |= __hip_atomic_fetch_or(&mnxt[v], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+        }
+    }
+    listed = wave_or(listed);
+    need = wave_or(need);
+    if (lane_id() == 0) {
+        if (listed) atomicOr(&s_listed, listed);
+        if (need) s_need = 1u;
+    }
+    __syncthreads();
+    if (threadIdx.x < LANES) ms.lpart[(uint64_t)blockIdx.x * LANES + threadIdx.x] = s_min[threadIdx.x];
+    if (threadIdx.x == 0) {
+        if (s_listed) __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_RLISTED], s_listed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_need) __hip_atomic_store(&ms.ctl[GGA_CTL_RNEED], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One workgroup: per candidate lane, a listing refill raises its threshold by
+// delta; with nothing under the new threshold but deferred pairs left, the
+// threshold jumps to the smallest deferred distance (listed by the next
+// step's refill); with nothing deferred the lane has converged.
+__global__ __launch_bounds__(BLOCK) void k_ms_refill_reduce(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_done;
+    if (!ms.L) return;
+    const uint32_t sout = (step + 1u) % 3u;
+    const uint32_t cand = ms.ctl[GGA_CTL_RUN] & ~ms.ctl[GGA_CTL_LANES(sout)];
+    if (!cand) return;
+    if (threadIdx.x == 0) s_done = 0;
+    __syncthreads();
+    const uint32_t listed = ms.ctl[GGA_CTL_RLISTED];
+    const uint32_t s = threadIdx.x;
+    if (s < LANES && ((cand >> s) & 1u)) {
+        uint32_t gmin = 0xFFFFFFFFu;
+        for (uint32_t b = 0; b < RBLOCKS; b++) {
+            const uint32_t x = ms.lpart[(uint64_t)b * LANES + s];
+            gmin = x < gmin ? x : gmin;
+        }
+        if ((listed >> s) & 1u) ms.ctl[GGA_CTL_LTHR(s)] = sat_add(ms.ctl[GGA_CTL_LTHR(s)], ms.delta);
+        else if (gmin != 0xFFFFFFFFu) ms.ctl[GGA_CTL_LTHR(s)] = gmin;
+        else atomicOr(&s_done, 1u << s);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ms.ctl[GGA_CTL_RUN] &= ~s_done;
+        ms.ctl[GGA_CTL_LANES(sout)] |= listed;
+        ms.ctl[GGA_CTL_RLISTED] = 0u;
+    }
+    (void)g;
+}
+
+// After a refill listed vertices that were not listed yet: rebuild the next
+// list from all non-zero next masks (the same three passes as a flag step).
+__global__ __launch_bounds__(CBLOCK) void k_ms_rcount(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    if (!ms.L || !ms.ctl[GGA_CTL_RNEED]) return;
+    const ListOut o = ms_list_out(g, ms, step);
+    const MaskMarked mk{ms_mnxt(ms, step)};
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, mk, s_wave);
+}
+
+__global__ __launch_bounds__(CBLOCK) void k_ms_rscan(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    if (!ms.L || !ms.ctl[GGA_CTL_RNEED]) return;
+    compact_scan(ms_list_out(g, ms, step), g.n, s_wave);
+}
+
+__global__ __launch_bounds__(CBLOCK) void k_ms_rwrite(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
+    if (!ms.L || !ms.ctl[GGA_CTL_RNEED]) return;
+    const ListOut o = ms_list_out(g, ms, step);
+    const MaskMarked mk{ms_mnxt(ms, step)};
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
 }
 
 // All steps in one cooperative launch: relax, then the three compaction
@@ -588,6 +742,13 @@ extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int c
     k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
     k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     k_ms_finish<<<grid_for(g.C, 1024), BLOCK, 0, s>>>(g, ms, step);
+    if (ms.L) {
+        k_ms_refill<<<RBLOCKS, BLOCK, 0, s>>>(g, ms, step);
+        k_ms_refill_reduce<<<1, BLOCK, 0, s>>>(g, ms, step);
+        k_ms_rcount<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+        k_ms_rscan<<<1, CBLOCK, 0, s>>>(g, ms, step);
+        k_ms_rwrite<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+    }
     return launch_status();
 }
 

@@ -191,7 +191,6 @@ __global__ __launch_bounds__(BLOCK) void k_query_init(gga_dev_graph g, gga_dev_w
         for (uint32_t i = threadIdx.x; i < nch; i += blockDim.x) w.q0[i] = cs + i;
         if (threadIdx.x < 3) {
             w.ctl[GGA_CTL_TAIL(threadIdx.x)] = threadIdx.x == 0 ? nch : 0u;
-            w.ctl[GGA_CTL_BC(threadIdx.x)] = 0;
             w.ctl[GGA_CTL_ITER(threadIdx.x)] = epoch;
             // near-far: the first bucket is [0, delta); plain worklist: no bound
             w.ctl[GGA_CTL_THR(threadIdx.x)] = w.last ? (w.delta < w.inf ? w.delta : w.inf) : 0xFFFFFFFFu;

@@ -357,6 +357,10 @@ class MultiSourceEngine:
                 unit |= 1 << lane
         check(load().gga_ms_start(self._h, mask, _ptr(src), unit), "gga_ms_start")
 
+    def set_delta(self, delta: int) -> None:
+        """gga_ms_set_delta: near-far bucket width of the SSSP lanes (0: off)."""
+        check(load().gga_ms_set_delta(self._h, delta), "gga_ms_set_delta")
+
     def run(self, max_steps: int = 0, mode: str = "worklist", count: bool = False, **kw) -> int:
         """Runs steps; returns the mask of lanes with work left."""
         o = _opts(mode, 0, count, kw)

@@ -306,6 +306,15 @@ typedef struct gga_ms gga_ms;
 #define GGA_MS_LANE_MAJOR    2
 int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out);
 int gga_ms_free(gga_ms *e);
+/* Near-far buckets per SSSP lane (as opts.delta for one source): a lane's
+ * improvement at or beyond its threshold waits until a refill reaches it.
+ * Set at creation from the graph (gga_tune's width, or 8 mean weights on
+ * mesh-like graphs, else off); delta 0 or 0xFFFFFFFF turns it off.  Only while
+ * no lane runs; near-far engines run in step mode (GGA_MODE_WORKLIST).
+ * +128 B per vertex. */
+int gga_ms_set_delta(gga_ms *e, uint32_t delta);
+/* Device bytes gga_ms_create would take on this graph (near-far included). */
+uint64_t gga_ms_graph_bytes(gga_graph *g);
 /* Restarts the lanes in `lanes` (bit i = lane i) at src[i] (src holds 32
  * entries; only those of `lanes` are read).  Lanes in unit_lanes count hops
  * (BFS levels), the others add edge weights.  A lane still running is EINVAL. */

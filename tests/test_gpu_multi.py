@@ -242,3 +242,49 @@ def test_autotuner_keeps_results_and_seeds_with_histogram(gpu):
             out, _ = dg.multi_sp(srcs, bfs_lanes=0xF0F0F0F0)
             for i, s in enumerate(srcs):
                 np.testing.assert_array_equal(out[i], want_for(g, s, (0xF0F0F0F0 >> i) & 1))
+
+
+@pytest.mark.parametrize("delta", [1, 500, 4000])
+def test_near_far_engine_lanes_equal_oracle(gpu, delta):
+    """Near-far SSSP lanes (per-lane thresholds, refills of deferred pairs)
+    beside BFS lanes: every lane equals the oracle on a pure grid and R-MAT,
+    both layouts, batched and with lanes recycled mid-run."""
+    for g in (gga.gen_grid(60, 80, 0.0, seed=17, wlo=1, whi=1000), gga.gen_rmat(13, 8, seed=5, wlo=1, whi=100)):
+        srcs = nonisolated(g, 32, delta)
+        with gga.DeviceGraph(g) as dg:
+            dg.split(mda=8)
+            for layout in LAYOUT:
+                announce(f"near-far engine n={g.n} delta={delta} layout={layout}")
+                out, st = dg.multi_sp(srcs, bfs_lanes=0x0000FF00, delta=delta, ms_layout=LAYOUT[layout],
+                                      use_graph=int(layout == "vertex"))
+                for i, s in enumerate(srcs):
+                    np.testing.assert_array_equal(out[i], want_for(g, s, 8 <= i < 16), err_msg=f"lane {i}")
+            with gga.MultiSourceEngine(dg) as eng:
+                eng.set_delta(delta)
+                queue = [(int(s), i % 3 == 0) for i, s in enumerate(nonisolated(g, 70, delta + 1))]
+                lane_q, pending, results, rounds = {}, list(enumerate(queue)), {}, 0
+                while pending or lane_q:
+                    start, bfs = {}, set()
+                    for l in range(32):
+                        if l in lane_q or not pending:
+                            continue
+                        qi, (s, b) = pending.pop(0)
+                        lane_q[l] = qi
+                        start[l] = s
+                        if b:
+                            bfs.add(l)
+                    if start:
+                        eng.start(start, bfs)
+                    live = eng.run(max_steps=3, mode="worklist")
+                    rounds += 1
+                    done = [l for l in lane_q if not (live >> l) & 1]
+                    if done:
+                        got = eng.fetch(done)
+                        for l in done:
+                            results[lane_q.pop(l)] = got[l]
+                    assert rounds < 20000
+                for qi, (s, b) in enumerate(queue):
+                    np.testing.assert_array_equal(results[qi], want_for(g, s, b), err_msg=f"query {qi}")
+                with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+                    eng.start({0: int(srcs[0])})
+                    eng.run(mode="persistent")
