@@ -440,7 +440,10 @@ int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live_
         gga_set_error("near-far engines run in step mode (gga_ms_set_delta(e, 0) for the cooperative launch)");
         return GGA_EINVAL;
     }
-    const uint32_t cap = max_steps ? max_steps : e->n + 1u;
+    /* n + 1 Bellman-Ford steps; near-far adds bucket changes (as gga_sssp) */
+    uint32_t cap = max_steps ? max_steps : e->n + 1u;
+    if (!max_steps && e->d.L)
+        cap = (uint64_t)64u * (e->n + 1u) < 0xFFFFFFF0ull ? 64u * (e->n + 1u) : 0xFFFFFFF0u;
     e->d.dense_min = oo.dense_min ? oo.dense_min : e->dense_default;
     uint32_t live = 0, tail = 0, ran = 0;
     float t = 0.f;
