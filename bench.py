@@ -105,11 +105,12 @@ def cpu_baseline(g, src, m_reach):
                        f"{os.cpu_count()} host cores")}, d
 
 
-def batched_leg(gga, np, quick=False):
+def batched_leg(gga, np, quick=False, coop=True):
     """Config 4: 256 BFS (bitmask lanes, one cooperative launch) + 32 SSSP
     (32-lane engine, hipGraph-replayed steps) on R-MAT s20 ef16 seed 13,
     sources uniform among vertices with out-edges; a sample of lanes is
-    checked bit-exact against single-source runs of the same sources."""
+    checked bit-exact against single-source runs of the same sources.
+    coop=False leaves out the cooperative 256-lane BFS."""
     g = gga.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=13, wlo=1, whi=100)
     rng = np.random.default_rng(13)
     cand = np.flatnonzero(g.degrees() > 0)
@@ -121,12 +122,14 @@ def batched_leg(gga, np, quick=False):
         out["tuned"] = {"split_threshold": tr["mda"], "seed_threshold": tr["seed_mda"], "grid": tr["grid"],
                         "dense_min": tr["dense_min"], "layout": {1: "vertex", 2: "lane"}[tr["layout"]]}
         reps = 3 if quick else 5
-        for _ in range(2):
+        wb, sb = None, None
+        for _ in range(2 if coop else 0):
             dg.multi_bfs(bsrc, keep_on_device=True)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            _, sb = dg.multi_bfs(bsrc, keep_on_device=True)
-        wb = (time.perf_counter() - t0) / reps
+        if coop:
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _, sb = dg.multi_bfs(bsrc, keep_on_device=True)
+            wb = (time.perf_counter() - t0) / reps
         for _ in range(2):
             dg.multi_sp(ssrc, use_graph=1, keep_on_device=True)
         t0 = time.perf_counter()
@@ -140,29 +143,32 @@ def batched_leg(gga, np, quick=False):
         t0 = time.perf_counter()
         mr2 = sum(dg.sssp(int(s), keep_on_device=True)[1]["m_reach"] for s in ssrc)
         w1s = (time.perf_counter() - t0) / 32
-        lv, _ = dg.multi_bfs(bsrc)
         ds, _ = dg.multi_sp(ssrc, use_graph=1)
-        for i in range(0, 256, 37):
-            assert np.array_equal(lv[i], dg.bfs(int(bsrc[i]))[0]), f"BFS lane {i}"
+        if coop:
+            lv, _ = dg.multi_bfs(bsrc)
+            for i in range(0, 256, 37):
+                assert np.array_equal(lv[i], dg.bfs(int(bsrc[i]))[0]), f"BFS lane {i}"
         for i in range(0, 32, 5):
             assert np.array_equal(ds[i], dg.sssp(int(ssrc[i]))[0]), f"SSSP lane {i}"
-    out["bfs256"] = {"ms": round(wb * 1e3, 3), "gteps": round(sb["m_reach"] / wb / 1e9, 2),
-                     "levels": sb["iterations"], "pull_levels": sb["pull_levels"],
-                     "single_source_ms_per_query": round(w1b * 1e3, 3),
-                     "speedup_vs_single": round(w1b * 256 / wb, 2)}
+    if coop:
+        out["bfs256"] = {"ms": round(wb * 1e3, 3), "gteps": round(sb["m_reach"] / wb / 1e9, 2),
+                         "levels": sb["iterations"], "pull_levels": sb["pull_levels"],
+                         "single_source_ms_per_query": round(w1b * 1e3, 3),
+                         "speedup_vs_single": round(w1b * 256 / wb, 2)}
     out["sssp32"] = {"ms": round(ws * 1e3, 3), "gteps": round(ss["m_reach"] / ws / 1e9, 2),
                      "steps": ss["iterations"], "single_source_ms_per_query": round(w1s * 1e3, 3),
                      "speedup_vs_single": round(w1s * 32 / ws, 2)}
-    out["checked_lanes_bit_exact"] = len(range(0, 256, 37)) + len(range(0, 32, 5))
+    out["checked_lanes_bit_exact"] = (len(range(0, 256, 37)) if coop else 0) + len(range(0, 32, 5))
     del mr1, mr2
     return out
 
 
-def grid_leg(gga, np, quick=False):
+def grid_leg(gga, np, quick=False, coop=True):
     """Config 3: 4096 x 4096 grid, 4-neighbour undirected, + 0.5% random
     shortcut edges (seed 11), weights [1, 1000], source 0 (corner); BFS and
     SSSP in each launch mode.  Checked on the device (fixpoint + tightness +
-    dist[src] = 0, which fixes the shortest-path distances)."""
+    dist[src] = 0, which fixes the shortest-path distances).  coop=False
+    leaves out the persistent (cooperative) launch mode."""
     side = 1024 if quick else 4096
     g = gga.gen_grid(side, side, 0.005, seed=11, wlo=1, whi=1000)
     out = {"workload": f"{side}x{side} grid + 0.5% shortcuts, weights [1,1000], source 0 (config 3)",
@@ -172,7 +178,7 @@ def grid_leg(gga, np, quick=False):
         for algo in ("bfs", "sssp"):
             run = dg.bfs if algo == "bfs" else dg.sssp
             res = {}
-            for mode in ("auto", "worklist", "persistent"):
+            for mode in ("auto", "worklist", "persistent") if coop else ("worklist",):
                 run(0, mode=mode, keep_on_device=True)
                 reps = 3
                 t0 = time.perf_counter()
@@ -188,7 +194,7 @@ def grid_leg(gga, np, quick=False):
     return out
 
 
-def serving_leg(gga, np, full=False, quick=False):
+def serving_leg(gga, np, full=True, quick=False):
     """Config 5 serving: resident R-MAT graphs (s16.. ef16, weights [1,100])
     and 4-neighbour grids (weights [1,1000]), seed 17; BFS/SSSP 50/50, uniform
     graph and source (among vertices with out-edges).  The server (32-lane
@@ -209,9 +215,9 @@ def serving_leg(gga, np, full=False, quick=False):
         dgs.append(dg)
     sizes = [g.n for _, g in gs]
     degs = [np.flatnonzero(g.degrees() > 0).astype(np.uint32) for _, g in gs]
-    nq = 120 if quick else (2000 if full else 400)
-    nplain = 40 if quick else (200 if full else 80)
-    out = {"workload": ("config 5 resident set" if full else
+    nq = 120 if quick else 400
+    nplain = 40 if quick else 80
+    out = {"workload": ("config 5 resident set: R-MAT s16-s23 + grids 512^2, 1024^2, 2048^2, 4096^2" if full else
                         "reduced config 5 set: R-MAT s16-s20 + grids 512^2, 1024^2") +
                        ", BFS/SSSP 50/50, uniform graph and source",
            "graphs": [n for n, _ in gs], "resident_gb": round(sum(d.device_bytes for d in dgs) / 1e9, 3)}
@@ -262,8 +268,14 @@ def main() -> int:
     ap.add_argument("--no-grid", action="store_true", help="skip the grid leg (config 3)")
     ap.add_argument("--no-batched", action="store_true", help="skip the multi-source leg (config 4)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg (config 5)")
-    ap.add_argument("--serve-full", action="store_true",
-                    help="serving leg on the full config 5 set (R-MAT s16-s23, grids up to 4096^2)")
+    ap.add_argument("--serve-reduced", action="store_true",
+                    help="serving leg on a reduced config 5 set (R-MAT s16-s20, grids 512^2 and 1024^2) "
+                         "instead of the full one (R-MAT s16-s23, grids 512^2 to 4096^2)")
+    ap.add_argument("--leg", choices=("grid", "batched", "serve"),
+                    help="profiling run: only this leg (configs 3, 4, 5), its JSON object on stdout")
+    ap.add_argument("--no-coop", action="store_true",
+                    help="with --leg: leave out cooperative launches (rocprofv3 can lose its output at "
+                         "process exit after them)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times), to scale rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count")
@@ -285,6 +297,12 @@ def main() -> int:
 
     import numpy as np
     import gpugraphalgo_amd as gga
+
+    if args.leg:
+        leg = {"grid": grid_leg, "batched": batched_leg}.get(args.leg)
+        res = leg(gga, np, coop=not args.no_coop) if leg else serving_leg(gga, np, full=not args.serve_reduced)
+        print(json.dumps({"leg": args.leg, **res}))
+        return 0
 
     log(f"generating R-MAT s{args.scale} ef{args.edge_factor} seed {args.seed}")
     g = gga.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed, wlo=1, whi=100)
@@ -324,8 +342,8 @@ def main() -> int:
         batched = batched_leg(gga, np)
         log(f"batched: {json.dumps(batched)}")
     if not (args.no_serve or args.quick):
-        log("serving leg (config 5%s)" % ("" if args.serve_full else ", reduced set"))
-        serving = serving_leg(gga, np, full=args.serve_full)
+        log("serving leg (config 5%s)" % (", reduced set" if args.serve_reduced else ""))
+        serving = serving_leg(gga, np, full=not args.serve_reduced)
         log(f"serving: {json.dumps(serving)}")
 
     traffic = None

@@ -95,7 +95,8 @@ typedef struct gga_dev_ms {
     uint32_t dense_min;  /* a step whose list holds >= dense_min chunks rebuilds the next list
                             by compaction of the masks; smaller ones push */
     /* near-far per lane (NULL / 0: plain Bellman-Ford lanes) */
-    uint32_t *L;         /* distance of (vertex, lane) at its last expansion, D's layout */
+    uint32_t *far;       /* per vertex: lanes whose distance there dropped by a deferred
+                            improvement and was not listed or expanded since (n words) */
     uint32_t *lpart;     /* refill: per-workgroup per-lane minimum deferred distance */
     uint32_t delta;      /* bucket width */
     uint32_t pad2;

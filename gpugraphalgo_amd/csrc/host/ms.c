@@ -99,7 +99,7 @@ int gga_ms_free(gga_ms *e)
     for (int i = 0; i < 4; i++)
         if (e->ev[i]) hipEventDestroy(e->ev[i]);
     ms_release(e->d_sum);
-    ms_release(e->d.L);
+    ms_release(e->d.far);
     ms_release(e->d.lpart);
     if (e->h_sum) hipHostFree(e->h_sum);
     free(e);
@@ -164,17 +164,17 @@ int gga_ms_set_delta(gga_ms *e, uint32_t delta)
     }
     if (delta == 0xFFFFFFFFu) delta = 0;
     HIP_TRY(hipSetDevice(e->v.device));
-    if (delta && !e->d.L) {
-        if ((rc = ms_alloc(e, (void **)&e->d.L, 4ull * GGA_MS_LANES * e->n))) return rc;
+    if (delta && !e->d.far) {
+        if ((rc = ms_alloc(e, (void **)&e->d.far, 4ull * e->n))) return rc;
         if ((rc = ms_alloc(e, (void **)&e->d.lpart, 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS))) return rc;
-        K_TRY(ggak_fill_u32(e->d.L, e->d.inf, (uint64_t)GGA_MS_LANES * e->n, e->v.stream));
+        K_TRY(ggak_fill_u32(e->d.far, 0u, e->n, e->v.stream));
         HIP_TRY(hipStreamSynchronize(e->v.stream));
-    } else if (!delta && e->d.L) {
+    } else if (!delta && e->d.far) {
         HIP_TRY(hipStreamSynchronize(e->v.stream));
-        ms_release(e->d.L);
+        ms_release(e->d.far);
         ms_release(e->d.lpart);
-        e->dev_bytes -= 4ull * GGA_MS_LANES * e->n + 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
-        e->d.L = NULL;
+        e->dev_bytes -= 4ull * e->n + 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
+        e->d.far = NULL;
         e->d.lpart = NULL;
     }
     e->d.delta = delta;
@@ -194,7 +194,7 @@ uint64_t gga_ms_graph_bytes(gga_graph *g)
     gga_graph_view_t v;
     if (!g || gga_graph_view(g, &v)) return 0;
     uint64_t b = gga_ms_bytes(v.dg.n, v.dg.C);
-    if (gga_graph_auto_delta(g, 0)) b += 4ull * GGA_MS_LANES * v.dg.n + 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
+    if (gga_graph_auto_delta(g, 0)) b += 4ull * v.dg.n + 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
     return b;
 }
 
@@ -306,7 +306,7 @@ fail:
 static void ms_slot(const gga_ms *e, uint32_t *live, uint32_t *tail)
 {
     *tail = e->h_pin[GGA_CTL_TAIL(e->step % 3u)];
-    if (e->d.L) *live = e->h_pin[GGA_CTL_RUN];
+    if (e->d.far) *live = e->h_pin[GGA_CTL_RUN];
     else *live = *tail ? e->h_pin[GGA_CTL_LANES(e->step % 3u)] : 0u;
 }
 
@@ -436,13 +436,13 @@ int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live_
     else gga_opts_init(&oo);
     const int count = oo.count ? 1 : 0;
     const int persistent = oo.mode == GGA_MODE_PERSISTENT;
-    if (persistent && e->d.L) {
+    if (persistent && e->d.far) {
         gga_set_error("near-far engines run in step mode (gga_ms_set_delta(e, 0) for the cooperative launch)");
         return GGA_EINVAL;
     }
     /* n + 1 Bellman-Ford steps; near-far adds bucket changes (as gga_sssp) */
     uint32_t cap = max_steps ? max_steps : e->n + 1u;
-    if (!max_steps && e->d.L)
+    if (!max_steps && e->d.far)
         cap = (uint64_t)64u * (e->n + 1u) < 0xFFFFFFF0ull ? 64u * (e->n + 1u) : 0xFFFFFFF0u;
     e->d.dense_min = oo.dense_min ? oo.dense_min : e->dense_default;
     uint32_t live = 0, tail = 0, ran = 0;

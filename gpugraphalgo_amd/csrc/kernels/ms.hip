@@ -61,7 +61,7 @@ __device__ __forceinline__ uint32_t *ms_mcur(const gga_dev_ms &ms, uint32_t step
 __device__ __forceinline__ uint32_t *ms_mnxt(const gga_dev_ms &ms, uint32_t step) {
     return (step & 1u) ? ms.M0 : ms.M1;
 }
-// index of (vertex v, lane s) in D / L for the engine's layout
+// index of (vertex v, lane s) in D for the engine's layout
 __device__ __forceinline__ uint64_t didx(const gga_dev_ms &ms, uint32_t n, uint32_t v, uint32_t s) {
     return ms.lane_major ? (uint64_t)s * n + v : (uint64_t)v * LANES + s;
 }
@@ -123,8 +123,9 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     const uint32_t l = lane_id(), half = l >> 5, s = l & (LANES - 1u);
     const bool unit = (ms.ctl[GGA_CTL_UNIT] >> s) & 1u;
     // near-far: an improvement at or beyond the lane's threshold lowers the
-    // distance but is deferred (no mask bit) until a refill reaches it
-    const uint32_t thr_s = ms.L ? ms.ctl[GGA_CTL_LTHR(s)] : 0xFFFFFFFFu;
+    // distance but is deferred (no mask bit, a far bit instead) until a refill
+    // reaches it
+    const uint32_t thr_s = ms.far ? ms.ctl[GGA_CTL_LTHR(s)] : 0xFFFFFFFFu;
     for (uint32_t b = wid; b < nbatch; b += waves) {
         const uint32_t idx = b * bsz + l;
         const bool valid = l < bsz && idx < nq;
@@ -181,6 +182,10 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                 bool first = false;
                 if (s == 0 && hm)
                     first = __hip_atomic_fetch_or(&mnxt[v[i]], hm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+                if (ms.far) {
+                    const uint32_t fm = (uint32_t)(__ballot(imp && a >= thr_s) >> (32u * half));
+                    if (s == 0 && fm) atomicOr(&ms.far[v[i]], fm);
+                }
                 if (COUNT) {
                     cnt.edges += (uint32_t)__popcll(__ballot(on));
                     cnt.relax += (uint32_t)__popcll(__ballot(imp));
@@ -222,23 +227,15 @@ __device__ __forceinline__ void ms_finish(const gga_dev_graph &g, const gga_dev_
     const uint32_t *mnxt = ms_mnxt(ms, step);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // the thread holding an owner's first chunk retires the owner: records the
-    // distances its lanes were expanded at (near-far) and clears its mask.  A
-    // lane whose distance dropped during the step has its bit in the next mask
-    // (the old distance was under the threshold, so the new one is), so the
-    // current distance is the expanded one for every lane not re-listed.
+    // the thread holding an owner's first chunk retires the owner: clears its
+    // mask and (near-far) the far bits of the lanes it expanded.  No expanded
+    // lane can have been deferred at the owner during the step: its distance
+    // was under the threshold, so every drop during the step was too.
     for (uint64_t i = tid; i < nin; i += stride) {
         const uint32_t c = qin[i], u = g.cown[c];
         if (c != g.cstart[u]) continue;
-        if (ms.L) {
-            uint32_t m = mcur[u];
-            while (m) {
-                const uint32_t s = (uint32_t)__builtin_ctz(m);
-                m &= m - 1u;
-                const uint64_t k = didx(ms, g.n, u, s);
-                ms.L[k] = ms.D[k];
-            }
-        }
+        const uint32_t m = mcur[u];
+        if (ms.far && (ms.far[u] & m)) ms.far[u] &= ~m;
         mcur[u] = 0u;
     }
     uint32_t x = 0;
@@ -268,10 +265,8 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reset(gga_dev_graph g, gga_dev_ms 
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t s = (uint32_t)(i & (LANES - 1u)), v = (uint32_t)(i / LANES);
-        if ((lanes >> s) & 1u) {
-            ms.D[i] = v == src.v[s] ? 0u : ms.inf;
-            if (ms.L) ms.L[i] = ms.inf;
-        }
+        if ((lanes >> s) & 1u) ms.D[i] = v == src.v[s] ? 0u : ms.inf;
+        if (s == 0 && ms.far) ms.far[v] &= ~lanes;
     }
 }
 
@@ -281,10 +276,11 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reset_lm(gga_dev_graph g, gga_dev_
     const uint32_t s = blockIdx.y;
     if (!((lanes >> s) & 1u)) return;
     uint32_t *row = ms.D + (uint64_t)s * g.n;
-    uint32_t *lrow = ms.L ? ms.L + (uint64_t)s * g.n : nullptr;
+    // the row of the lowest restarted lane clears the far bits of all of them
+    const bool clear_far = ms.far && s == (uint32_t)__builtin_ctz(lanes);
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x) {
         row[v] = v == src.v[s] ? 0u : ms.inf;
-        if (lrow) lrow[v] = ms.inf;
+        if (clear_far) ms.far[v] &= ~lanes;
     }
 }
 
@@ -299,7 +295,7 @@ __global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, 
         ms.ctl[GGA_CTL_RUN] |= lanes;
     }
     if (s < LANES && ((lanes >> s) & 1u))   // BFS lanes are one bucket per level already
-        ms.ctl[GGA_CTL_LTHR(s)] = (ms.L && ms.delta && !((unit >> s) & 1u)) ? ms.delta : 0xFFFFFFFFu;
+        ms.ctl[GGA_CTL_LTHR(s)] = (ms.far && ms.delta && !((unit >> s) & 1u)) ? ms.delta : 0xFFFFFFFFu;
     if (s >= LANES || !((lanes >> s) & 1u)) return;
     const uint32_t x = src.v[s];
     uint32_t *mcur = ms_mcur(ms, step);
@@ -358,13 +354,14 @@ __global__ __launch_bounds__(BLOCK) void k_ms_finish(gga_dev_graph g, gga_dev_ms
 
 // ---- near-far refill, after the finish pass of step `step`: lanes still
 // running but without listed work (candidates) list their deferred
-// (vertex, lane) pairs -- distance below the distance last expanded at -- up
-// to threshold + delta.  Per-lane minima of the deferred distances beyond it
-// go to lpart (one row of 32 per workgroup) for the reduction.
+// (vertex, lane) pairs -- the far bits -- whose distance is under threshold +
+// delta, and clear those bits.  Per-lane minima of the deferred distances
+// beyond it go to lpart (one row of 32 per workgroup) for the reduction.  The
+// scan reads one far word per vertex and D only where a candidate's bit is set.
 constexpr uint32_t RBLOCKS = GGA_MS_RBLOCKS;
 __global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_t1[LANES], s_min[LANES], s_listed, s_need;
-    if (!ms.L) return;
+    if (!ms.far) return;
     const uint32_t cand = ms.ctl[GGA_CTL_RUN] & ~ms.ctl[GGA_CTL_LANES((step + 1u) % 3u)];
     if (!cand) return;
     uint32_t *mnxt = ms_mnxt(ms, step);
@@ -379,18 +376,17 @@ __global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms
     __syncthreads();
     uint32_t listed = 0, need = 0;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t m = cand, bits = 0;
+        const uint32_t fw = ms.far[v];
+        uint32_t m = fw & cand, bits = 0;
         while (m) {
             const uint32_t s = (uint32_t)__builtin_ctz(m);
             m &= m - 1u;
-            const uint64_t k = didx(ms, g.n, (uint32_t)v, s);
-            const uint32_t d = ms.D[k];
-            if (d < ms.L[k]) {
-                if (d < s_t1[s]) bits |= 1u << s;
-                else atomicMin(&s_min[s], d);
-            }
+            const uint32_t d = ms.D[didx(ms, g.n, (uint32_t)v, s)];
+            if (d < s_t1[s]) bits |= 1u << s;
+            else atomicMin(&s_min[s], d);
         }
         if (bits) {
+            ms.far[v] = fw & ~bits;
             listed |= bits;
             need |= __hip_atomic_fetch_or(&mnxt[v], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
         }
@@ -415,7 +411,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms
 // step's refill); with nothing deferred the lane has converged.
 __global__ __launch_bounds__(BLOCK) void k_ms_refill_reduce(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_done;
-    if (!ms.L) return;
+    if (!ms.far) return;
     const uint32_t sout = (step + 1u) % 3u;
     const uint32_t cand = ms.ctl[GGA_CTL_RUN] & ~ms.ctl[GGA_CTL_LANES(sout)];
     if (!cand) return;
@@ -446,7 +442,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_refill_reduce(gga_dev_graph g, gga
 // list from all non-zero next masks (the same three passes as a flag step).
 __global__ __launch_bounds__(CBLOCK) void k_ms_rcount(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
-    if (!ms.L || !ms.ctl[GGA_CTL_RNEED]) return;
+    if (!ms.far || !ms.ctl[GGA_CTL_RNEED]) return;
     const ListOut o = ms_list_out(g, ms, step);
     const MaskMarked mk{ms_mnxt(ms, step)};
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, mk, s_wave);
@@ -454,13 +450,13 @@ __global__ __launch_bounds__(CBLOCK) void k_ms_rcount(gga_dev_graph g, gga_dev_m
 
 __global__ __launch_bounds__(CBLOCK) void k_ms_rscan(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
-    if (!ms.L || !ms.ctl[GGA_CTL_RNEED]) return;
+    if (!ms.far || !ms.ctl[GGA_CTL_RNEED]) return;
     compact_scan(ms_list_out(g, ms, step), g.n, s_wave);
 }
 
 __global__ __launch_bounds__(CBLOCK) void k_ms_rwrite(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
-    if (!ms.L || !ms.ctl[GGA_CTL_RNEED]) return;
+    if (!ms.far || !ms.ctl[GGA_CTL_RNEED]) return;
     const ListOut o = ms_list_out(g, ms, step);
     const MaskMarked mk{ms_mnxt(ms, step)};
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
@@ -742,7 +738,7 @@ extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int c
     k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
     k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     k_ms_finish<<<grid_for(g.C, 1024), BLOCK, 0, s>>>(g, ms, step);
-    if (ms.L) {
+    if (ms.far) {
         k_ms_refill<<<RBLOCKS, BLOCK, 0, s>>>(g, ms, step);
         k_ms_refill_reduce<<<1, BLOCK, 0, s>>>(g, ms, step);
         k_ms_rcount<<<cg, CBLOCK, 0, s>>>(g, ms, step);

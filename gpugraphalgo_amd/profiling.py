@@ -8,6 +8,7 @@
         D1..D4/*counter_collection.csv (rocprofv3 --pmc FETCH_SIZE, --pmc WRITE_SIZE, for
         the workload and for a plain copy of B bytes) -> HBM bytes per dispatch of NAME,
         each counter scaled by what it reads for the copy (bytes / (KiB * 1024)).
+        --last N: only the last N dispatches of NAME (one run at the end of the workload).
 
 Summaries are `key = value` text; read_summary() parses them back.
 """
@@ -50,20 +51,22 @@ def _counter(d: str, name: str, kernel_substr: str):
 
 
 def summarize_traffic(kernel: str, fetch: str, write: str, copy_fetch: str, copy_write: str,
-                      copy_bytes: int, cmd: str) -> str:
+                      copy_bytes: int, cmd: str, last: int = 0) -> str:
     cf = _counter(copy_fetch, "FETCH_SIZE", "copyBuffer")
     cw = _counter(copy_write, "WRITE_SIZE", "copyBuffer")
     f_fetch = copy_bytes / (sum(cf) / len(cf) * 1024.0)
     f_write = copy_bytes / (sum(cw) / len(cw) * 1024.0)
     kf = _counter(fetch, "FETCH_SIZE", kernel)
     kw = _counter(write, "WRITE_SIZE", kernel)
+    if last:
+        kf, kw = kf[-last:], kw[-last:]
     fetch_b = sum(kf) / len(kf) * 1024.0 * f_fetch
     write_b = sum(kw) / len(kw) * 1024.0 * f_write
     lines = [
         "# HBM traffic per dispatch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)",
         f"command = {cmd}",
         f"kernel = {kernel}",
-        f"dispatches = {len(kf)}",
+        f"dispatches = {len(kf)}" + (f" (the last {last})" if last else ""),
         f"copy_bytes = {copy_bytes}",
         f"fetch_factor = {f_fetch:.4f}",
         f"write_factor = {f_write:.4f}",
@@ -95,12 +98,13 @@ def main(argv=None) -> int:
     for k in ("kernel", "fetch", "write", "copy-fetch", "copy-write", "cmd"):
         t.add_argument("--" + k, required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
+    t.add_argument("--last", type=int, default=0)
     a = ap.parse_args(argv)
     if a.what == "stats":
         sys.stdout.write(summarize_stats(a.dir, a.cmd))
     else:
         sys.stdout.write(summarize_traffic(a.kernel, a.fetch, a.write, a.copy_fetch, a.copy_write,
-                                           a.copy_bytes, a.cmd))
+                                           a.copy_bytes, a.cmd, a.last))
     return 0
 
 
