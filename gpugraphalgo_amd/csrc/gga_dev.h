@@ -48,6 +48,7 @@ typedef struct gga_dev_work {
     uint32_t qcap;       /* capacity of q0 / q1 in entries (>= C) */
     uint32_t dense_min;  /* a step whose list holds >= dense_min chunks is a flag step */
     uint32_t delta;      /* near-far bucket width (0: plain Bellman-Ford worklist) */
+    uint32_t bmin;       /* fewest chunks per wave batch */
 } gga_dev_work;
 
 /* Per-workgroup timeline record of an instrumented relax step: start on the
@@ -99,7 +100,7 @@ typedef struct gga_dev_ms {
                             improvement and was not listed or expanded since (n words) */
     uint32_t *lpart;     /* refill: per-workgroup per-lane minimum deferred distance */
     uint32_t delta;      /* bucket width */
-    uint32_t pad2;
+    uint32_t bmin;       /* fewest chunks per wave batch (more on meshes, whose lanes rarely meet) */
 } gga_dev_ms;
 typedef struct gga_ms_sources { uint32_t v[GGA_MS_LANES]; } gga_ms_sources;
 typedef struct gga_ms_outputs { uint32_t *p[GGA_MS_LANES]; } gga_ms_outputs;

@@ -30,6 +30,7 @@ void gga_graph_set_tuned_sssp(gga_graph *g, int32_t mode, uint32_t delta);
 uint32_t gga_graph_wmean(const gga_graph *g);
 /* Near-far width a query with opts.delta == 0 would use (0: off). */
 uint32_t gga_graph_auto_delta(const gga_graph *g, int bfs);
+int gga_graph_mesh_like(const gga_graph *g);   /* max degree within 4x the mean (+8) */
 uint32_t gga_graph_maxdeg(const gga_graph *g);
 uint32_t gga_graph_default_grid(gga_graph *g, int bfs);
 /* State of gga_multi_bfs (msbfs.c), owned by the graph. */

@@ -17,6 +17,11 @@
 #include "../gga_kernels.h"
 #include "gga_host.h"
 
+#ifndef GGA_MESH_BMIN
+#define GGA_MESH_BMIN 16u   /* chunks per wave batch on mesh-like graphs */
+#endif
+static int mesh_like(const gga_graph *g);
+
 #define GGA_VERSION "gga 0.1.0 (gfx950)"
 
 struct gga_graph {
@@ -141,6 +146,7 @@ static gga_dev_work dev_work(const gga_graph *g, uint32_t inf)
     w.last = NULL;
     w.tmin = NULL;
     w.delta = 0;
+    w.bmin = 1;
     return w;
 }
 
@@ -555,6 +561,7 @@ static uint32_t auto_delta(const gga_graph *g, int bfs)
 }
 
 uint32_t gga_graph_auto_delta(const gga_graph *g, int bfs) { return g ? auto_delta(g, bfs) : 0u; }
+int gga_graph_mesh_like(const gga_graph *g) { return g ? mesh_like(g) : 0; }
 
 static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gga_opts *user,
                     gga_stats *st)
@@ -602,6 +609,7 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
     gga_dev_work dw = dev_work(g, inf);
     dw.dense_min = o.dense_min ? o.dense_min
                                : g->tuned_dense ? g->tuned_dense : (g->n / 64u > 4096u ? g->n / 64u : 4096u);
+    dw.bmin = mesh_like(g) ? GGA_MESH_BMIN : 1u;
     /* near-far buckets: SSSP only (BFS levels are already one bucket per level) */
     if (near_far) {
         if (!g->d_last && (rc = dmalloc(g, (void **)&g->d_last, 4ull * g->n))) return rc;

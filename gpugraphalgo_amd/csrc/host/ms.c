@@ -128,6 +128,9 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
     e->d.lane_major = layout == GGA_MS_LANE_MAJOR ? 1u : 0u;
     e->dense_default = e->n / 64u > 4096u ? e->n / 64u : 4096u;
     e->d.dense_min = e->dense_default;
+    /* meshes: lanes rarely share a vertex, batches flatten (edge, lane) pairs,
+     * so each wave takes at least 16 low-degree chunks per batch */
+    e->d.bmin = gga_graph_mesh_like(g) ? 16u : 1u;
     HIP_TRY(hipSetDevice(e->v.device));
     for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&e->ev[i]));
     HIP_TRY(hipHostMalloc((void **)&e->h_pin, 4ull * GGA_CTL_WORDS, hipHostMallocDefault));

@@ -12,7 +12,9 @@
 // relaxes one edge for all 32 lanes: lane s of the half reads dist[u][s] and
 // dist[v][s] (both one coalesced line), col[e] / w[e] are broadcast loads,
 // and the lanes that improved are published with one atomicOr into the
-// destination's next mask.  The next list, chosen per step from the length of
+// destination's next mask.  A batch whose chunks carry few lanes each (lanes
+// started far apart on a mesh rarely meet) is instead flattened into
+// (edge, lane) pairs, one per wave lane, so idle lanes cost no slots.  The next list, chosen per step from the length of
 // the current one: a small list pushes each vertex whose next mask was empty
 // (the atomicOr's old value says which improver is first, so a vertex is
 // listed once however many lanes reach it); a large list lets the compaction
@@ -35,6 +37,10 @@ constexpr uint32_t LANES = GGA_MS_LANES;
 #define GGA_MS_UNROLL 4
 #endif
 constexpr uint32_t MS_UNROLL = GGA_MS_UNROLL;  // edges per half wave per round
+#ifndef GGA_MS_FLAT
+#define GGA_MS_FLAT 16
+#endif
+// a batch averaging at most GGA_MS_FLAT lanes per edge slot is flattened
 constexpr uint32_t PBUF = 256;                 // pushed vertices staged per wave before a flush
 
 struct Counters {
@@ -92,6 +98,12 @@ __device__ __forceinline__ ListOut ms_list_out(const gga_dev_graph &g, const gga
     return o;
 }
 
+// index of the i-th set bit of m (i < popcount(m))
+__device__ __forceinline__ uint32_t nth_set_bit(uint32_t m, uint32_t i) {
+    for (; i; i--) m &= m - 1u;
+    return (uint32_t)__builtin_ctz(m);
+}
+
 __device__ __forceinline__ uint32_t wave_or(uint32_t x) {
 #pragma unroll
     for (uint32_t o = 32; o >= 1; o >>= 1) x |= __shfl_xor(x, (int)o, GGA_WAVE);
@@ -117,11 +129,13 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     const uint32_t inf = ms.inf;
     const uint32_t waves = gridDim.x * WAVES;
     const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
-    uint32_t bsz = nq / (8u * waves);
+    uint32_t bsz = ms.bmin > 1u ? nq >> 10 : nq / (8u * waves);   // as in the single-source step
+    bsz = bsz < ms.bmin ? ms.bmin : bsz;
     bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
     const uint32_t nbatch = (nq + bsz - 1) / bsz;
     const uint32_t l = lane_id(), half = l >> 5, s = l & (LANES - 1u);
-    const bool unit = (ms.ctl[GGA_CTL_UNIT] >> s) & 1u;
+    const uint32_t unit_word = ms.ctl[GGA_CTL_UNIT];
+    const bool unit = (unit_word >> s) & 1u;
     // near-far: an improvement at or beyond the lane's threshold lowers the
     // distance but is deferred (no mask bit, a far bit instead) until a refill
     // reaches it
@@ -141,6 +155,80 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
         const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
         const uint32_t excl = incl - ne;
         if (COUNT) cnt.chunks += (uint32_t)__popcll(__ballot(ne > 0));
+        // (edge, lane) pairs of the batch; with few lanes per edge slot on
+        // average, one pair per wave lane beats one edge per half wave
+        const uint32_t pc = (uint32_t)__popc(m);
+        const uint32_t pairs = pc * ne;
+        const uint32_t pincl = wave_incl_scan(pairs);
+        const uint32_t ptotal = wave_bcast(pincl, GGA_WAVE - 1);
+        if (ptotal <= GGA_MS_FLAT * total) {
+            const uint32_t pexcl = pincl - pairs;
+            for (uint32_t base = 0; base < ptotal; base += GGA_WAVE * MS_UNROLL) {
+                uint32_t v[MS_UNROLL], wt[MS_UNROLL], du[MS_UNROLL], dv[MS_UNROLL], sl[MS_UNROLL];
+                bool act[MS_UNROLL];
+#pragma unroll
+                for (uint32_t i = 0; i < MS_UNROLL; i++) {
+                    // pair k of owner j: edge r / pc_j, the (r % pc_j)-th lane of its mask,
+                    // so consecutive pairs are the lanes of one edge (one D line)
+                    const uint32_t k = base + i * GGA_WAVE + l;
+                    act[i] = k < ptotal;
+                    const uint32_t kk = act[i] ? k : ptotal - 1u;
+                    const uint32_t j = wave_owner(pincl, kk);
+                    const uint32_t r = kk - wave_bcast(pexcl, j);
+                    const uint32_t pcj = wave_bcast(pc, j);
+                    const uint32_t eo = r / pcj;
+                    sl[i] = nth_set_bit(wave_bcast(m, j), r - eo * pcj);
+                    const uint32_t e = wave_bcast(eb, j) + eo;
+                    const uint32_t uu = wave_bcast(u, j);
+                    v[i] = stream_load(&g.col[e]);
+                    wt[i] = stream_load(&g.w[e]);
+                    du[i] = D[LM ? (uint64_t)sl[i] * g.n + uu : (uint64_t)uu * LANES + sl[i]];
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < MS_UNROLL; i++)
+                    dv[i] = D[LM ? (uint64_t)sl[i] * g.n + v[i] : (uint64_t)v[i] * LANES + sl[i]];
+#pragma unroll
+                for (uint32_t i = 0; i < MS_UNROLL; i++) {
+                    const uint32_t thr = (uint32_t)__shfl(thr_s, (int)sl[i], GGA_WAVE);
+                    const bool on = act[i] && du[i] < inf;
+                    const uint32_t a = du[i] + (((unit_word >> sl[i]) & 1u) ? 1u : wt[i]);
+                    bool imp = false, first = false;
+                    if (on && a >= du[i] && a < inf && a < dv[i]) {
+                        const uint32_t old = atomic_min_u32(
+                            &D[LM ? (uint64_t)sl[i] * g.n + v[i] : (uint64_t)v[i] * LANES + sl[i]], a);
+                        imp = a < old;
+                    }
+                    if (imp && a < thr)
+                        first = __hip_atomic_fetch_or(&mnxt[v[i]], 1u << sl[i], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT) == 0u;
+                    else if (imp && ms.far)
+                        atomicOr(&ms.far[v[i]], 1u << sl[i]);
+                    if (COUNT) {
+                        cnt.edges += (uint32_t)__popcll(__ballot(on));
+                        cnt.relax += (uint32_t)__popcll(__ballot(imp));
+                    }
+                    if (push) {
+                        const uint64_t pm = __ballot(first);
+                        if (pm) {
+                            const uint32_t np = (uint32_t)__popcll(pm);
+                            if (pcount + np > PBUF) {
+                                flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
+                                pcount = 0;
+                            }
+                            if (first) {
+                                const uint32_t pos = pcount + __builtin_amdgcn_mbcnt_hi(
+                                                                  (uint32_t)(pm >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+                                pbuf[pos] = v[i];
+                            }
+                            pcount += np;
+                            __builtin_amdgcn_wave_barrier();
+                        }
+                    }
+                }
+            }
+            continue;
+        }
         // edge slot k goes to half (k & 1); slots past the end are clamped to
         // the last slot so every address below is in bounds
         for (uint32_t base = 0; base < total; base += 2u * MS_UNROLL) {

@@ -242,11 +242,15 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     const uint32_t *qin = (step & 1u) ? w.q1 : w.q0;
     uint32_t *qout = (step & 1u) ? w.q0 : w.q1;
     uint32_t *tail = &w.ctl[GGA_CTL_TAIL(sout)];
-    // batches of bsz chunks dealt round-robin: at least ~8 batches per wave
-    // when the list is long enough, at most 64 chunks per batch
+    // batches of bsz chunks dealt round-robin, at most 64 chunks each.
+    // Ragged chunks (bmin 1): at least ~8 batches per wave when the list is
+    // long enough, for balance.  Uniform small chunks (meshes, bmin > 1): few
+    // large batches -- every batch is a chain of dependent loads -- with one
+    // batch per 1/1024 of the list, but at least bmin chunks
     const uint32_t waves = gridDim.x * WAVES;
     const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
-    uint32_t bsz = nq / (8u * waves);
+    uint32_t bsz = w.bmin > 1u ? nq >> 10 : nq / (8u * waves);
+    bsz = bsz < w.bmin ? w.bmin : bsz;
     bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
     const uint32_t nbatch = (nq + bsz - 1) / bsz;
     const bool flags = flag_step(w, nq);
