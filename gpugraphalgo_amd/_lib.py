@@ -72,7 +72,7 @@ class Stats(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", u64), ("steps_per_round", u32), ("lanes", u32), ("use_graph", u32),
                 ("dense_min", u32), ("host_results", u32), ("dry_run", u32), ("dry_steps", u32),
-                ("layout", u32)]
+                ("layout", u32), ("engines_per_graph", u32)]
 
 
 class Query(ctypes.Structure):
@@ -82,7 +82,7 @@ class Query(ctypes.Structure):
 
 class QueryResult(ctypes.Structure):
     _fields_ = [("id", u64), ("graph", u32), ("src", u32), ("bfs", u32), ("lane", u32),
-                ("status", i32), ("pad", u32), ("arrival", ctypes.c_double),
+                ("status", i32), ("engine", u32), ("arrival", ctypes.c_double),
                 ("admitted", ctypes.c_double), ("done", ctypes.c_double), ("checksum", u64)]
 
 

@@ -345,6 +345,9 @@ int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets,
     if (!g) return GGA_EINVAL;
     gga_clear_error();
     HIP_TRY(hipSetDevice(g->device));
+    /* multi-source engines read the chunk table from streams of their own:
+     * let their work drain before it is rebuilt */
+    HIP_TRY(hipDeviceSynchronize());
     uint32_t T;
     uint32_t bucket_rec = 0;
     int32_t level_rec = level;

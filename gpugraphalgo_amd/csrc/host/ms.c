@@ -26,6 +26,7 @@ struct gga_ms {
     uint32_t *h_pin;          /* GGA_CTL_WORDS */
     uint32_t *d_stage;        /* 32 x n staging for host fetches (lazy) */
     hipGraphExec_t gx[6];     /* six-step groups captured per phase */
+    hipStream_t own;          /* the engine's stream (v.stream points to it) */
     uint32_t gx_grid[6], gx_dense[6];
     int gx_count[6];
     uint32_t dense_default;
@@ -102,6 +103,7 @@ int gga_ms_free(gga_ms *e)
     ms_release(e->d.far);
     ms_release(e->d.lpart);
     if (e->h_sum) hipHostFree(e->h_sum);
+    if (e->own) hipStreamDestroy(e->own);
     free(e);
     return GGA_OK;
 }
@@ -132,6 +134,10 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
      * so each wave takes at least 16 low-degree chunks per batch */
     e->d.bmin = gga_graph_mesh_like(g) ? 16u : 1u;
     HIP_TRY(hipSetDevice(e->v.device));
+    /* a stream of its own, so that engines of one graph run side by side */
+    HIP_TRY(hipStreamSynchronize(e->v.stream));
+    HIP_TRY(hipStreamCreate(&e->own));
+    e->v.stream = e->own;
     for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&e->ev[i]));
     HIP_TRY(hipHostMalloc((void **)&e->h_pin, 4ull * GGA_CTL_WORDS, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&e->h_sum, 8ull * GGA_MS_LANES, hipHostMallocDefault));

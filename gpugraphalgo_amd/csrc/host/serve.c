@@ -3,19 +3,21 @@
  * admitted by device-memory budget and batched across queries as they arrive.
  *
  * No reference counterpart (the reference answers one query per process,
- * bfs_sssp.cu:103-539).  Each resident graph gets, on demand, one 32-lane
- * multi-source engine (ms.c); a query occupies one lane from admission to
- * completion.  One round of gga_server_pump:
- *   1. admission, in arrival order: a query whose graph has an engine with a
- *      free lane takes it; a graph without an engine gets one if the budget
- *      allows, after evicting idle engines (least recently used) if needed; a
- *      query whose engine alone exceeds the budget is rejected (GGA_EBUDGET);
- *      a query that cannot be placed waits without blocking later queries of
- *      other graphs;
+ * bfs_sssp.cu:103-539).  Each resident graph gets, on demand, up to
+ * `engines_per_graph` 32-lane multi-source engines (ms.c); a query occupies
+ * one lane from admission to completion.  One round of gga_server_pump:
+ *   1. admission, in arrival order: a query takes a free lane of its graph's
+ *      fullest engine that has one; with every engine of the graph full (or
+ *      none yet) the graph gets another engine if it may hold one more and
+ *      the budget allows, after evicting idle engines (least recently used)
+ *      if needed; a query whose engine alone exceeds the budget is rejected
+ *      (GGA_EBUDGET); a query that cannot be placed waits without blocking
+ *      later queries of other graphs;
  *   2. restart of the assigned lanes (per engine, one call);
  *   3. `steps_per_round` steps launched on every engine with running lanes --
- *      each graph owns a stream, so different graphs' steps overlap on the
- *      device -- then collected;
+ *      each engine owns a stream, so steps of different engines overlap on
+ *      the device (thin frontiers of long-diameter graphs leave it mostly
+ *      idle otherwise) -- then collected;
  *   4. lanes that converged deliver their result (device fingerprint, and a
  *      copy to the query's buffer if it gave one) and free up for the next
  *      round's admissions.
@@ -30,11 +32,8 @@
 
 #include "gga_host.h"
 
-typedef struct srv_graph {
-    gga_graph *g;             /* NULL on a dry-run server */
-    uint32_t n, C;
-    uint64_t engine_bytes;
-    gga_ms *eng;
+typedef struct srv_engine {
+    gga_ms *e;                /* NULL on a dry-run server */
     int resident;
     uint32_t busy;            /* lanes running */
     uint64_t lane_q[GGA_MS_LANES];
@@ -43,6 +42,13 @@ typedef struct srv_graph {
     uint32_t start_mask, start_unit;
     uint32_t start_src[GGA_MS_LANES];
     int launched;
+} srv_engine;
+
+typedef struct srv_graph {
+    gga_graph *g;             /* NULL on a dry-run server */
+    uint32_t n, C;
+    uint64_t engine_bytes;
+    srv_engine eng[GGA_SERVER_MAX_ENGINES];
 } srv_graph;
 
 typedef struct srv_query {
@@ -103,6 +109,8 @@ int gga_server_create(const gga_server_opts *o, gga_server **out)
     if (!s->o.steps_per_round) s->o.steps_per_round = 6;
     if (!s->o.lanes || s->o.lanes > GGA_MS_LANES) s->o.lanes = GGA_MS_LANES;
     if (!s->o.dry_steps) s->o.dry_steps = 3;
+    if (!s->o.engines_per_graph) s->o.engines_per_graph = 2;
+    if (s->o.engines_per_graph > GGA_SERVER_MAX_ENGINES) s->o.engines_per_graph = GGA_SERVER_MAX_ENGINES;
     clock_gettime(CLOCK_MONOTONIC, &s->t0);
     pthread_mutex_init(&s->mu, NULL);
     *out = s;
@@ -113,7 +121,8 @@ int gga_server_free(gga_server *s)
 {
     if (!s) return GGA_OK;
     for (uint32_t i = 0; i < s->ngr; i++)
-        if (s->gr[i].eng) gga_ms_free(s->gr[i].eng);
+        for (uint32_t k = 0; k < GGA_SERVER_MAX_ENGINES; k++)
+            if (s->gr[i].eng[k].e) gga_ms_free(s->gr[i].eng[k].e);
     free(s->gr);
     free(s->qs);
     free(s->pend);
@@ -195,29 +204,62 @@ static void finish_query(gga_server *s, uint64_t qid, int status, double now)
 }
 
 /* Frees idle engines, least recently used first, until `need` more bytes fit. */
-static int make_room(gga_server *s, uint64_t need, uint32_t keep)
+static int make_room(gga_server *s, uint64_t need)
 {
     while (s->used + need > s->o.hbm_budget) {
-        int victim = -1;
-        for (uint32_t i = 0; i < s->ngr; i++) {
-            srv_graph *x = &s->gr[i];
-            if (i == keep || !x->resident || x->busy || x->start_mask) continue;
-            if (victim < 0 || x->last_used < s->gr[victim].last_used) victim = (int)i;
-        }
-        if (victim < 0) return 0;
-        srv_graph *v = &s->gr[victim];
-        if (v->eng) gga_ms_free(v->eng);
-        v->eng = NULL;
+        srv_graph *vg = NULL;
+        srv_engine *v = NULL;
+        for (uint32_t i = 0; i < s->ngr; i++)
+            for (uint32_t k = 0; k < GGA_SERVER_MAX_ENGINES; k++) {
+                srv_engine *x = &s->gr[i].eng[k];
+                if (!x->resident || x->busy || x->start_mask) continue;
+                if (!v || x->last_used < v->last_used) {
+                    v = x;
+                    vg = &s->gr[i];
+                }
+            }
+        if (!v) return 0;
+        if (v->e) gga_ms_free(v->e);
+        v->e = NULL;
         v->resident = 0;
-        s->used -= v->engine_bytes;
+        s->used -= vg->engine_bytes;
         s->st.evictions++;
     }
     return 1;
 }
 
+/* The engine a query of G goes to: the fullest resident engine with a free
+ * lane, else a new one (budget and engines_per_graph permitting), else NULL. */
+static srv_engine *place(gga_server *s, srv_graph *G, uint32_t lanes_all, int *rc)
+{
+    srv_engine *best = NULL;
+    uint32_t best_free = 0;
+    for (uint32_t k = 0; k < s->o.engines_per_graph; k++) {
+        srv_engine *E = &G->eng[k];
+        if (!E->resident) continue;
+        const uint32_t nfree = (uint32_t)__builtin_popcount(lanes_all & ~(E->busy | E->start_mask));
+        if (nfree && (!best || nfree < best_free)) {
+            best = E;
+            best_free = nfree;
+        }
+    }
+    if (best) return best;
+    for (uint32_t k = 0; k < s->o.engines_per_graph; k++) {
+        srv_engine *E = &G->eng[k];
+        if (E->resident) continue;
+        if (!make_room(s, G->engine_bytes)) return NULL;
+        if (!s->o.dry_run && (*rc = gga_ms_create(G->g, 0, s->o.layout, &E->e))) return NULL;
+        E->resident = 1;
+        s->used += G->engine_bytes;
+        if (s->used > s->peak) s->peak = s->used;
+        s->st.engines_created++;
+        return E;
+    }
+    return NULL;
+}
+
 static int admit(gga_server *s, double now)
 {
-    int rc = GGA_OK;
     pthread_mutex_lock(&s->mu);
     uint64_t keep = 0;
     for (uint64_t i = 0; i < s->npend; i++) {
@@ -229,36 +271,25 @@ static int admit(gga_server *s, double now)
             continue;
         }
         const uint32_t lanes_all = s->o.lanes == 32 ? 0xFFFFFFFFu : ((1u << s->o.lanes) - 1u);
-        const uint32_t free_lanes = lanes_all & ~(G->busy | G->start_mask);
-        if (!free_lanes) {
+        int crc = GGA_OK;
+        srv_engine *E = place(s, G, lanes_all, &crc);
+        if (crc) {
+            finish_query(s, qid, crc, now);
+            continue;
+        }
+        if (!E) {
             s->pend[keep++] = qid;
             continue;
         }
-        if (!G->resident) {
-            if (!make_room(s, G->engine_bytes, x->q.graph)) {
-                s->pend[keep++] = qid;
-                continue;
-            }
-            if (!s->o.dry_run) {
-                rc = gga_ms_create(G->g, 0, s->o.layout, &G->eng);
-                if (rc) {
-                    finish_query(s, qid, rc, now);
-                    continue;
-                }
-            }
-            G->resident = 1;
-            s->used += G->engine_bytes;
-            if (s->used > s->peak) s->peak = s->used;
-            s->st.engines_created++;
-        }
-        const uint32_t lane = (uint32_t)__builtin_ctz(free_lanes);
-        G->start_mask |= 1u << lane;
-        G->start_src[lane] = x->q.src;
-        if (x->q.bfs) G->start_unit |= 1u << lane;
-        G->lane_q[lane] = qid;
-        G->dry_left[lane] = s->o.dry_steps;
+        const uint32_t lane = (uint32_t)__builtin_ctz(lanes_all & ~(E->busy | E->start_mask));
+        E->start_mask |= 1u << lane;
+        E->start_src[lane] = x->q.src;
+        if (x->q.bfs) E->start_unit |= 1u << lane;
+        E->lane_q[lane] = qid;
+        E->dry_left[lane] = s->o.dry_steps;
         x->r.admitted = now;
         x->r.lane = lane;
+        x->r.engine = (uint32_t)(E - G->eng);
         x->r.status = 2;   /* running */
         s->running++;
     }
@@ -267,14 +298,14 @@ static int admit(gga_server *s, double now)
     return GGA_OK;
 }
 
-/* Results of the converged lanes of graph G. */
-static int deliver(gga_server *s, srv_graph *G, uint32_t lanes)
+/* Results of the converged lanes of engine G. */
+static int deliver(gga_server *s, srv_engine *G, uint32_t lanes)
 {
     int rc = GGA_OK;
     uint64_t sums[GGA_MS_LANES];
     memset(sums, 0, sizeof sums);
     if (!s->o.dry_run) {
-        if ((rc = gga_ms_checksum(G->eng, lanes, sums))) return rc;
+        if ((rc = gga_ms_checksum(G->e, lanes, sums))) return rc;
         uint32_t *hp[GGA_MS_LANES], *dp[GGA_MS_LANES];
         uint32_t hmask = 0, dmask = 0;
         memset(hp, 0, sizeof hp);
@@ -291,8 +322,8 @@ static int deliver(gga_server *s, srv_graph *G, uint32_t lanes)
                 hmask |= 1u << l;
             }
         }
-        if (dmask && (rc = gga_ms_fetch(G->eng, dmask, dp, 1))) return rc;
-        if (hmask && (rc = gga_ms_fetch(G->eng, hmask, hp, 0))) return rc;
+        if (dmask && (rc = gga_ms_fetch(G->e, dmask, dp, 1))) return rc;
+        if (hmask && (rc = gga_ms_fetch(G->e, hmask, hp, 0))) return rc;
     }
     const double now = gga_server_now(s);
     pthread_mutex_lock(&s->mu);
@@ -325,11 +356,12 @@ int gga_server_pump(gga_server *s, uint32_t *active)
      * steps are launched, while the others keep running. */
     uint32_t act = 0;
     int progressed = 0, inflight = 0;
-    for (uint32_t i = 0; i < s->ngr; i++) {
-        srv_graph *G = &s->gr[i];
+    for (uint32_t i = 0; i < s->ngr * GGA_SERVER_MAX_ENGINES; i++) {
+        srv_engine *G = &s->gr[i / GGA_SERVER_MAX_ENGINES].eng[i % GGA_SERVER_MAX_ENGINES];
+        if (!G->resident) continue;
         if (G->launched) {
             if (!s->o.dry_run) {
-                const int r = gga_ms_ready(G->eng);
+                const int r = gga_ms_ready(G->e);
                 if (r < 0) return r;
                 if (r == 0) {
                     inflight = 1;
@@ -339,7 +371,7 @@ int gga_server_pump(gga_server *s, uint32_t *active)
             }
             uint32_t live = 0;
             if (!s->o.dry_run) {
-                if ((rc = gga_ms_collect(G->eng, &live))) return rc;
+                if ((rc = gga_ms_collect(G->e, &live))) return rc;
             } else {
                 for (uint32_t l = 0; l < GGA_MS_LANES; l++)
                     if (((G->busy >> l) & 1u) && G->dry_left[l] > 1) {
@@ -356,14 +388,14 @@ int gga_server_pump(gga_server *s, uint32_t *active)
             }
         }
         if (G->start_mask) {
-            if (!s->o.dry_run && (rc = gga_ms_start(G->eng, G->start_mask, G->start_src, G->start_unit)))
+            if (!s->o.dry_run && (rc = gga_ms_start(G->e, G->start_mask, G->start_src, G->start_unit)))
                 return rc;
             G->busy |= G->start_mask;
             G->start_mask = 0;
             G->start_unit = 0;
         }
         if (G->busy) {
-            if (!s->o.dry_run && (rc = gga_ms_launch(G->eng, s->o.steps_per_round, &o))) return rc;
+            if (!s->o.dry_run && (rc = gga_ms_launch(G->e, s->o.steps_per_round, &o))) return rc;
             G->launched = 1;
             G->last_used = now;
             progressed = 1;
@@ -516,7 +548,8 @@ int gga_server_stats(const gga_server *s, gga_server_counters *st)
     st->bytes_in_use = s->used;
     st->peak_bytes = s->peak;
     uint32_t r = 0;
-    for (uint32_t i = 0; i < s->ngr; i++) r += s->gr[i].resident ? 1u : 0u;
+    for (uint32_t i = 0; i < s->ngr; i++)
+        for (uint32_t k = 0; k < GGA_SERVER_MAX_ENGINES; k++) r += s->gr[i].eng[k].resident ? 1u : 0u;
     st->engines_resident = r;
     return GGA_OK;
 }

@@ -431,7 +431,11 @@ typedef struct gga_server_opts {
                                  admission and batching on hosts without a GPU) */
     uint32_t dry_steps;       /* 0 -> 3 */
     uint32_t layout;          /* GGA_MS_* layout of every engine (0 -> per graph, GGA_MS_AUTO) */
+    uint32_t engines_per_graph; /* engines one graph may hold at once, 1..GGA_SERVER_MAX_ENGINES
+                                   (0 -> 2): a second engine takes a graph's queries while the
+                                   first one's lanes are all busy */
 } gga_server_opts;
+#define GGA_SERVER_MAX_ENGINES 4
 typedef struct gga_query {
     uint32_t graph;           /* id from gga_server_add_graph */
     uint32_t src;
@@ -444,7 +448,7 @@ typedef struct gga_query_result {
     uint64_t id;
     uint32_t graph, src, bfs, lane;
     int32_t  status;          /* GGA_OK; GGA_EBUDGET if its engine exceeds the whole budget */
-    uint32_t pad;
+    uint32_t engine;          /* which of its graph's engines ran it */
     double   arrival, admitted, done;   /* seconds (replays: from the replay's start) */
     uint64_t checksum;        /* gga_checksum of the distances */
 } gga_query_result;

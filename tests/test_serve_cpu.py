@@ -59,7 +59,7 @@ def test_freed_lanes_are_refilled_continuously():
     """4 lanes, 10 queries at once on one graph, 3 collected step batches
     each: admission in waves of 4, each wave admitted in the round that frees
     the lanes, FIFO order."""
-    with Server(lanes=4, dry_run=True, dry_steps=3) as s:
+    with Server(lanes=4, dry_run=True, dry_steps=3, engines_per_graph=1) as s:
         g = s.add_graph_dry(100, 100)
         for i in range(10):
             s.submit(g, i, i % 2 == 0)
@@ -84,7 +84,7 @@ def test_freed_lanes_are_refilled_continuously():
 def test_no_head_of_line_blocking_across_graphs():
     """Graph A's engine is full; a later query of graph B is admitted in the
     same round instead of waiting behind A's queue."""
-    with Server(lanes=1, dry_run=True, dry_steps=5) as s:
+    with Server(lanes=1, dry_run=True, dry_steps=5, engines_per_graph=1) as s:
         a = s.add_graph_dry(100, 100)
         b = s.add_graph_dry(100, 100)
         s.submit(a, 1, True)
@@ -99,6 +99,25 @@ def test_no_head_of_line_blocking_across_graphs():
     done = {i: r["done"][k] for i, k in by_id.items()}
     assert adm[1] >= done[0]          # A's second query waited for A's lane
     assert adm[2] == adm[0]           # B's query was admitted in the first round
+
+
+def test_second_engine_takes_a_full_engines_queries():
+    """Graph A's only lane is busy: its next query starts at once on a second
+    engine of A; a third waits (two engines per graph).  With the budget for
+    one engine, the second query waits for the first lane instead."""
+    with Server(lanes=1, dry_run=True, dry_steps=4, engines_per_graph=2) as s:
+        a = s.add_graph_dry(100, 100)
+        res = s.replay(trace_of([(a, 1, 1, 0.0), (a, 2, 0, 0.0), (a, 3, 1, 0.0)]))
+        st = s.stats()
+    assert (res["status"] == 0).all() and st["engines_created"] == 2
+    assert res["admitted"][1] == res["admitted"][0] and res["engine"][1] != res["engine"][0]
+    assert res["admitted"][2] >= min(res["done"][0], res["done"][1])
+    with Server(hbm_budget=engine_bytes(100, 100), lanes=1, dry_run=True, dry_steps=4,
+                engines_per_graph=2) as s:
+        a = s.add_graph_dry(100, 100)
+        res = s.replay(trace_of([(a, 1, 1, 0.0), (a, 2, 0, 0.0)]))
+        st = s.stats()
+    assert st["engines_created"] == 1 and res["admitted"][1] >= res["done"][0]
 
 
 def test_queries_wait_for_busy_engines_then_run():
