@@ -90,6 +90,7 @@ typedef struct gga_dev_ms {
     uint32_t *q1;
     uint32_t *ctl;       /* GGA_CTL_WORDS words, same layout as gga_dev_work.ctl */
     uint32_t *tiles;
+    uint64_t *tstate;    /* one-pass compaction: per-tile look-back states (GGA_TILES(n)) */
     uint64_t *cnt;       /* GGA_CNT_SLOTS x 4 work counters */
     uint32_t inf;
     uint32_t qcap;

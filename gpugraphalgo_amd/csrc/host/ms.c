@@ -94,6 +94,7 @@ int gga_ms_free(gga_ms *e)
     ms_release(e->d.q1);
     ms_release(e->d.ctl);
     ms_release(e->d.tiles);
+    ms_release(e->d.tstate);
     ms_release(e->d.cnt);
     ms_release(e->d_stage);
     if (e->h_pin) hipHostFree(e->h_pin);
@@ -149,10 +150,12 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
     if ((rc = ms_alloc(e, (void **)&e->d.q1, 4ull * dg->C))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.ctl, 4ull * GGA_CTL_WORDS))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.tiles, 4ull * (e->ntiles + 1ull)))) goto fail;
+    if ((rc = ms_alloc(e, (void **)&e->d.tstate, 8ull * e->ntiles))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.cnt, 32ull * GGA_CNT_SLOTS))) goto fail;
     K_TRY(ggak_fill_u32(e->d.D, e->d.inf, (uint64_t)GGA_MS_LANES * e->n, e->v.stream));
     K_TRY(ggak_fill_u32(e->d.M0, 0u, e->n, e->v.stream));
     K_TRY(ggak_fill_u32(e->d.M1, 0u, e->n, e->v.stream));
+    K_TRY(ggak_fill_u32((uint32_t *)e->d.tstate, 0u, 2ull * e->ntiles, e->v.stream));
     K_TRY(ggak_ms_clear(e->d, e->v.stream));
     HIP_TRY(hipStreamSynchronize(e->v.stream));
     if ((rc = gga_ms_set_delta(e, gga_graph_auto_delta(g, 0)))) goto fail;
@@ -424,7 +427,7 @@ fail:
 uint64_t gga_ms_bytes(uint32_t n, uint32_t chunks)
 {
     return 4ull * GGA_MS_LANES * n + 8ull * n + 8ull * chunks + 4ull * (GGA_TILES((uint64_t)n) + 1) +
-           4ull * GGA_CTL_WORDS + 32ull * GGA_CNT_SLOTS + 8ull * GGA_MS_LANES;
+           8ull * GGA_TILES((uint64_t)n) + 4ull * GGA_CTL_WORDS + 32ull * GGA_CNT_SLOTS + 8ull * GGA_MS_LANES;
 }
 
 uint64_t gga_checksum(const uint32_t *dist, uint32_t n)
