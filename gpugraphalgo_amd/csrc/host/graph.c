@@ -769,7 +769,7 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
             }
             if (c[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
                 rc = GGA_ETIMEDOUT;
-                gga_set_error("persistent kernel: grid barrier spin limit");
+                gga_set_error("spin limit of a device-side wait (grid barrier or compaction look-back)");
                 goto fail;
             }
             if (mode == GGA_MODE_PERSISTENT) {

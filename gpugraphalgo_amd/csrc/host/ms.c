@@ -304,7 +304,7 @@ static int ms_readback(gga_ms *e)
         return GGA_ERANGE;
     }
     if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
-        gga_set_error("multi-source persistent kernel: grid barrier spin limit");
+        gga_set_error("multi-source engine: spin limit of a device-side wait (grid barrier or look-back)");
         return GGA_ETIMEDOUT;
     }
     return GGA_OK;
@@ -378,6 +378,10 @@ int gga_ms_collect(gga_ms *e, uint32_t *live_out)
     if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_QFULL) {
         gga_set_error("multi-source list overflow (capacity %u)", e->d.qcap);
         return GGA_ERANGE;
+    }
+    if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
+        gga_set_error("multi-source engine: spin limit of a device-side wait (look-back)");
+        return GGA_ETIMEDOUT;
     }
     e->steps_seen = e->h_pin[GGA_CTL_STEPS];
     ms_slot(e, &live, &tail);
