@@ -663,8 +663,13 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
             gga_set_error("occupancy query failed");
             goto fail;
         }
-        uint32_t per = o.sync_every ? o.sync_every : 8u;
-        if ((rc = ensure_kev(g, 3 * per))) goto fail;
+        /* steps per host check: sync_every, else 8 -- but the first batch is
+         * as long as the last query on this graph needed (up to 32), so that
+         * repeated queries end after one check and few empty launches */
+        const uint32_t per = o.sync_every ? o.sync_every : 8u;
+        const uint32_t prev = g->last_steps[bfs ? 1 : 0];
+        uint32_t nb = o.sync_every ? per : (prev ? (prev < 32u ? prev : 32u) : per);
+        if ((rc = ensure_kev(g, 3 * (nb > per ? nb : per)))) goto fail;
         /* instrumentation buffers (freed below, also on failure) */
         const uint32_t tl_steps = o.timeline && o.timeline_cap >= grid ? o.timeline_cap / grid : 0u;
         const uint32_t conv_n = o.conv_final && o.conv_rows ? o.conv_cap : 0u;
@@ -682,7 +687,7 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
         for (;;) {
             /* `per` steps back to back: every kernel reads its list length on
              * the device, so steps after convergence are empty no-ops */
-            for (uint32_t k = 0; k < per; k++) {
+            for (uint32_t k = 0; k < nb; k++) {
                 const uint32_t abs_step = step + k;
                 gga_wg_record *tl = abs_step < tl_steps ? d_tl + (uint64_t)abs_step * grid : NULL;
                 HIP_TRY(hipEventRecord(g->kev[3 * k], g->stream));
@@ -693,19 +698,20 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
                 HIP_TRY(hipEventRecord(g->kev[3 * k + 2], g->stream));
                 if (abs_step < conv_n) K_TRY(ggak_conv(g->d_dist, d_fin, g->n, d_conv + 2ull * abs_step, g->stream));
             }
-            step += per;
+            step += nb;
             HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_TAIL(step % 3u), 4, hipMemcpyDeviceToHost,
                                    g->stream));
             HIP_TRY(hipMemcpyAsync(g->h_pin + 1, g->d_ctl + GGA_CTL_FAR, 4, hipMemcpyDeviceToHost, g->stream));
             HIP_TRY(hipStreamSynchronize(g->stream));
-            for (uint32_t k = 0; k < per; k++) {
+            for (uint32_t k = 0; k < nb; k++) {
                 float t = 0.f, tc = 0.f;
                 HIP_TRY(hipEventElapsedTime(&t, g->kev[3 * k], g->kev[3 * k + 1]));
                 HIP_TRY(hipEventElapsedTime(&tc, g->kev[3 * k + 1], g->kev[3 * k + 2]));
                 ktime += t;
                 s.ctime_ms += tc;
             }
-            s.launches += per;
+            s.launches += nb;
+            nb = per;
             if (g->h_pin[0] == 0 && (!dw.last || g->h_pin[1] == 0)) break;
             if (step >= max_iters) {
                 rc = GGA_ETIMEDOUT;

@@ -180,8 +180,11 @@ typedef struct gga_opts {
     uint32_t inf;         /* unreached value: GGA_INF (0 -> default) or GGA_REF_INF */
     uint32_t max_iters;   /* 0 -> n + 1 (the serial bound, graph.h:122) */
     uint32_t block;       /* workgroup size, 0 -> default (256) */
-    uint32_t sync_every;  /* WORKLIST: steps launched per host convergence check (0 -> 8) */
-    uint32_t use_graph;   /* WORKLIST: replay the sync_every steps as one captured hipGraph */
+    uint32_t sync_every;  /* WORKLIST: steps launched per host convergence check (0 -> 8, the
+                             first check after as many steps as the graph's last query needed,
+                             at most 32) */
+    uint32_t use_graph;   /* multi-source engines: replay steps from captured hipGraphs (six
+                             steps per graph) */
     uint32_t count;       /* 1 -> collect edges_inspected / chunks_expanded / relaxations */
     uint32_t grid;        /* PERSISTENT / WORKLIST grid in workgroups, 0 -> occupancy bound */
     uint32_t dense_min;   /* WORKLIST / PERSISTENT: a step whose list holds >= dense_min chunks
@@ -320,9 +323,9 @@ uint64_t gga_ms_graph_bytes(gga_graph *g);
  * (BFS levels), the others add edge weights.  A lane still running is EINVAL. */
 int gga_ms_start(gga_ms *e, uint32_t lanes, const uint32_t *src, uint32_t unit_lanes);
 /* Runs up to max_steps steps (0: until every lane converged).  mode:
- * GGA_MODE_WORKLIST (four kernels per step, `sync_every` steps per host check,
- * replayed from a captured hipGraph when use_graph) or GGA_MODE_PERSISTENT (one
- * cooperative launch).  o may be NULL.  *live = lanes with work left; a lane
+ * GGA_MODE_WORKLIST (three kernels per step, three more with near-far lanes,
+ * `sync_every` steps per host check, replayed from captured hipGraphs when
+ * use_graph) or GGA_MODE_PERSISTENT (one cooperative launch).  o may be NULL.  *live = lanes with work left; a lane
  * started and not live has finished. */
 int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live);
 /* Asynchronous form of a run of nsteps steps, for callers that drive several
