@@ -64,6 +64,30 @@ def test_served_results_equal_single_source(gpu, use_graph):
         dg.close()
 
 
+def test_second_engines_match_single_source(gpu):
+    """Four lanes per engine and two engines per graph: queries spill to a
+    graph's second engine (its own stream) while the first is full; every
+    result still equals the single-source one."""
+    gs = graphs_small()
+    dgs = [gga.DeviceGraph(g) for g in gs]
+    for dg in dgs:
+        dg.split(mda=16)
+    trace = poisson_trace([g.n for g in gs], 90, rate=0, seed=12, degrees=sources_with_edges(gs))
+    with Server(lanes=4, engines_per_graph=2, use_graph=True, steps_per_round=6) as srv:
+        for dg in dgs:
+            srv.add_graph(dg)
+        announce("serve 90 queries, 4 lanes x 2 engines per graph")
+        res = srv.replay(trace)
+        st = srv.stats()
+    assert (res["status"] == 0).all()
+    assert (res["engine"] == 1).any() and st["engines_created"] > len(gs)
+    for i, r in enumerate(trace):
+        want = want_for(gs[int(r["graph"])], r["src"], bool(r["bfs"]))
+        assert int(res["checksum"][i]) == checksum(want), f"query {i} engine {res['engine'][i]}"
+    for dg in dgs:
+        dg.close()
+
+
 class DeviceBuffer:
     """n uint32 of device memory from the HIP runtime the library links."""
 
