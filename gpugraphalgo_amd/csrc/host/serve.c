@@ -109,7 +109,7 @@ int gga_server_create(const gga_server_opts *o, gga_server **out)
     if (!s->o.steps_per_round) s->o.steps_per_round = 6;
     if (!s->o.lanes || s->o.lanes > GGA_MS_LANES) s->o.lanes = GGA_MS_LANES;
     if (!s->o.dry_steps) s->o.dry_steps = 3;
-    if (!s->o.engines_per_graph) s->o.engines_per_graph = 2;
+    if (!s->o.engines_per_graph) s->o.engines_per_graph = 1;
     if (s->o.engines_per_graph > GGA_SERVER_MAX_ENGINES) s->o.engines_per_graph = GGA_SERVER_MAX_ENGINES;
     clock_gettime(CLOCK_MONOTONIC, &s->t0);
     pthread_mutex_init(&s->mu, NULL);

@@ -435,8 +435,8 @@ typedef struct gga_server_opts {
     uint32_t dry_steps;       /* 0 -> 3 */
     uint32_t layout;          /* GGA_MS_* layout of every engine (0 -> per graph, GGA_MS_AUTO) */
     uint32_t engines_per_graph; /* engines one graph may hold at once, 1..GGA_SERVER_MAX_ENGINES
-                                   (0 -> 2): a second engine takes a graph's queries while the
-                                   first one's lanes are all busy */
+                                   (0 -> 1): a further engine takes a graph's queries while the
+                                   others' lanes are all busy */
 } gga_server_opts;
 #define GGA_SERVER_MAX_ENGINES 4
 typedef struct gga_query {
