@@ -42,7 +42,6 @@ typedef struct gga_dev_work {
     uint32_t *ctl;
     uint64_t *cnt;       /* GGA_CNT_SLOTS x 4 work counters, striped over many addresses */
     uint32_t *tiles;     /* list compaction: per-tile counts, then offsets (GGA_TILES(n) + 1) */
-    uint64_t *tstate;    /* one-pass compaction: per-tile look-back states (GGA_TILES(n)) */
     uint32_t *last;      /* near-far: distance of each vertex at its last expansion (NULL: off) */
     uint32_t *tmin;      /* near-far: per-tile minimum deferred distance (GGA_TILES(n)) */
     uint32_t inf;
@@ -90,7 +89,6 @@ typedef struct gga_dev_ms {
     uint32_t *q1;
     uint32_t *ctl;       /* GGA_CTL_WORDS words, same layout as gga_dev_work.ctl */
     uint32_t *tiles;
-    uint64_t *tstate;    /* one-pass compaction: per-tile look-back states (GGA_TILES(n)) */
     uint64_t *cnt;       /* GGA_CNT_SLOTS x 4 work counters */
     uint32_t inf;
     uint32_t qcap;
@@ -152,7 +150,6 @@ typedef struct gga_dev_msbfs {
 #define GGA_CTL_RUN       (4u * GGA_CTL_LINE)           /* lanes started and not converged */
 #define GGA_CTL_RLISTED   (4u * GGA_CTL_LINE + 1u)      /* lanes a refill listed vertices for */
 #define GGA_CTL_RNEED     (4u * GGA_CTL_LINE + 2u)      /* a refill listed new vertices: rebuild */
-#define GGA_CTL_CEPOCH    (5u * GGA_CTL_LINE)           /* one-pass compactions completed */
 #define GGA_CTL_ITER(s)   (6u * GGA_CTL_LINE + (s))     /* step number of a ring slot */
 #define GGA_CTL_CHANGED   (7u * GGA_CTL_LINE)           /* topology-mode `changed` flag */
 #define GGA_CTL_STEPS     (7u * GGA_CTL_LINE + 1u)      /* steps that had a non-empty list */

@@ -37,7 +37,6 @@ struct gga_graph {
     gga_split_stats sst;
     /* query workspace */
     uint32_t *d_dist, *d_mark, *d_q0, *d_q1, *d_ctl, *d_tiles;
-    uint64_t *d_tstate;
     uint64_t *d_cnt;
     uint32_t qcap;
     uint32_t *h_pin;          /* pinned: GGA_CTL_WORDS words */
@@ -141,7 +140,6 @@ static gga_dev_work dev_work(const gga_graph *g, uint32_t inf)
     w.ctl = g->d_ctl;
     w.cnt = g->d_cnt;
     w.tiles = g->d_tiles;
-    w.tstate = g->d_tstate;
     w.inf = inf;
     w.qcap = g->qcap;
     w.dense_min = 0xFFFFFFFFu;
@@ -188,7 +186,6 @@ int gga_graph_free(gga_graph *g)
     dfree(g, (void **)&g->d_ctl, 4ull * GGA_CTL_WORDS);
     dfree(g, (void **)&g->d_cnt, 32ull * GGA_CNT_SLOTS);
     dfree(g, (void **)&g->d_tiles, 4ull * (GGA_TILES((uint64_t)g->n) + 1));
-    dfree(g, (void **)&g->d_tstate, 8ull * GGA_TILES((uint64_t)g->n));
     dfree(g, (void **)&g->d_last, 4ull * g->n);
     dfree(g, (void **)&g->d_tmin, 4ull * GGA_TILES((uint64_t)g->n));
     if (g->h_pin) hipHostFree(g->h_pin);
@@ -235,7 +232,6 @@ static int graph_create(uint32_t n, uint32_t m, const uint32_t *row_ptr, const u
     if ((rc = dmalloc(g, (void **)&g->d_ctl, 4ull * GGA_CTL_WORDS))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_cnt, 32ull * GGA_CNT_SLOTS))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_tiles, 4ull * (GGA_TILES((uint64_t)n) + 1)))) goto fail;
-    if ((rc = dmalloc(g, (void **)&g->d_tstate, 8ull * GGA_TILES((uint64_t)n)))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_dist, 4ull * n))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_mark, 4ull * n))) goto fail;
     HIP_TRY(hipMemcpyAsync(g->d_row, row_ptr, 4ull * ((uint64_t)n + 1), hipMemcpyHostToDevice, g->stream));
@@ -252,7 +248,6 @@ static int graph_create(uint32_t n, uint32_t m, const uint32_t *row_ptr, const u
         if (g->wmean == 0) g->wmean = 1;
     }
     K_TRY(ggak_fill_u32(g->d_ctl, 0u, GGA_CTL_WORDS, g->stream));
-    K_TRY(ggak_fill_u32((uint32_t *)g->d_tstate, 0u, 2ull * GGA_TILES((uint64_t)n), g->stream));
     K_TRY(ggak_fill_u32(g->d_mark, 0xFFFFFFFFu, n, g->stream));
     /* max degree + first index on the device (graph.h:338-358) */
     {
@@ -775,7 +770,7 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
             }
             if (c[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
                 rc = GGA_ETIMEDOUT;
-                gga_set_error("spin limit of a device-side wait (grid barrier or compaction look-back)");
+                gga_set_error("persistent kernel: grid barrier spin limit");
                 goto fail;
             }
             if (mode == GGA_MODE_PERSISTENT) {

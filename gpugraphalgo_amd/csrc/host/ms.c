@@ -94,7 +94,6 @@ int gga_ms_free(gga_ms *e)
     ms_release(e->d.q1);
     ms_release(e->d.ctl);
     ms_release(e->d.tiles);
-    ms_release(e->d.tstate);
     ms_release(e->d.cnt);
     ms_release(e->d_stage);
     if (e->h_pin) hipHostFree(e->h_pin);
@@ -150,12 +149,10 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
     if ((rc = ms_alloc(e, (void **)&e->d.q1, 4ull * dg->C))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.ctl, 4ull * GGA_CTL_WORDS))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.tiles, 4ull * (e->ntiles + 1ull)))) goto fail;
-    if ((rc = ms_alloc(e, (void **)&e->d.tstate, 8ull * e->ntiles))) goto fail;
     if ((rc = ms_alloc(e, (void **)&e->d.cnt, 32ull * GGA_CNT_SLOTS))) goto fail;
     K_TRY(ggak_fill_u32(e->d.D, e->d.inf, (uint64_t)GGA_MS_LANES * e->n, e->v.stream));
     K_TRY(ggak_fill_u32(e->d.M0, 0u, e->n, e->v.stream));
     K_TRY(ggak_fill_u32(e->d.M1, 0u, e->n, e->v.stream));
-    K_TRY(ggak_fill_u32((uint32_t *)e->d.tstate, 0u, 2ull * e->ntiles, e->v.stream));
     K_TRY(ggak_ms_clear(e->d, e->v.stream));
     HIP_TRY(hipStreamSynchronize(e->v.stream));
     if ((rc = gga_ms_set_delta(e, gga_graph_auto_delta(g, 0)))) goto fail;
@@ -304,7 +301,7 @@ static int ms_readback(gga_ms *e)
         return GGA_ERANGE;
     }
     if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
-        gga_set_error("multi-source engine: spin limit of a device-side wait (grid barrier or look-back)");
+        gga_set_error("multi-source persistent kernel: grid barrier spin limit");
         return GGA_ETIMEDOUT;
     }
     return GGA_OK;
@@ -379,10 +376,6 @@ int gga_ms_collect(gga_ms *e, uint32_t *live_out)
         gga_set_error("multi-source list overflow (capacity %u)", e->d.qcap);
         return GGA_ERANGE;
     }
-    if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
-        gga_set_error("multi-source engine: spin limit of a device-side wait (look-back)");
-        return GGA_ETIMEDOUT;
-    }
     e->steps_seen = e->h_pin[GGA_CTL_STEPS];
     ms_slot(e, &live, &tail);
     HIP_TRY(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));
@@ -431,7 +424,7 @@ fail:
 uint64_t gga_ms_bytes(uint32_t n, uint32_t chunks)
 {
     return 4ull * GGA_MS_LANES * n + 8ull * n + 8ull * chunks + 4ull * (GGA_TILES((uint64_t)n) + 1) +
-           8ull * GGA_TILES((uint64_t)n) + 4ull * GGA_CTL_WORDS + 32ull * GGA_CNT_SLOTS + 8ull * GGA_MS_LANES;
+           4ull * GGA_CTL_WORDS + 32ull * GGA_CNT_SLOTS + 8ull * GGA_MS_LANES;
 }
 
 uint64_t gga_checksum(const uint32_t *dist, uint32_t n)

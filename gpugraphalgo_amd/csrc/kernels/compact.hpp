@@ -146,83 +146,4 @@ __device__ __forceinline__ void compact_write_tile(const ListOut &o, uint32_t n,
     write_items(o, cs, nch, o.tiles[t] + excl);
 }
 
-// ---- one pass: count, offset and write in a single launch (decoupled
-// look-back, Merrill & Garland 2016).  Workgroup t owns tile t: it counts,
-// publishes its count, sums the counts of the tiles before it by looking back
-// over their published states, publishes its inclusive total and writes.  A
-// tile waits only on tiles of lower index, which the dispatcher has started
-// before it, so every wait ends; the wait is bounded by the clock anyway.
-// Tile state words: bits 63-62 flag (0 none, 1 count, 2 inclusive total),
-// 61-32 the compaction's epoch, 31-0 the value.  The epoch (a ctl word) goes
-// up by one per compaction, so states of the previous one read as "none" and
-// the array never needs clearing.
-constexpr uint64_t TS_AGG = 1ull << 62;
-constexpr uint64_t TS_INC = 2ull << 62;
-constexpr uint64_t LOOKBACK_SPIN_TICKS = 500000000ull;   // 5 s of the 100 MHz wall clock
-
-__device__ __forceinline__ uint64_t ts_pack(uint32_t epoch, uint64_t flag, uint32_t v) {
-    return flag | ((uint64_t)(epoch & 0x3FFFFFFFu) << 32) | v;
-}
-
-// Wave 0: the sum of the values of tiles [0, t).
-__device__ __forceinline__ uint32_t lookback(const uint64_t *ts, uint32_t t, uint32_t epoch, uint32_t *ctl) {
-    const uint32_t l = lane_id();
-    const uint64_t t0 = wall_clock64();
-    uint32_t prefix = 0;
-    int64_t j = (int64_t)t - 1;
-    while (j >= 0) {
-        // lane l reads tile j - l; tiles before 0 read as an inclusive total of 0
-        const int64_t idx = j - (int64_t)l;
-        const uint64_t wd = idx >= 0 ? __hip_atomic_load(&ts[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : ts_pack(epoch, TS_INC, 0u);
-        const bool ready = (wd >> 62) != 0u && (uint32_t)((wd >> 32) & 0x3FFFFFFFu) == (epoch & 0x3FFFFFFFu);
-        const uint64_t nr = __ballot(!ready), inc = __ballot(ready && (wd >> 62) == 2u);
-        const uint32_t k_inc = inc ? (uint32_t)__builtin_ctzll(inc) : GGA_WAVE;
-        const uint32_t k_nr = nr ? (uint32_t)__builtin_ctzll(nr) : GGA_WAVE;
-        if (k_nr < k_inc) {   // a tile between us and the nearest total has not counted yet
-            if (__ballot(l == 0 && wall_clock64() - t0 > LOOKBACK_SPIN_TICKS)) {
-                if (l == 0) __hip_atomic_fetch_or(&ctl[GGA_CTL_ERR], GGA_DEVERR_SPIN, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        prefix += wave_sum(l <= k_inc ? (uint32_t)wd : 0u);
-        if (k_inc < GGA_WAVE) break;
-        j -= GGA_WAVE;
-    }
-    return prefix;
-}
-
-// Launch with exactly GGA_TILES(n) workgroups of CBLOCK threads, all taking
-// part (the caller returns from every workgroup or from none).
-template <class Marked>
-__device__ __forceinline__ void compact_onepass(const ListOut &o, uint32_t n, const Marked &marked, uint64_t *ts,
-                                                uint32_t *epoch_word, uint32_t *s_wave, uint32_t *s_pre) {
-    const uint32_t t = blockIdx.x, ntiles = GGA_TILES(n);
-    const uint32_t epoch = load_relaxed(epoch_word);
-    uint32_t cs[CITEMS], nch[CITEMS], excl;
-    const uint32_t tot = block_excl_scan(tile_items(o, n, t, marked, cs, nch), excl, s_wave);
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&ts[t], ts_pack(epoch, t == 0 ? TS_INC : TS_AGG, tot), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    if (threadIdx.x < GGA_WAVE) {
-        const uint32_t p = t ? lookback(ts, t, epoch, o.ctl) : 0u;
-        if (threadIdx.x == 0) {
-            s_pre[0] = p;
-            if (t) __hip_atomic_store(&ts[t], ts_pack(epoch, TS_INC, p + tot), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-            // the last tile has seen every tile publish, so every workgroup
-            // has read the epoch: the next compaction may use the next one
-            if (t == ntiles - 1u) {
-                *o.tail = p + tot;
-                __hip_atomic_store(epoch_word, epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-    }
-    __syncthreads();
-    write_items(o, cs, nch, s_pre[0] + excl);
-}
-
 }  // namespace gga

@@ -435,15 +435,6 @@ __global__ __launch_bounds__(CBLOCK) void k_ms_write(gga_dev_graph g, gga_dev_ms
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
 }
 
-// The three passes in one launch of GGA_TILES(n) workgroups (look-back offsets).
-__global__ __launch_bounds__(CBLOCK) void k_ms_compact_1p(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE], s_pre[1];
-    const uint32_t nq = ms_len(ms, step);
-    if (nq == 0 || !ms_flag_step(ms, nq)) return;
-    compact_onepass(ms_list_out(g, ms, step), g.n, MaskMarked{ms_mnxt(ms, step)}, ms.tstate,
-                    &ms.ctl[GGA_CTL_CEPOCH], s_wave, s_pre);
-}
-
 __global__ __launch_bounds__(BLOCK) void k_ms_finish(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_or[WAVES];
     ms_finish(g, ms, step, s_or);
@@ -557,13 +548,6 @@ __global__ __launch_bounds__(CBLOCK) void k_ms_rwrite(gga_dev_graph g, gga_dev_m
     const ListOut o = ms_list_out(g, ms, step);
     const MaskMarked mk{ms_mnxt(ms, step)};
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
-}
-
-__global__ __launch_bounds__(CBLOCK) void k_ms_rebuild_1p(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE], s_pre[1];
-    if (!ms.far || !ms.ctl[GGA_CTL_RNEED]) return;
-    compact_onepass(ms_list_out(g, ms, step), g.n, MaskMarked{ms_mnxt(ms, step)}, ms.tstate,
-                    &ms.ctl[GGA_CTL_CEPOCH], s_wave, s_pre);
 }
 
 // All steps in one cooperative launch: relax, then the three compaction
@@ -837,25 +821,17 @@ extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int c
     void *args[] = {&g, &ms, &step};
     hipError_t e = hipLaunchKernel(pick(0, count, ms.lane_major), dim3(grid), dim3(BLOCK), args, 0, s);
     if (e != hipSuccess) return (int)e;
-    const uint32_t cg = compact_grid(g.n), ntiles = GGA_TILES(g.n);
-    if (ms.tstate) {
-        k_ms_compact_1p<<<ntiles, CBLOCK, 0, s>>>(g, ms, step);
-    } else {
-        k_ms_count<<<cg, CBLOCK, 0, s>>>(g, ms, step);
-        k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
-        k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
-    }
+    const uint32_t cg = compact_grid(g.n);
+    k_ms_count<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+    k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
+    k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     k_ms_finish<<<grid_for(g.C, 1024), BLOCK, 0, s>>>(g, ms, step);
     if (ms.far) {
         k_ms_refill<<<RBLOCKS, BLOCK, 0, s>>>(g, ms, step);
         k_ms_refill_reduce<<<1, BLOCK, 0, s>>>(g, ms, step);
-        if (ms.tstate) {
-            k_ms_rebuild_1p<<<ntiles, CBLOCK, 0, s>>>(g, ms, step);
-        } else {
-            k_ms_rcount<<<cg, CBLOCK, 0, s>>>(g, ms, step);
-            k_ms_rscan<<<1, CBLOCK, 0, s>>>(g, ms, step);
-            k_ms_rwrite<<<cg, CBLOCK, 0, s>>>(g, ms, step);
-        }
+        k_ms_rcount<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+        k_ms_rscan<<<1, CBLOCK, 0, s>>>(g, ms, step);
+        k_ms_rwrite<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     }
     return launch_status();
 }

@@ -369,16 +369,6 @@ __global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_de
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
 }
 
-// The same in one launch of GGA_TILES(n) workgroups (look-back offsets).
-__global__ __launch_bounds__(BLOCK) void k_compact_1p(gga_dev_graph g, gga_dev_work w, uint32_t step) {
-    __shared__ uint32_t s_wave[WAVES], s_pre[1];
-    const uint32_t nq = step_len(w, step);
-    if (nq == 0 || !flag_step(w, nq)) return;
-    const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
-    compact_onepass(list_out(g, w, step), g.n, StampMarked{w.mark, it}, w.tstate, &w.ctl[GGA_CTL_CEPOCH], s_wave,
-                    s_pre);
-}
-
 // ---- near-far refill: after a step whose next list came out empty, list
 // the deferred vertices (dist < last) under the next threshold.
 struct DeferredUnder {
@@ -801,10 +791,6 @@ extern "C" int ggak_refill(gga_dev_graph g, gga_dev_work w, uint32_t step, hipSt
 
 extern "C" int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t step, hipStream_t s) {
     const uint32_t ntiles = GGA_TILES(g.n);
-    if (w.tstate && ntiles) {
-        k_compact_1p<<<ntiles, BLOCK, 0, s>>>(g, w, step);
-        return launch_status();
-    }
     const uint32_t grid = ntiles < 2048u ? (ntiles ? ntiles : 1u) : 2048u;
     k_compact_count<<<grid, BLOCK, 0, s>>>(g, w, step);
     k_compact_scan<<<1, BLOCK, 0, s>>>(g, w, step);

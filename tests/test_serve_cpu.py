@@ -26,8 +26,7 @@ def trace_of(rows):
 def test_engine_bytes_formula():
     n, c = 1000, 1500
     ntiles = (n + 2047) // 2048
-    assert engine_bytes(n, c) == (128 * n + 8 * n + 8 * c + 4 * (ntiles + 1) + 8 * ntiles + 4 * 44 * 32
-                                  + 32 * 1024 + 8 * 32)
+    assert engine_bytes(n, c) == 128 * n + 8 * n + 8 * c + 4 * (ntiles + 1) + 4 * 44 * 32 + 32 * 1024 + 8 * 32
 
 
 def test_budget_never_exceeded_and_all_complete():
