@@ -36,14 +36,14 @@ static void spread_sources(uint32_t n, uint32_t src, uint32_t *srcs)
     for (uint32_t i = 0; i < GGA_MS_LANES; i++) srcs[i] = (uint32_t)((src + (uint64_t)i * (n / GGA_MS_LANES + 1)) % n);
 }
 
-/* Best of two runs (device time, ms) of the tuning objective: one query from
- * src, or with GGA_TUNE_MULTI one 32-lane multi-source run. */
+/* Best of three runs (device time, ms) of the tuning objective: one query
+ * from src, or with GGA_TUNE_MULTI one 32-lane multi-source run. */
 static int time_query(gga_graph *g, uint32_t src, int bfs, uint32_t flags, const gga_opts *o, float *ms)
 {
     float best = 0.f;
     uint32_t srcs[GGA_MS_LANES];
     if (flags & GGA_TUNE_MULTI) spread_sources(gga_graph_n(g), src, srcs);
-    for (int rep = 0; rep < 2; rep++) {
+    for (int rep = 0; rep < 3; rep++) {
         gga_stats st;
         int rc;
         if (flags & GGA_TUNE_MULTI)
