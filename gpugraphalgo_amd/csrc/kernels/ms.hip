@@ -400,8 +400,6 @@ __global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, 
 template <bool COUNT, bool LM>
 __global__ __launch_bounds__(BLOCK) void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
-    // the previous step's refill rebuild has run (stream order): re-arm it
-    if (blockIdx.x == 0 && threadIdx.x == 0) ms.ctl[GGA_CTL_RNEED] = 0u;
     const uint32_t nq = ms_len(ms, step);
     if (nq == 0) return;
     Counters cnt;
@@ -443,54 +441,72 @@ __global__ __launch_bounds__(BLOCK) void k_ms_finish(gga_dev_graph g, gga_dev_ms
 // ---- near-far refill, after the finish pass of step `step`: lanes still
 // running but without listed work (candidates) list their deferred
 // (vertex, lane) pairs -- the far bits -- whose distance is under threshold +
-// delta, and clear those bits.  Per-lane minima of the deferred distances
-// beyond it go to lpart (one row of 32 per workgroup) for the reduction.  The
-// scan reads one far word per vertex and D only where a candidate's bit is set.
+// delta, and clear those bits; a vertex whose next mask was empty is appended
+// to the next list (wave-staged pushes, as in a push step).  Per-lane minima
+// of the deferred distances beyond it go to lpart (one row of 32 per
+// workgroup) for the reduction.  The scan reads one far word per vertex and D
+// only where a candidate's bit is set.
 constexpr uint32_t RBLOCKS = GGA_MS_RBLOCKS;
 __global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_t1[LANES], s_min[LANES], s_listed, s_need;
+    __shared__ uint32_t s_t1[LANES], s_min[LANES], s_listed;
+    __shared__ uint32_t s_pbuf[WAVES][PBUF];
     if (!ms.far) return;
     const uint32_t cand = ms.ctl[GGA_CTL_RUN] & ~ms.ctl[GGA_CTL_LANES((step + 1u) % 3u)];
     if (!cand) return;
     uint32_t *mnxt = ms_mnxt(ms, step);
+    uint32_t *qout = (step & 1u) ? ms.q0 : ms.q1;
+    uint32_t *tail = &ms.ctl[GGA_CTL_TAIL((step + 1u) % 3u)];
+    uint32_t *pbuf = s_pbuf[threadIdx.x / GGA_WAVE];
     if (threadIdx.x < LANES) {
         s_t1[threadIdx.x] = sat_add(ms.ctl[GGA_CTL_LTHR(threadIdx.x)], ms.delta);
         s_min[threadIdx.x] = 0xFFFFFFFFu;
     }
-    if (threadIdx.x == 0) {
-        s_listed = 0;
-        s_need = 0;
-    }
+    if (threadIdx.x == 0) s_listed = 0;
     __syncthreads();
-    uint32_t listed = 0, need = 0;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t fw = ms.far[v];
-        uint32_t m = fw & cand, bits = 0;
-        while (m) {
-            const uint32_t s = (uint32_t)__builtin_ctz(m);
-            m &= m - 1u;
-            const uint32_t d = ms.D[didx(ms, g.n, (uint32_t)v, s)];
-            if (d < s_t1[s]) bits |= 1u << s;
-            else atomicMin(&s_min[s], d);
+    uint32_t listed = 0, pcount = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // wave-uniform loop (the pushes below use whole-wave operations)
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(GGA_WAVE - 1u)); base < g.n;
+         base += stride) {
+        const uint64_t v = base + lane_id();
+        bool first = false;
+        if (v < g.n) {
+            const uint32_t fw = ms.far[v];
+            uint32_t m = fw & cand, bits = 0;
+            while (m) {
+                const uint32_t s = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                const uint32_t d = ms.D[didx(ms, g.n, (uint32_t)v, s)];
+                if (d < s_t1[s]) bits |= 1u << s;
+                else atomicMin(&s_min[s], d);
+            }
+            if (bits) {
+                ms.far[v] = fw & ~bits;
+                listed |= bits;
+                first = __hip_atomic_fetch_or(&mnxt[v], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+            }
         }
-        if (bits) {
-            ms.far[v] = fw & ~bits;
-            listed |= bits;
-            need |= __hip_atomic_fetch_or(&mnxt[v], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+        const uint64_t pm = __ballot(first);
+        if (pm) {
+            const uint32_t np = (uint32_t)__popcll(pm);
+            if (pcount + np > PBUF) {
+                flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
+                pcount = 0;
+            }
+            if (first)
+                pbuf[pcount + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] =
+                    (uint32_t)v;
+            pcount += np;
+            __builtin_amdgcn_wave_barrier();
         }
     }
+    if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
     listed = wave_or(listed);
-    need = wave_or(need);
-    if (lane_id() == 0) {
-        if (listed) atomicOr(&s_listed, listed);
-        if (need) s_need = 1u;
-    }
+    if (lane_id() == 0 && listed) atomicOr(&s_listed, listed);
     __syncthreads();
     if (threadIdx.x < LANES) ms.lpart[(uint64_t)blockIdx.x * LANES + threadIdx.x] = s_min[threadIdx.x];
-    if (threadIdx.x == 0) {
-        if (s_listed) __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_RLISTED], s_listed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (s_need) __hip_atomic_store(&ms.ctl[GGA_CTL_RNEED], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0 && s_listed)
+        __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_RLISTED], s_listed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One workgroup: per candidate lane, a listing refill raises its threshold by
@@ -524,30 +540,6 @@ __global__ __launch_bounds__(BLOCK) void k_ms_refill_reduce(gga_dev_graph g, gga
         ms.ctl[GGA_CTL_RLISTED] = 0u;
     }
     (void)g;
-}
-
-// After a refill listed vertices that were not listed yet: rebuild the next
-// list from all non-zero next masks (the same three passes as a flag step).
-__global__ __launch_bounds__(CBLOCK) void k_ms_rcount(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
-    if (!ms.far || !ms.ctl[GGA_CTL_RNEED]) return;
-    const ListOut o = ms_list_out(g, ms, step);
-    const MaskMarked mk{ms_mnxt(ms, step)};
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, mk, s_wave);
-}
-
-__global__ __launch_bounds__(CBLOCK) void k_ms_rscan(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
-    if (!ms.far || !ms.ctl[GGA_CTL_RNEED]) return;
-    compact_scan(ms_list_out(g, ms, step), g.n, s_wave);
-}
-
-__global__ __launch_bounds__(CBLOCK) void k_ms_rwrite(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
-    if (!ms.far || !ms.ctl[GGA_CTL_RNEED]) return;
-    const ListOut o = ms_list_out(g, ms, step);
-    const MaskMarked mk{ms_mnxt(ms, step)};
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
 }
 
 // All steps in one cooperative launch: relax, then the three compaction
@@ -822,16 +814,15 @@ extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int c
     hipError_t e = hipLaunchKernel(pick(0, count, ms.lane_major), dim3(grid), dim3(BLOCK), args, 0, s);
     if (e != hipSuccess) return (int)e;
     const uint32_t cg = compact_grid(g.n);
-    k_ms_count<<<cg, CBLOCK, 0, s>>>(g, ms, step);
-    k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
-    k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+    if (ms.dense_min != 0xFFFFFFFFu) {   // push-only engines never rebuild from the masks
+        k_ms_count<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+        k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
+        k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
+    }
     k_ms_finish<<<grid_for(g.C, 1024), BLOCK, 0, s>>>(g, ms, step);
     if (ms.far) {
         k_ms_refill<<<RBLOCKS, BLOCK, 0, s>>>(g, ms, step);
         k_ms_refill_reduce<<<1, BLOCK, 0, s>>>(g, ms, step);
-        k_ms_rcount<<<cg, CBLOCK, 0, s>>>(g, ms, step);
-        k_ms_rscan<<<1, CBLOCK, 0, s>>>(g, ms, step);
-        k_ms_rwrite<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     }
     return launch_status();
 }
