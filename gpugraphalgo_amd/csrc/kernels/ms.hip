@@ -319,15 +319,26 @@ __device__ __forceinline__ void ms_finish(const gga_dev_graph &g, const gga_dev_
     // mask and (near-far) the far bits of the lanes it expanded.  No expanded
     // lane can have been deferred at the owner during the step: its distance
     // was under the threshold, so every drop during the step was too.
-    for (uint64_t i = tid; i < nin; i += stride) {
-        const uint32_t c = qin[i], u = g.cown[c];
-        if (c != g.cstart[u]) continue;
-        const uint32_t m = mcur[u];
-        if (ms.far && (ms.far[u] & m)) ms.far[u] &= ~m;
-        mcur[u] = 0u;
-    }
+    // One loop over both lists, so that the two chains of dependent loads
+    // (list -> owner -> first chunk -> mask / far word; list -> owner -> next
+    // mask) are in flight together.
     uint32_t x = 0;
-    for (uint64_t i = tid; i < nout; i += stride) x |= mnxt[g.cown[qout[i]]];
+    const uint32_t nmax = nin > nout ? nin : nout;
+    for (uint64_t i = tid; i < nmax; i += stride) {
+        const bool in_ok = i < nin, out_ok = i < nout;
+        const uint32_t c = in_ok ? qin[i] : 0u, co = out_ok ? qout[i] : 0u;
+        const uint32_t u = in_ok ? g.cown[c] : 0u, uo = out_ok ? g.cown[co] : 0u;
+        const bool first = in_ok && c == g.cstart[u];
+        if (out_ok) x |= mnxt[uo];
+        if (first) {
+            const uint32_t m = mcur[u];
+            if (ms.far) {
+                const uint32_t fw = ms.far[u];
+                if (fw & m) ms.far[u] = fw & ~m;
+            }
+            mcur[u] = 0u;
+        }
+    }
     x = wave_or(x);
     if (lane_id() == 0) s_or[threadIdx.x / GGA_WAVE] = x;
     __syncthreads();
