@@ -221,7 +221,7 @@ def serving_leg(gga, np, full=True, quick=False):
                         "reduced config 5 set: R-MAT s16-s20 + grids 512^2, 1024^2") +
                        ", BFS/SSSP 50/50, uniform graph and source",
            "graphs": [n for n, _ in gs], "resident_gb": round(sum(d.device_bytes for d in dgs) / 1e9, 3)}
-    with Server(hbm_budget=16 << 30, steps_per_round=6, use_graph=True) as srv:
+    with Server(hbm_budget=16 << 30, steps_per_round=12, use_graph=True) as srv:
         for dg in dgs:
             srv.add_graph(dg)
         sat = poisson_trace(sizes, nq, rate=0, seed=21, degrees=degs)

@@ -106,7 +106,7 @@ int gga_server_create(const gga_server_opts *o, gga_server **out)
     if (!s) return GGA_ENOMEM;
     if (o) s->o = *o;
     if (!s->o.hbm_budget) s->o.hbm_budget = 8ull << 30;
-    if (!s->o.steps_per_round) s->o.steps_per_round = 6;
+    if (!s->o.steps_per_round) s->o.steps_per_round = 12;
     if (!s->o.lanes || s->o.lanes > GGA_MS_LANES) s->o.lanes = GGA_MS_LANES;
     if (!s->o.dry_steps) s->o.dry_steps = 3;
     if (!s->o.engines_per_graph) s->o.engines_per_graph = 1;

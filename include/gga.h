@@ -424,7 +424,7 @@ uint32_t gga_tune_layout(gga_graph *g);
 typedef struct gga_server gga_server;
 typedef struct gga_server_opts {
     uint64_t hbm_budget;      /* bytes of engine memory (0 -> 8 GiB) */
-    uint32_t steps_per_round; /* 0 -> 6 */
+    uint32_t steps_per_round; /* 0 -> 12 (two captured six-step graphs per round) */
     uint32_t lanes;           /* lanes used per engine, 1..32 (0 -> 32) */
     uint32_t use_graph;       /* steps replayed from captured hipGraphs */
     uint32_t dense_min;       /* engines' push / compaction switch (0 -> default) */
