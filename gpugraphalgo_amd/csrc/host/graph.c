@@ -659,11 +659,12 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
             goto fail;
         }
         /* steps per host check: sync_every, else 8 -- but the first batch is
-         * as long as the last query on this graph needed (up to 32), so that
-         * repeated queries end after one check and few empty launches */
+         * one longer than the last query on this graph needed (up to 32; step
+         * counts vary by one from run to run), so that repeated queries end
+         * after one check and at most two empty launches */
         const uint32_t per = o.sync_every ? o.sync_every : 8u;
         const uint32_t prev = g->last_steps[bfs ? 1 : 0];
-        uint32_t nb = o.sync_every ? per : (prev ? (prev < 32u ? prev : 32u) : per);
+        uint32_t nb = o.sync_every ? per : (prev ? (prev < 31u ? prev + 1u : 32u) : per);
         if ((rc = ensure_kev(g, 3 * (nb > per ? nb : per)))) goto fail;
         /* instrumentation buffers (freed below, also on failure) */
         const uint32_t tl_steps = o.timeline && o.timeline_cap >= grid ? o.timeline_cap / grid : 0u;

@@ -409,7 +409,11 @@ __global__ __launch_bounds__(64) void k_ms_seed(gga_dev_graph g, gga_dev_ms ms, 
 }
 
 template <bool COUNT, bool LM>
-__global__ __launch_bounds__(BLOCK) void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+// Eight waves per SIMD (64 VGPRs): the compiler's own choice, 67 VGPRs, allows
+// seven; the eighth wave of memory-bound relax took 32 SSSP lanes on R-MAT s20
+// from 7.4 to 5.9 ms (grids unchanged).
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
     const uint32_t nq = ms_len(ms, step);
     if (nq == 0) return;

@@ -315,10 +315,12 @@ __device__ __forceinline__ uint32_t hw_place() {
 
 // TL: instrumented variant (the reference's my_drelax_metric, cudaKernels.h:2-70,
 // which stores each block's clock64 span): each workgroup writes its start,
-// duration and placement to tl[blockIdx.x].
+// duration and placement to tl[blockIdx.x].  Every variant keeps eight waves
+// per SIMD (at most 64 VGPRs; the counting variants would otherwise take 62-65
+// and drop to seven): the relax is bound by the latency of its gathers.
 template <bool BFS, bool COUNT, bool TL>
-__global__ __launch_bounds__(BLOCK) void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step,
-                                                   gga_wg_record *tl) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
     uint64_t t0 = 0;
     if (TL && threadIdx.x == 0) t0 = wall_clock64();

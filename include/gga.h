@@ -181,8 +181,8 @@ typedef struct gga_opts {
     uint32_t max_iters;   /* 0 -> n + 1 (the serial bound, graph.h:122) */
     uint32_t block;       /* workgroup size, 0 -> default (256) */
     uint32_t sync_every;  /* WORKLIST: steps launched per host convergence check (0 -> 8, the
-                             first check after as many steps as the graph's last query needed,
-                             at most 32) */
+                             first check after one step more than the graph's last query
+                             needed, at most 32) */
     uint32_t use_graph;   /* multi-source engines: replay steps from captured hipGraphs (six
                              steps per graph) */
     uint32_t count;       /* 1 -> collect edges_inspected / chunks_expanded / relaxations */
