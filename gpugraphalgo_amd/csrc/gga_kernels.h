@@ -95,7 +95,8 @@ int ggak_ms_fold(gga_dev_ms ms, hipStream_t s);
 int ggak_indeg(const uint32_t *col, uint32_t m, uint32_t *indeg, hipStream_t s);
 /* tcol[pos[v]++] = u for every edge (u, v) of the split graph g. */
 int ggak_transpose_fill(gga_dev_graph g, uint32_t *pos, uint32_t *tcol, hipStream_t s);
-uint32_t ggak_msbfs_grid(int device);
+/* Workgroups of the cooperative bitmask BFS: per_cu per CU, within the occupancy bound. */
+uint32_t ggak_msbfs_grid(int device, uint32_t per_cu);
 /* Clears the bitmaps, seeds the k sources (d_src, device), builds level 0's push list. */
 int ggak_msbfs_init(gga_dev_graph g, gga_dev_msbfs b, const uint32_t *d_src, hipStream_t s);
 /* All levels in one cooperative launch (t: transposed graph with its chunk table). */

@@ -378,14 +378,18 @@ extern "C" int ggak_transpose_fill(gga_dev_graph g, uint32_t *pos, uint32_t *tco
     return launch_status();
 }
 
-extern "C" uint32_t ggak_msbfs_grid(int device) {
+extern "C" uint32_t ggak_msbfs_grid(int device, uint32_t per_cu) {
     int cus = 0, per = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_msbfs), BLOCK, 0) !=
             hipSuccess ||
         per <= 0)
         return 0;
-    return (uint32_t)cus;  // one workgroup per CU for the cooperative launch
+    // the cooperative launch keeps well under the co-residency the occupancy
+    // query reports (6 per CU for this kernel)
+    if (per_cu == 0) per_cu = 1;
+    const uint32_t pc = (uint32_t)per < per_cu ? (uint32_t)per : per_cu;
+    return (uint32_t)cus * pc;
 }
 
 extern "C" int ggak_msbfs_init(gga_dev_graph g, gga_dev_msbfs b, const uint32_t *d_src, hipStream_t s) {
