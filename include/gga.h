@@ -329,8 +329,8 @@ int gga_ms_start(gga_ms *e, uint32_t lanes, const uint32_t *src, uint32_t unit_l
  * started and not live has finished. */
 int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live);
 /* Asynchronous form of a run of nsteps steps, for callers that drive several
- * engines (graphs) at once: launch enqueues the steps and a read-back on the
- * graph's stream and returns; collect waits for them and reports *live as
+ * engines at once: launch enqueues the steps and a read-back on the engine's
+ * own stream and returns; collect waits for them and reports *live as
  * gga_ms_run does.  Every launch is followed by one collect. */
 int gga_ms_launch(gga_ms *e, uint32_t nsteps, const gga_opts *o);
 int gga_ms_collect(gga_ms *e, uint32_t *live);
@@ -413,14 +413,15 @@ uint32_t gga_tune_layout(gga_graph *g);
 /* ------------------------------------------------------------- serving
  * No reference counterpart (one query per process in the reference).  A
  * server keeps graphs resident in HBM and answers BFS/SSSP queries on them:
- * each graph gets, when a query needs it, one 32-lane multi-source engine
- * whose device memory (gga_ms_bytes) is admitted against hbm_budget, evicting
- * idle engines least-recently-used first; a query takes one lane from
- * admission to convergence, and lanes freed by finished queries are refilled
- * from the waiting queries at the next round (continuous batching).  Every
- * round launches `steps_per_round` steps on every engine with running lanes
- * (one stream per graph, so graphs overlap on the device).  Results do not
- * depend on the mix: each lane is the single-source computation. */
+ * each graph gets, when a query needs it, a 32-lane multi-source engine (up
+ * to engines_per_graph of them) whose device memory (gga_ms_bytes) is
+ * admitted against hbm_budget, evicting idle engines least-recently-used
+ * first; a query takes one lane from admission to convergence, and lanes
+ * freed by finished queries are refilled from the waiting queries at the next
+ * round (continuous batching).  Every round launches `steps_per_round` steps
+ * on every engine with running lanes (one stream per engine, so engines
+ * overlap on the device).  Results do not depend on the mix: each lane is the
+ * single-source computation. */
 typedef struct gga_server gga_server;
 typedef struct gga_server_opts {
     uint64_t hbm_budget;      /* bytes of engine memory (0 -> 8 GiB) */
