@@ -296,8 +296,9 @@ void gga_graph_name(const char *path, char *out, size_t cap);
  * batching).  Results per lane equal gga_bfs / gga_sssp from that source. */
 #define GGA_MS_LANES 32
 typedef struct gga_ms gga_ms;
-/* Engine on a resident graph (split with the defaults first if it is not).
- * Device memory: 136 B per vertex + 8 B per chunk + the per-tile scratch.
+/* Engine on a resident graph (split with the defaults first if it is not),
+ * with a stream of its own.  Device memory: 136 B per vertex + 8 B per chunk
+ * + the per-tile scratch.
  * inf: unreached value (0 -> GGA_INF).  layout: GGA_MS_VERTEX_MAJOR (the 32
  * lanes of a vertex in one 128-byte line: lanes that meet at the same
  * vertices share every gather -- small-world graphs), GGA_MS_LANE_MAJOR (each
@@ -311,10 +312,10 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out);
 int gga_ms_free(gga_ms *e);
 /* Near-far buckets per SSSP lane (as opts.delta for one source): a lane's
  * improvement at or beyond its threshold waits until a refill reaches it.
- * Set at creation from the graph (gga_tune's width, or 8 mean weights on
+ * Set at creation from the graph (gga_tune's width, or 32 mean weights on
  * mesh-like graphs, else off); delta 0 or 0xFFFFFFFF turns it off.  Only while
  * no lane runs; near-far engines run in step mode (GGA_MODE_WORKLIST).
- * +128 B per vertex. */
+ * +4 B per vertex (the far masks). */
 int gga_ms_set_delta(gga_ms *e, uint32_t delta);
 /* Device bytes gga_ms_create would take on this graph (near-far included). */
 uint64_t gga_ms_graph_bytes(gga_graph *g);
