@@ -197,7 +197,8 @@ typedef struct gga_opts {
                              only while its distance is below the current threshold, the rest
                              wait until the threshold passes them (Davidson et al. 2014), so
                              fewer vertices are expanded more than once; 0 -> the tuned width
-                             (gga_tune) else off; 0xFFFFFFFF -> off */
+                             (gga_tune), else 32 mean edge weights on mesh-like graphs (max
+                             degree within 4x the mean + 8), else off; 0xFFFFFFFF -> off */
     /* WORKLIST instrumentation (both off when NULL) */
     gga_wg_record *timeline;    /* host buffer: grid records per step, steps in order */
     uint32_t timeline_cap;      /* records it holds; steps beyond it run uninstrumented */
