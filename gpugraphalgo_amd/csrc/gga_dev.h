@@ -36,7 +36,9 @@ typedef struct gga_dev_graph {
 
 typedef struct gga_dev_work {
     uint32_t *dist;
-    uint32_t *mark;
+    uint32_t *mark;      /* queue steps: step number of a vertex's last listing (de-duplication) */
+    uint8_t *flag;       /* flag steps: low byte of the step that last lowered a vertex (n bytes,
+                            set to the byte before the query's first step at query start) */
     uint32_t *q0;
     uint32_t *q1;
     uint32_t *ctl;

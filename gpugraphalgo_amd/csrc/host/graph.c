@@ -37,6 +37,7 @@ struct gga_graph {
     gga_split_stats sst;
     /* query workspace */
     uint32_t *d_dist, *d_mark, *d_q0, *d_q1, *d_ctl, *d_tiles;
+    uint8_t *d_flag;
     uint64_t *d_cnt;
     uint32_t qcap;
     uint32_t *h_pin;          /* pinned: GGA_CTL_WORDS words */
@@ -135,6 +136,7 @@ static gga_dev_work dev_work(const gga_graph *g, uint32_t inf)
     gga_dev_work w;
     w.dist = g->d_dist;
     w.mark = g->d_mark;
+    w.flag = g->d_flag;
     w.q0 = g->d_q0;
     w.q1 = g->d_q1;
     w.ctl = g->d_ctl;
@@ -183,6 +185,7 @@ int gga_graph_free(gga_graph *g)
     free_queues(g);
     dfree(g, (void **)&g->d_dist, 4ull * g->n);
     dfree(g, (void **)&g->d_mark, 4ull * g->n);
+    dfree(g, (void **)&g->d_flag, (uint64_t)g->n);
     dfree(g, (void **)&g->d_ctl, 4ull * GGA_CTL_WORDS);
     dfree(g, (void **)&g->d_cnt, 32ull * GGA_CNT_SLOTS);
     dfree(g, (void **)&g->d_tiles, 4ull * (GGA_TILES((uint64_t)g->n) + 1));
@@ -234,6 +237,7 @@ static int graph_create(uint32_t n, uint32_t m, const uint32_t *row_ptr, const u
     if ((rc = dmalloc(g, (void **)&g->d_tiles, 4ull * (GGA_TILES((uint64_t)n) + 1)))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_dist, 4ull * n))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_mark, 4ull * n))) goto fail;
+    if ((rc = dmalloc(g, (void **)&g->d_flag, (uint64_t)n))) goto fail;
     HIP_TRY(hipMemcpyAsync(g->d_row, row_ptr, 4ull * ((uint64_t)n + 1), hipMemcpyHostToDevice, g->stream));
     if (m) HIP_TRY(hipMemcpyAsync(g->d_col, dst, 4ull * m, hipMemcpyHostToDevice, g->stream));
     if (m && w) HIP_TRY(hipMemcpyAsync(g->d_w, w, 4ull * m, hipMemcpyHostToDevice, g->stream));

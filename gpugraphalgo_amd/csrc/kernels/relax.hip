@@ -121,7 +121,7 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
             if (act[u] && alt >= ds[u] && alt < inf && alt < cur[u]) {
                 if (OUT == OUT_FLAG) {
                     (void)atomic_min_u32(&dist[v[u]], alt);  // result unused: no-return atomic
-                    if (alt < thr) w.mark[v[u]] = it;
+                    if (alt < thr) w.flag[v[u]] = (uint8_t)it;
                     imp = true;
                 } else {
                     const uint32_t old = atomic_min_u32(&dist[v[u]], alt);
@@ -183,6 +183,7 @@ __global__ __launch_bounds__(BLOCK) void k_query_init(gga_dev_graph g, gga_dev_w
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t nth = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = tid; v < g.n; v += nth) w.dist[v] = (v == src) ? 0u : w.inf;
+    for (uint64_t v = tid; v < g.n; v += nth) w.flag[v] = (uint8_t)(epoch - 1u);
     if (w.last)
         for (uint64_t v = tid; v < g.n; v += nth) w.last[v] = w.inf;
     for (uint64_t i = tid; i < 4ull * GGA_CNT_SLOTS; i += nth) w.cnt[i] = 0;
@@ -270,11 +271,15 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
 }
 
-// A vertex belongs to the next list when the current step stamped it.
+// A vertex belongs to the next list when the current (flag) step stamped it.
+// One byte per vertex, so the two passes over the stamps read n bytes, not
+// 4n.  The byte repeats every 256 steps of a query: a vertex last stamped 256
+// steps before is listed again -- one extra expansion at an unchanged
+// distance, no effect on any result.
 struct StampMarked {
-    const uint32_t *mark;
-    uint32_t it;
-    __device__ __forceinline__ bool operator()(uint32_t v) const { return mark[v] == it; }
+    const uint8_t *flag;
+    uint8_t it8;
+    __device__ __forceinline__ bool operator()(uint32_t v) const { return flag[v] == it8; }
 };
 
 // The single list of the plain layout for step `step`.
@@ -352,7 +357,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_de
     if (nq == 0 || !flag_step(w, nq)) return;
     const ListOut o = list_out(g, w, step);
     const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_compact_scan(gga_dev_graph g, gga_dev_work w, uint32_t step) {
@@ -368,7 +373,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_de
     if (nq == 0 || !flag_step(w, nq)) return;
     const ListOut o = list_out(g, w, step);
     const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
 }
 
 // ---- near-far refill: after a step whose next list came out empty, list
@@ -513,12 +518,12 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
             const ListOut o = list_out(g, w, step);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
             for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
-                compact_count_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
+                compact_count_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
             if (blockIdx.x == 0) compact_scan(o, g.n, s_wave);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
             for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
-                compact_write_tile(o, g.n, t, StampMarked{w.mark, it}, s_wave);
+                compact_write_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
         }
         grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
         // near-far: an empty next list is refilled from the deferred vertices,
