@@ -37,6 +37,14 @@ constexpr uint32_t LANES = GGA_MS_LANES;
 #define GGA_MS_UNROLL 4
 #endif
 constexpr uint32_t MS_UNROLL = GGA_MS_UNROLL;  // edges per half wave per round
+// (edge, lane) pairs per wave lane per round of the flattened path: one
+// (with the few lanes per chunk of a mesh a round rarely fills, and a deeper
+// round only lengthens its chain -- 32 SSSP lanes on the 4096^2 grid: 1.28 s
+// at 4, 1.20 at 2, 1.16 at 1; R-MAT s20 unchanged)
+#ifndef GGA_MS_FLAT_UNROLL
+#define GGA_MS_FLAT_UNROLL 1
+#endif
+constexpr uint32_t FLAT_UNROLL = GGA_MS_FLAT_UNROLL;
 #ifndef GGA_MS_FLAT
 #define GGA_MS_FLAT 16
 #endif
@@ -163,11 +171,11 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
         const uint32_t ptotal = wave_bcast(pincl, GGA_WAVE - 1);
         if (ptotal <= GGA_MS_FLAT * total) {
             const uint32_t pexcl = pincl - pairs;
-            for (uint32_t base = 0; base < ptotal; base += GGA_WAVE * MS_UNROLL) {
-                uint32_t v[MS_UNROLL], wt[MS_UNROLL], du[MS_UNROLL], dv[MS_UNROLL], sl[MS_UNROLL];
-                bool act[MS_UNROLL];
+            for (uint32_t base = 0; base < ptotal; base += GGA_WAVE * FLAT_UNROLL) {
+                uint32_t v[FLAT_UNROLL], wt[FLAT_UNROLL], du[FLAT_UNROLL], dv[FLAT_UNROLL], sl[FLAT_UNROLL];
+                bool act[FLAT_UNROLL];
 #pragma unroll
-                for (uint32_t i = 0; i < MS_UNROLL; i++) {
+                for (uint32_t i = 0; i < FLAT_UNROLL; i++) {
                     // pair k of owner j: edge r / pc_j, the (r % pc_j)-th lane of its mask,
                     // so consecutive pairs are the lanes of one edge (one D line)
                     const uint32_t k = base + i * GGA_WAVE + l;
@@ -185,10 +193,10 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                     du[i] = D[LM ? (uint64_t)sl[i] * g.n + uu : (uint64_t)uu * LANES + sl[i]];
                 }
 #pragma unroll
-                for (uint32_t i = 0; i < MS_UNROLL; i++)
+                for (uint32_t i = 0; i < FLAT_UNROLL; i++)
                     dv[i] = D[LM ? (uint64_t)sl[i] * g.n + v[i] : (uint64_t)v[i] * LANES + sl[i]];
 #pragma unroll
-                for (uint32_t i = 0; i < MS_UNROLL; i++) {
+                for (uint32_t i = 0; i < FLAT_UNROLL; i++) {
                     const uint32_t thr = (uint32_t)__shfl(thr_s, (int)sl[i], GGA_WAVE);
                     const bool on = act[i] && du[i] < inf;
                     const uint32_t a = du[i] + (((unit_word >> sl[i]) & 1u) ? 1u : wt[i]);
