@@ -120,6 +120,7 @@ def batched_leg(gga, np, quick=False, coop=True):
     with gga.DeviceGraph(g) as dg:
         tr = dg.tune(int(ssrc[0]), bfs=False, layout=not quick, multi=True)
         out["tuned"] = {"split_threshold": tr["mda"], "seed_threshold": tr["seed_mda"], "grid": tr["grid"],
+                        "tried_ms": {str(t): round(ms, 3) for t, ms in tr["tried"].items()},
                         "dense_min": tr["dense_min"], "layout": {1: "vertex", 2: "lane"}[tr["layout"]]}
         reps = 3 if quick else 5
         wb, sb = None, None
