@@ -310,6 +310,7 @@ static int deliver(gga_server *s, srv_engine *G, uint32_t lanes)
         uint32_t hmask = 0, dmask = 0;
         memset(hp, 0, sizeof hp);
         memset(dp, 0, sizeof dp);
+        pthread_mutex_lock(&s->mu);   /* a concurrent submit may move qs */
         for (uint32_t l = 0; l < GGA_MS_LANES; l++) {
             if (!((lanes >> l) & 1u)) continue;
             const gga_query *q = &s->qs[G->lane_q[l]].q;
@@ -322,6 +323,7 @@ static int deliver(gga_server *s, srv_engine *G, uint32_t lanes)
                 hmask |= 1u << l;
             }
         }
+        pthread_mutex_unlock(&s->mu);
         if (dmask && (rc = gga_ms_fetch(G->e, dmask, dp, 1))) return rc;
         if (hmask && (rc = gga_ms_fetch(G->e, hmask, hp, 0))) return rc;
     }
@@ -450,19 +452,34 @@ int gga_server_drain(gga_server *s)
     return GGA_OK;
 }
 
+/* Is qid one of the ascending ids[0..n)? */
+static int in_ids(const uint64_t *ids, uint32_t n, uint64_t qid)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (ids[mid] < qid) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && ids[lo] == qid;
+}
+
 int gga_server_replay(gga_server *s, const gga_query *trace, uint32_t nq, gga_query_result *res)
 {
     int rc = GGA_OK;
     if (!s || (nq && (!trace || !res))) return GGA_EINVAL;
+    /* ids of this replay's queries: ascending, and not contiguous when
+     * another thread submits meanwhile */
+    uint64_t *ids = (uint64_t *)malloc(sizeof(uint64_t) * (nq ? nq : 1));
+    if (!ids) return GGA_ENOMEM;
     const double base = gga_server_now(s);
-    const uint64_t first = s->nq;
     uint32_t next = 0;
     while (next < nq || !idle(s)) {
         double now = gga_server_now(s) - base;
         for (; next < nq && trace[next].arrival <= now; next++) {
             gga_query q = trace[next];
             q.arrival = base + trace[next].arrival;
-            if ((rc = gga_server_submit(s, &q, NULL))) return rc;
+            if ((rc = gga_server_submit(s, &q, &ids[next]))) goto out;
         }
         if (idle(s)) {
             if (next < nq) {   /* nothing to do before the next arrival */
@@ -477,22 +494,24 @@ int gga_server_replay(gga_server *s, const gga_query *trace, uint32_t nq, gga_qu
             continue;
         }
         uint32_t act = 0;
-        if ((rc = gga_server_pump(s, &act))) return rc;
+        if ((rc = gga_server_pump(s, &act))) goto out;
     }
+    pthread_mutex_lock(&s->mu);
     for (uint32_t i = 0; i < nq; i++) {
-        res[i] = s->qs[first + i].r;
+        res[i] = s->qs[ids[i]].r;
         res[i].arrival -= base;
         res[i].admitted -= base;
         res[i].done -= base;
     }
     /* results were delivered here; drop them from the poll queue */
-    pthread_mutex_lock(&s->mu);
     uint64_t keep = 0;
     for (uint64_t i = 0; i < s->nfin; i++)
-        if (s->fin[i] < first || s->fin[i] >= first + nq) s->fin[keep++] = s->fin[i];
+        if (!in_ids(ids, nq, s->fin[i])) s->fin[keep++] = s->fin[i];
     s->nfin = keep;
     pthread_mutex_unlock(&s->mu);
-    return GGA_OK;
+out:
+    free(ids);
+    return rc;
 }
 
 int gga_server_replay_plain(gga_server *s, const gga_query *trace, uint32_t nq, int32_t mode,

@@ -464,6 +464,10 @@ typedef struct gga_server_counters {
     uint32_t engines_resident, pad;
 } gga_server_counters;
 
+/* Threading: gga_server_submit, gga_server_poll and gga_server_now may be
+ * called from any thread while one thread drives pump / drain / replay;
+ * add_graph, stats and free run on that thread (or with no other call in
+ * flight). */
 int gga_server_create(const gga_server_opts *o, gga_server **out);
 int gga_server_free(gga_server *s);
 /* The graph stays owned by the caller and must outlive the server. */

@@ -167,3 +167,50 @@ def test_checksum_is_order_independent_fingerprint():
     e = d.copy()
     e[17] ^= 1
     assert checksum(e) != checksum(d)
+
+
+def test_submit_and_poll_from_other_threads_while_pumping():
+    """Submitters on two threads (thousands of queries, so the query table
+    grows under the pumping thread) and a poller on a third: every query
+    completes and is polled at most once, each replay result matches its own
+    query."""
+    import threading
+    with Server(dry_run=True, dry_steps=2, lanes=8) as s:
+        g = [s.add_graph_dry(1000, 1100), s.add_graph_dry(2000, 2100)]
+        n_each = 3000
+        ids = {0: [], 1: []}
+        polled = []
+        done = threading.Event()
+
+        def submitter(k):
+            for i in range(n_each):
+                ids[k].append(s.submit(g[k], i % 1000, bool(i & 1)))
+
+        def poller():
+            while not done.is_set():
+                polled.extend(int(x) for x in s.poll()["id"])
+            polled.extend(int(x) for x in s.poll()["id"])
+
+        th = [threading.Thread(target=submitter, args=(k,)) for k in (0, 1)]
+        pt = threading.Thread(target=poller)
+        pt.start()
+        for t in th:
+            t.start()
+        rows = [(g[0], 7, 1, 0.0), (g[1], 9, 0, 0.0)] * 50
+        res = s.replay(trace_of(rows))   # pumps until idle, meanwhile others submit
+        for t in th:
+            t.join()
+        s.drain()
+        done.set()
+        pt.join()
+        st = s.stats()
+    assert (res["status"] == 0).all()
+    assert list(res["graph"]) == [r[0] for r in rows] and list(res["src"]) == [r[1] for r in rows]
+    submitted = ids[0] + ids[1]
+    assert len(set(submitted)) == 2 * n_each
+    # every result is polled at most once; the submitted ones all arrive, and
+    # the replay's own may reach a concurrent poller before the replay ends
+    assert len(polled) == len(set(polled))
+    assert set(submitted) <= set(polled)
+    assert set(polled) - set(submitted) <= {int(x) for x in res["id"]}
+    assert st["completed"] == 2 * n_each + len(rows)
