@@ -136,3 +136,49 @@ def test_budget_eviction_and_device_outputs(gpu):
         assert int(res["checksum"][i]) == checksum(want)
     for dg in dgs:
         dg.close()
+
+
+def test_submit_from_another_thread_while_serving(gpu):
+    """A second thread submits queries with host outputs while this one
+    replays a trace (the query table grows under the serving thread); every
+    result, from either thread, equals the oracle's."""
+    import threading
+    gs = graphs_small()
+    dgs = [gga.DeviceGraph(g) for g in gs]
+    for dg in dgs:
+        dg.split(mda=16)
+    srcs = sources_with_edges(gs)
+    rng = np.random.default_rng(21)
+    side = [(int(k), int(rng.choice(srcs[k])), bool(i & 1))
+            for i, k in enumerate(rng.integers(0, len(gs), 200))]
+    side_out = [np.empty(gs[k].n, dtype=np.uint32) for k, _, _ in side]
+    ids = []
+    trace = poisson_trace([g.n for g in gs], 60, rate=0, seed=22, degrees=srcs)
+    outs = [np.empty(gs[int(r["graph"])].n, dtype=np.uint32) for r in trace]
+    with Server(host_results=True, use_graph=True, steps_per_round=6) as srv:
+        for dg in dgs:
+            srv.add_graph(dg)
+
+        def submitter():
+            for (k, s, b), o in zip(side, side_out):
+                ids.append(srv.submit(k, s, b, out=o))
+
+        th = threading.Thread(target=submitter)
+        announce("serve 60 replayed + 200 submitted from a second thread")
+        th.start()
+        res = srv.replay(trace, outs=outs)
+        th.join()
+        srv.drain()
+        polled = srv.poll()
+        st = srv.stats()
+    assert (res["status"] == 0).all()
+    assert st["completed"] == 260
+    done = {int(i): int(s) for i, s in zip(polled["id"], polled["status"])}
+    assert all(done.get(i) == 0 for i in ids)
+    for i, r in enumerate(trace):
+        np.testing.assert_array_equal(outs[i], want_for(gs[int(r["graph"])], r["src"], bool(r["bfs"])),
+                                      err_msg=f"replayed query {i}")
+    for j, ((k, s, b), o) in enumerate(zip(side, side_out)):
+        np.testing.assert_array_equal(o, want_for(gs[k], s, b), err_msg=f"submitted query {j}")
+    for dg in dgs:
+        dg.close()
