@@ -28,9 +28,9 @@
 //     succeeds once per vertex).  Queued vertices are staged in wave-private
 //     LDS and flushed as chunk ids with one atomicAdd per 64 vertices.
 //   flag step (large frontier): fire-and-forget atomicMin and a plain store of
-//     the step stamp into mark[v]; then a three-pass compaction (count per
-//     tile, scan of tile counts, write) rebuilds the list from mark[] with no
-//     shared counter at all.
+//     the step's low byte into flag[v] (a byte array: a quarter of mark[]'s
+//     lines); then a three-pass compaction (count per tile, scan of tile
+//     counts, write) rebuilds the list from flag[] with no shared counter.
 // Device-scope atomics on MI355X execute past the per-XCD L2 and atomics on
 // one address serialize (~16 ns each), so no hot path funnels through a single
 // counter: batch assignment is static, work counters are striped, list
