@@ -13,6 +13,7 @@
  * Per-query workspace:
  *   dist[n]         result, indexed by ORIGINAL vertex (no pseudo-children)
  *   mark[n]         last step that queued the vertex (worklist de-duplication)
+ *   flag[n]         bytes: low byte of the last flag step that lowered the vertex
  *   q[2][C]         chunk worklists (a vertex enters a list at most once per step)
  *   ctl[]           control words, one 128-byte line per hot counter
  */
