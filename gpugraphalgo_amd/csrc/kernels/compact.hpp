@@ -95,16 +95,30 @@ __device__ __forceinline__ void compact_count_tile(const ListOut &o, uint32_t n,
     if (threadIdx.x == 0) o.tiles[t] = tot;
 }
 
-// pass 2 (one workgroup): exclusive scan of tiles[]; the total is the new list length
+// pass 2 (one workgroup): exclusive scan of tiles[]; the total is the new list
+// length.  Each thread takes SCAN_ITEMS consecutive tiles (all loaded before
+// the first is used), so a round covers CBLOCK * SCAN_ITEMS tiles with one
+// workgroup scan: the 8 192 tiles of a 16.8 M-vertex graph take 4 rounds, not 32.
+constexpr uint32_t SCAN_ITEMS = 8;
 __device__ __forceinline__ void compact_scan(const ListOut &o, uint32_t n, uint32_t *s_wave) {
     const uint32_t ntiles = GGA_TILES(n);
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < ntiles; base += CBLOCK) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t x = i < ntiles ? o.tiles[i] : 0u;
+    for (uint32_t base = 0; base < ntiles; base += CBLOCK * SCAN_ITEMS) {
+        const uint32_t i0 = base + threadIdx.x * SCAN_ITEMS;
+        uint32_t x[SCAN_ITEMS], sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_ITEMS; k++) {
+            x[k] = i0 + k < ntiles ? o.tiles[i0 + k] : 0u;
+            sum += x[k];
+        }
         uint32_t excl;
-        const uint32_t tot = block_excl_scan(x, excl, s_wave);
-        if (i < ntiles) o.tiles[i] = carry + excl;
+        const uint32_t tot = block_excl_scan(sum, excl, s_wave);
+        uint32_t run = carry + excl;
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_ITEMS; k++) {
+            if (i0 + k < ntiles) o.tiles[i0 + k] = run;
+            run += x[k];
+        }
         carry += tot;
     }
     if (threadIdx.x == 0) *o.tail = carry;
