@@ -199,6 +199,26 @@ def test_multi_bfs_256_lanes_equal_oracle(gpu, alpha):
                 assert st["pull_levels"] == 0
 
 
+def test_multi_bfs_golden_graphs(gpu, golden):
+    """The bitmask BFS on every golden graph (a single vertex, no edges,
+    self-loops and duplicate edges, the reference's testgraph, a grid): every
+    vertex as a source (up to 256 lanes), push-only and pull-eager."""
+    index, arrays, _ = golden
+    for name in sorted({e["graph"] for e in index}):
+        rp, dst, w = golden_graph(arrays, name)
+        g = gga.csr_from_arrays(rp, dst, w)
+        srcs = np.arange(min(g.n, 256), dtype=np.uint32)
+        with gga.DeviceGraph(g) as dg:
+            for mda in (1, 3):
+                dg.split(mda=mda)
+                for pa in (0xFFFFFFFF, 1):
+                    announce(f"multi_bfs golden {name} mda={mda} k={srcs.size} pull_alpha={pa}")
+                    out, _ = dg.multi_bfs(srcs, pull_alpha=pa)
+                    for i, s in enumerate(srcs):
+                        np.testing.assert_array_equal(out[i], want_for(g, s, True),
+                                                      err_msg=f"{name} mda={mda} lane {i}")
+
+
 def test_multi_bfs_directed_grid_and_isolated(gpu):
     """Directed graph (pull must use in-edges), an isolated source, REF_INF."""
     rp = np.array([0, 2, 3, 3, 4, 4], dtype=np.uint32)     # 0->1,0->2  1->3  3->1  4 isolated
