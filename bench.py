@@ -91,6 +91,17 @@ def summarize(wall, stats, mode, bfs):
     }
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(g, src, m_reach):
     import oracle
     import numpy as np
@@ -102,7 +113,7 @@ def cpu_baseline(g, src, m_reach):
     return {"value": mr / t / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
             "sample": (f"one full run of the restated serial Gauss-Seidel SSSP (graph.h:115-148) "
                        f"on the same graph and source: {t:.2f} s, {sweeps} sweeps, 1 of "
-                       f"{os.cpu_count()} host cores")}, d
+                       f"{os.cpu_count()} host cores ({cpu_model()})")}, d
 
 
 def batched_leg(gga, np, quick=False, coop=True):
