@@ -82,8 +82,15 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
         const uint32_t ee = stream_load(&g.cbeg[c + 1]);
         const uint32_t own = stream_load(&g.cown[c]);
         du = dist[own];
-        n = du < inf ? ee - eb : 0u;
-        if (w.last && du < inf) w.last[own] = du;
+        // SSSP: an owner already lowered and listed again during this step is
+        // expanded in the next one with its newer distance, which relaxes
+        // every edge at least as far -- expanding it now would only repeat
+        // that work, so its chunks are left to the next step.
+        bool again = false;
+        if (!BFS && OUT == OUT_FLAG) again = w.flag[own] == (uint8_t)it;
+        if (!BFS && OUT == OUT_QUEUE) again = w.mark[own] == it;
+        n = du < inf && !again ? ee - eb : 0u;
+        if (w.last && du < inf && !again) w.last[own] = du;
     }
     const uint32_t incl = wave_incl_scan(n);
     const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
