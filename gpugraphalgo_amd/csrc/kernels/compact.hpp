@@ -66,7 +66,12 @@ struct ListOut {
     uint32_t *ctl;           // error bits
 };
 
-// Per-thread view of one tile: CITEMS vertices v0 + i * CBLOCK + tid (coalesced).
+// Per-thread view of one tile: CITEMS consecutive vertices v0 + tid * CITEMS + i.
+// The list comes out in vertex order, so the next step's batches walk
+// adjacent chunks (contiguous col / w runs across chunk boundaries), and a
+// thread's ids go to consecutive positions.  Against vertices strided by
+// CBLOCK: R-MAT s22 SSSP 2.24 -> 2.18 ms, the 4096² grid's persistent BFS /
+// SSSP 9.5 / 15.4 -> 8.7 / 14.1 ms.
 template <class Marked>
 __device__ __forceinline__ uint32_t tile_items(const ListOut &o, uint32_t n, uint32_t t, const Marked &marked,
                                                uint32_t *cs, uint32_t *nch) {
@@ -74,7 +79,7 @@ __device__ __forceinline__ uint32_t tile_items(const ListOut &o, uint32_t n, uin
     uint32_t mine = 0;
 #pragma unroll
     for (uint32_t i = 0; i < CITEMS; i++) {
-        const uint64_t v = v0 + (uint64_t)i * CBLOCK + threadIdx.x;
+        const uint64_t v = v0 + (uint64_t)threadIdx.x * CITEMS + i;
         cs[i] = 0;
         nch[i] = 0;
         if (v < n && marked((uint32_t)v)) {
