@@ -310,7 +310,10 @@ def _opts(mode, inf: int, count: bool, kw: dict) -> _lib.Opts:
     o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
     o.inf = inf
     o.count = 1 if count else 0
+    names = {f for f, _ in _lib.Opts._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"unknown gga_opts field {k!r}")
         setattr(o, k, v)
     return o
 
