@@ -76,7 +76,7 @@ def test_freed_lanes_are_refilled_continuously():
         st = s.stats()
     assert done == list(range(10))                 # FIFO within one graph
     assert max(a for a, _ in rounds_seen) <= 4     # never more lanes than allowed
-    # round 0 launches; rounds 1-3 collect; the lanes freed in round 3 restart in round 3
+    # pump 0 launches; pumps 1-3 collect; the lanes freed in pump 3 restart in pump 3
     assert [d for _, d in rounds_seen[:10]] == [0, 0, 0, 4, 4, 4, 8, 8, 8, 10]
     assert st["engines_created"] == 1
 
