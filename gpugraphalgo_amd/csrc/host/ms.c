@@ -163,6 +163,21 @@ fail:
     return rc;
 }
 
+/* Frees the near-far state (far masks, refill partials). */
+static void ms_drop_far(gga_ms *e)
+{
+    if (e->d.far) {
+        ms_release(e->d.far);
+        e->dev_bytes -= 4ull * e->n;
+    }
+    if (e->d.lpart) {
+        ms_release(e->d.lpart);
+        e->dev_bytes -= 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
+    }
+    e->d.far = NULL;
+    e->d.lpart = NULL;
+}
+
 int gga_ms_set_delta(gga_ms *e, uint32_t delta)
 {
     int rc = GGA_OK;
@@ -174,17 +189,15 @@ int gga_ms_set_delta(gga_ms *e, uint32_t delta)
     if (delta == 0xFFFFFFFFu) delta = 0;
     HIP_TRY(hipSetDevice(e->v.device));
     if (delta && !e->d.far) {
-        if ((rc = ms_alloc(e, (void **)&e->d.far, 4ull * e->n))) return rc;
-        if ((rc = ms_alloc(e, (void **)&e->d.lpart, 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS))) return rc;
+        /* the kernels take near-far state from far / lpart being set: both or
+         * neither, so any failure below turns near-far off again */
+        if ((rc = ms_alloc(e, (void **)&e->d.far, 4ull * e->n))) goto fail;
+        if ((rc = ms_alloc(e, (void **)&e->d.lpart, 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS))) goto fail;
         K_TRY(ggak_fill_u32(e->d.far, 0u, e->n, e->v.stream));
         HIP_TRY(hipStreamSynchronize(e->v.stream));
     } else if (!delta && e->d.far) {
         HIP_TRY(hipStreamSynchronize(e->v.stream));
-        ms_release(e->d.far);
-        ms_release(e->d.lpart);
-        e->dev_bytes -= 4ull * e->n + 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
-        e->d.far = NULL;
-        e->d.lpart = NULL;
+        ms_drop_far(e);
     }
     e->d.delta = delta;
     /* captured steps hold the old engine arguments */
@@ -195,6 +208,10 @@ int gga_ms_set_delta(gga_ms *e, uint32_t delta)
         }
     return GGA_OK;
 fail:
+    if (e->d.far && !e->d.delta) {   /* only a half-built near-far state is undone */
+        hipStreamSynchronize(e->v.stream);
+        ms_drop_far(e);
+    }
     return rc;
 }
 
