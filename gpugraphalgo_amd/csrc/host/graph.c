@@ -341,11 +341,25 @@ static int ensure_queues(gga_graph *g, uint32_t cap)
     return GGA_OK;
 }
 
+/* gga_graph_split's histogram steps: on failure free the histogram buffers
+ * and return without touching the current chunk table */
+#define HIST_TRY(expr)                                                                     \
+    do {                                                                                   \
+        const int e_ = (int)(expr);                                                        \
+        if (e_ != 0) {                                                                     \
+            gga_set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,                       \
+                          hipGetErrorString((hipError_t)e_));                              \
+            dfree(g, (void **)&hist, 4ull * b);                                            \
+            free(h);                                                                       \
+            return GGA_EHIP;                                                               \
+        }                                                                                  \
+    } while (0)
+
 int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets, gga_split_stats *st)
 {
     int rc = GGA_OK;
-    uint32_t *d_tmp = NULL;
-    uint32_t tmp_words = 0;
+    uint32_t *d_tmp = NULL, *hist = NULL, *h = NULL;
+    uint32_t tmp_words = 0, b = 0;
     if (!g) return GGA_EINVAL;
     gga_clear_error();
     HIP_TRY(hipSetDevice(g->device));
@@ -362,21 +376,21 @@ int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets,
         T = (g->maxdeg + (uint32_t)level - 1u) / (uint32_t)level;   /* graph.h:390 */
         if (T == 0) T = 1;                                           /* no edges at all */
     } else if (level < 0) {
-        uint32_t b = buckets ? buckets : 10u;                        /* bfs_sssp.cu:166 */
+        b = buckets ? buckets : 10u;                                 /* bfs_sssp.cu:166 */
         if (b > 4096) return GGA_ERANGE;
         bucket_rec = b;
-        uint32_t *hist = NULL;
         rc = dmalloc(g, (void **)&hist, 4ull * b);
         if (rc) return rc;
-        K_TRY(ggak_fill_u32(hist, 0u, b, g->stream));
-        K_TRY(ggak_degree_hist(g->d_row, g->n, g->maxdeg, b, hist, g->stream));
-        uint32_t *h = (uint32_t *)malloc(4ull * b);
+        h = (uint32_t *)malloc(4ull * b);
         if (!h) {
             dfree(g, (void **)&hist, 4ull * b);
             return GGA_ENOMEM;
         }
-        HIP_TRY(hipMemcpyAsync(h, hist, 4ull * b, hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
+        /* a failure in here leaves the previous split in place */
+        HIST_TRY(ggak_fill_u32(hist, 0u, b, g->stream));
+        HIST_TRY(ggak_degree_hist(g->d_row, g->n, g->maxdeg, b, hist, g->stream));
+        HIST_TRY(hipMemcpyAsync(h, hist, 4ull * b, hipMemcpyDeviceToHost, g->stream));
+        HIST_TRY(hipStreamSynchronize(g->stream));
         dfree(g, (void **)&hist, 4ull * b);
         /* graph.h:375-384: most populated bucket, first on ties */
         int max = -1, maxindex = -1;
@@ -386,6 +400,7 @@ int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets,
                 maxindex = (int)i;
             }
         free(h);
+        h = NULL;
         maxindex++;
         int maxdeg1 = (int)g->maxdeg + 1;
         T = (uint32_t)((maxdeg1 / (int)b) * maxindex);
