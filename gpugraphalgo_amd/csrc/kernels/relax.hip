@@ -251,13 +251,16 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     uint32_t *qout = (step & 1u) ? w.q0 : w.q1;
     uint32_t *tail = &w.ctl[GGA_CTL_TAIL(sout)];
     // batches of bsz chunks dealt round-robin, at most 64 chunks each.
-    // Ragged chunks (bmin 1): at least ~8 batches per wave when the list is
-    // long enough, for balance.  Uniform small chunks (meshes, bmin > 1): few
+    // Ragged chunks (bmin 1): ~16 batches per wave when the list is long
+    // enough, for balance; small batches also expand more chunks late in the
+    // step, when more of their owners have been re-listed and can be left to
+    // the next step (R-MAT s22 SSSP, batches per wave 4 / 8 / 16 / 32: 2.33 /
+    // 2.17 / 2.14 / 2.26 ms).  Uniform small chunks (meshes, bmin > 1): few
     // large batches -- every batch is a chain of dependent loads -- with one
     // batch per 1/1024 of the list, but at least bmin chunks
     const uint32_t waves = gridDim.x * WAVES;
     const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
-    uint32_t bsz = w.bmin > 1u ? nq >> 10 : nq / (8u * waves);
+    uint32_t bsz = w.bmin > 1u ? nq >> 10 : nq / (16u * waves);
     bsz = bsz < w.bmin ? w.bmin : bsz;
     bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
     const uint32_t nbatch = (nq + bsz - 1) / bsz;
