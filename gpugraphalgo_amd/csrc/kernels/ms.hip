@@ -137,7 +137,7 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     const uint32_t inf = ms.inf;
     const uint32_t waves = gridDim.x * WAVES;
     const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
-    uint32_t bsz = ms.bmin > 1u ? nq >> 10 : nq / (8u * waves);   // ~8 batches per wave on ragged graphs
+    uint32_t bsz = ms.bmin > 1u ? nq >> 10 : nq / (16u * waves);   // ~16 batches per wave on ragged graphs
     bsz = bsz < ms.bmin ? ms.bmin : bsz;
     bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
     const uint32_t nbatch = (nq + bsz - 1) / bsz;
