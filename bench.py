@@ -5,7 +5,7 @@ Workload (the single-GPU configuration the metric is quoted on): R-MAT
 2^22 vertices, 16 * 2^22 generated arcs (a, b, c, d = .57, .19, .19, .05,
 seed 7, duplicates and self-loops removed), uint32 weights uniform in
 [1, 100], source = the highest-degree vertex (lowest id on ties).  Split
-threshold T = 64 by default (best, or within 1% of the best, of {32, 64, 128, 256, 1024} on MI355X;
+threshold T = 128 by default (the best of {32, 64, 128, 256, 1024} on MI355X;
 --sweep times all five and reports the best).
 
 A step is one full SSSP query (distance init -> convergence) with the graph
@@ -37,7 +37,7 @@ if ROOT not in sys.path:
 METRIC = "GTEPS + % HBM roofline (1 GPU); served BFS/SSSP queries/s with p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 SWEEP = (32, 64, 128, 256, 1024)
-DEFAULT_T = 64  # best or within 1% of the best of SWEEP on MI355X (profiles/sweep_rmat22_sssp.txt)
+DEFAULT_T = 128  # best of SWEEP on MI355X (profiles/sweep_rmat22_sssp.txt)
 TRAFFIC = os.path.join(ROOT, "profiles", "k_wl_step_rmat22_sssp_traffic.txt")
 
 
