@@ -207,3 +207,13 @@ def test_cli_usage_and_errors(tmp_path):
     assert r.returncode == 1 and "Usage:" in r.stdout
     r = subprocess.run([exe, str(tmp_path / "nope.gr"), "1"], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+def test_unknown_option_names_raise():
+    """A misspelt option must not be dropped silently (ctypes would keep it as
+    a plain Python attribute)."""
+    from gpugraphalgo_amd.graph import _opts
+    o = _opts("worklist", INF, False, {"dense_min": 7, "delta": 3})
+    assert (o.dense_min, o.delta, o.mode) == (7, 3, 1)
+    with pytest.raises(TypeError, match="dens_min"):
+        _opts("worklist", INF, False, {"dens_min": 7})
