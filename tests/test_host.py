@@ -207,6 +207,14 @@ def test_cli_usage_and_errors(tmp_path):
     assert r.returncode == 1 and "Usage:" in r.stdout
     r = subprocess.run([exe, str(tmp_path / "nope.gr"), "1"], capture_output=True, text=True)
     assert r.returncode == 2
+    # a file too short for the .gr header is rejected before any device work
+    bad = tmp_path / "bad.gr"
+    bad.write_bytes(b"not a graph")
+    r = subprocess.run([exe, str(bad), "1"], capture_output=True, text=True)
+    assert r.returncode == 2 and "malformed .gr" in r.stderr
+    # program name selects the algorithm, as the reference's n_bfs.exe / n_sssp.exe
+    r = subprocess.run([exe, "--algo", "bfs"], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stdout.startswith("BFS\n")
 
 
 def test_unknown_option_names_raise():
