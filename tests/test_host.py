@@ -212,7 +212,7 @@ def test_cli_usage_and_errors(tmp_path):
     bad.write_bytes(b"not a graph")
     r = subprocess.run([exe, str(bad), "1"], capture_output=True, text=True)
     assert r.returncode == 2 and "malformed .gr" in r.stderr
-    # program name selects the algorithm, as the reference's n_bfs.exe / n_sssp.exe
+    # --algo selects the algorithm (its name line comes first, as in bfs_sssp.cu)
     r = subprocess.run([exe, "--algo", "bfs"], capture_output=True, text=True)
     assert r.returncode == 1 and r.stdout.startswith("BFS\n")
 
