@@ -185,3 +185,42 @@ def test_m_reach():
     rp = np.array([0, 2, 3, 3, 5], dtype=np.uint32)
     d = np.array([0, INF, 4, 1], dtype=np.uint32)
     assert oracle.m_reach(rp, d) == 2 + 0 + 2
+
+
+def test_oracle_routines_agree_property():
+    """Property check across the oracle's independent routines on random small
+    multigraphs (self-loops, duplicate edges, isolated vertices, weights up to
+    near-saturating): the restated serial Bellman-Ford, Dijkstra, the
+    reference's split relaxation at several thresholds and, for unit weights,
+    the queue BFS all give one answer, and verify / tightness accept it."""
+    hyp = pytest.importorskip("hypothesis")
+    st = hyp.strategies
+
+    @hyp.settings(max_examples=150, deadline=None, derandomize=True)
+    @hyp.given(n=st.integers(1, 12), data=st.data())
+    def check(n, data):
+        m = data.draw(st.integers(0, 4 * n))
+        us = np.array(data.draw(st.lists(st.integers(0, n - 1), min_size=m, max_size=m)), dtype=np.int64)
+        vs = np.array(data.draw(st.lists(st.integers(0, n - 1), min_size=m, max_size=m)), dtype=np.uint32)
+        wmax = data.draw(st.sampled_from([1, 7, 1000, 2**31]))
+        ws = np.array(data.draw(st.lists(st.integers(1, wmax), min_size=m, max_size=m)), dtype=np.uint32)
+        order = np.argsort(us, kind="stable")
+        us, vs, ws = us[order], vs[order], ws[order]
+        rp = np.zeros(n + 1, dtype=np.uint32)
+        np.add.at(rp, us + 1, 1)
+        rp = np.cumsum(rp).astype(np.uint32)
+        src = data.draw(st.integers(0, n - 1))
+        inf = data.draw(st.sampled_from([INF, oracle.REF_INF]))
+        want = oracle.dijkstra(rp, vs, ws, src, inf)
+        d, _, _ = oracle.serial_sp(rp, vs, ws, src, inf)
+        assert d.tolist() == want.tolist()
+        for T in (1, 2, 5):
+            ds, _ = oracle.split_relax(rp, vs, ws, T, src, inf)
+            assert ds.tolist() == want.tolist(), T
+        assert oracle.verify(rp, vs, ws, want, inf) == 0
+        assert oracle.tight(rp, vs, ws, src, want, inf) == 0
+        bq = oracle.bfs_queue(rp, vs, src, inf)
+        db, _, _ = oracle.serial_sp(rp, vs, None, src, inf)
+        assert bq.tolist() == db.tolist()
+
+    check()
