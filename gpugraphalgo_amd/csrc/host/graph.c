@@ -41,7 +41,8 @@ struct gga_graph {
     uint64_t *d_cnt;
     uint32_t qcap;
     uint32_t *h_pin;          /* pinned: GGA_CTL_WORDS words */
-    uint32_t epoch;
+    uint32_t epoch;           /* step number of the next query's first step (mark[] stamps are below it) */
+    int mark_dirty;           /* mark[] holds stamps of unknown step numbers: refill before the next query */
     int have_result;
     uint32_t wl_grid[2][2];   /* [bfs][count] */
     uint32_t p_grid[2][2];
@@ -585,16 +586,262 @@ static uint32_t auto_delta(const gga_graph *g, int bfs)
 uint32_t gga_graph_auto_delta(const gga_graph *g, int bfs) { return g ? auto_delta(g, bfs) : 0u; }
 int gga_graph_mesh_like(const gga_graph *g) { return g ? mesh_like(g) : 0; }
 
-static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gga_opts *user,
-                    gga_stats *st)
+/* One query on a prepared workspace: dist init, the relax loop of the chosen
+ * mode, m_reach, error bits.  `own` = the graph's own workspace: per-launch
+ * timing events and the instrumentation buffers are available (they are
+ * created or allocated on demand); the static-scratch query runs with
+ * neither and allocates nothing.  On return h_pin holds the ctl words and
+ * *adv the number of step numbers the query used (the largest ring-slot
+ * step number minus the epoch): every stamp it wrote into mark[] is below
+ * epoch + adv + 1. */
+typedef struct query_plan {
+    int bfs, mode, count, own;
+    uint32_t src, max_iters, epoch;
+} query_plan;
+
+static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const query_plan *p, gga_opts *o,
+                     gga_stats *s, uint32_t *adv)
 {
     int rc = GGA_OK;
     gga_wg_record *d_tl = NULL;
     uint32_t *d_fin = NULL;
     uint64_t *d_conv = NULL;
     uint64_t tl_bytes = 0, fin_bytes = 0, conv_bytes = 0;
+    const int bfs = p->bfs, mode = p->mode, count = p->count, own = p->own;
+    const uint32_t max_iters = p->max_iters;
+    float ktime = 0.f;
+    *adv = 0;
+
+    HIP_TRY(hipEventRecord(g->ev[0], g->stream));              /* bfs_sssp.cu:350 */
+    K_TRY(ggak_query_init(dg, dw, p->src, p->epoch, g->stream));
+    if (mode == GGA_MODE_TOPO) {
+        /* the reference's loop: changed = false; relax; read changed (bfs_sssp.cu:372-468) */
+        if (own && (rc = ensure_kev(g, 2))) goto fail;
+        uint32_t it = 0;
+        for (;;) {
+            it++;
+            g->h_pin[0] = 0;
+            HIP_TRY(hipMemcpyAsync(dw.ctl + GGA_CTL_CHANGED, g->h_pin, 4, hipMemcpyHostToDevice, g->stream));
+            if (own) HIP_TRY(hipEventRecord(g->kev[0], g->stream));
+            K_TRY(ggak_topo_step(dg, dw, bfs, count, g->stream));
+            if (own) HIP_TRY(hipEventRecord(g->kev[1], g->stream));
+            HIP_TRY(hipMemcpyAsync(g->h_pin + 1, dw.ctl + GGA_CTL_CHANGED, 4, hipMemcpyDeviceToHost, g->stream));
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            if (own) {
+                float t = 0.f;
+                HIP_TRY(hipEventElapsedTime(&t, g->kev[0], g->kev[1]));
+                ktime += t;
+            }
+            s->launches++;
+            if (!g->h_pin[1]) break;
+            if (it >= max_iters) {
+                rc = GGA_ETIMEDOUT;
+                gga_set_error("topology mode: %u iterations without convergence", it);
+                goto fail;
+            }
+        }
+        s->iterations = it;
+    } else if (mode == GGA_MODE_WORKLIST) {
+        const uint32_t grid = o->grid ? o->grid : g->tuned_grid ? g->tuned_grid : wl_grid(g, 0, bfs, count);
+        if (grid == 0) {
+            rc = GGA_EHIP;
+            gga_set_error("occupancy query failed");
+            goto fail;
+        }
+        /* steps per host check: sync_every, else 8 -- but the first batch is
+         * one longer than the last query on this graph needed (up to 32; step
+         * counts vary by one from run to run), so that repeated queries end
+         * after one check and at most two empty launches */
+        const uint32_t per = o->sync_every ? o->sync_every : 8u;
+        const uint32_t prev = g->last_steps[bfs ? 1 : 0];
+        uint32_t nb = o->sync_every ? per : (prev ? (prev < 31u ? prev + 1u : 32u) : per);
+        if (own && (rc = ensure_kev(g, 3 * (nb > per ? nb : per)))) goto fail;
+        /* instrumentation buffers (freed below, also on failure) */
+        const uint32_t tl_steps = own && o->timeline && o->timeline_cap >= grid ? o->timeline_cap / grid : 0u;
+        const uint32_t conv_n = own && o->conv_final && o->conv_rows ? o->conv_cap : 0u;
+        tl_bytes = 16ull * tl_steps * grid;
+        if (tl_steps && (rc = dmalloc(g, (void **)&d_tl, tl_bytes))) goto fail;
+        if (conv_n) {
+            fin_bytes = 4ull * g->n;
+            conv_bytes = 16ull * conv_n;
+            if ((rc = dmalloc(g, (void **)&d_fin, fin_bytes))) goto fail;
+            if ((rc = dmalloc(g, (void **)&d_conv, conv_bytes))) goto fail;
+            HIP_TRY(hipMemcpyAsync(d_fin, o->conv_final, 4ull * g->n, hipMemcpyHostToDevice, g->stream));
+            K_TRY(ggak_fill_u32((uint32_t *)d_conv, 0u, 4ull * conv_n, g->stream));
+        }
+        uint32_t step = 0;
+        for (;;) {
+            /* `nb` steps back to back: every kernel reads its list length on
+             * the device, so steps after convergence are empty no-ops */
+            for (uint32_t k = 0; k < nb; k++) {
+                const uint32_t abs_step = step + k;
+                gga_wg_record *tl = abs_step < tl_steps ? d_tl + (uint64_t)abs_step * grid : NULL;
+                if (own) HIP_TRY(hipEventRecord(g->kev[3 * k], g->stream));
+                K_TRY(ggak_wl_step(dg, dw, abs_step % 6u, bfs, count, grid, tl, g->stream));
+                if (own) HIP_TRY(hipEventRecord(g->kev[3 * k + 1], g->stream));
+                K_TRY(ggak_compact(dg, dw, abs_step % 6u, g->stream));
+                K_TRY(ggak_refill(dg, dw, abs_step % 6u, g->stream));
+                if (own) HIP_TRY(hipEventRecord(g->kev[3 * k + 2], g->stream));
+                if (abs_step < conv_n) K_TRY(ggak_conv(dw.dist, d_fin, g->n, d_conv + 2ull * abs_step, g->stream));
+            }
+            step += nb;
+            HIP_TRY(hipMemcpyAsync(g->h_pin, dw.ctl + GGA_CTL_TAIL(step % 3u), 4, hipMemcpyDeviceToHost,
+                                   g->stream));
+            HIP_TRY(hipMemcpyAsync(g->h_pin + 1, dw.ctl + GGA_CTL_FAR, 4, hipMemcpyDeviceToHost, g->stream));
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            if (own)
+                for (uint32_t k = 0; k < nb; k++) {
+                    float t = 0.f, tc = 0.f;
+                    HIP_TRY(hipEventElapsedTime(&t, g->kev[3 * k], g->kev[3 * k + 1]));
+                    HIP_TRY(hipEventElapsedTime(&tc, g->kev[3 * k + 1], g->kev[3 * k + 2]));
+                    ktime += t;
+                    s->ctime_ms += tc;
+                }
+            s->launches += nb;
+            nb = per;
+            if (g->h_pin[0] == 0 && (!dw.last || g->h_pin[1] == 0)) break;
+            if (step >= max_iters) {
+                rc = GGA_ETIMEDOUT;
+                gga_set_error("worklist mode: %u steps without convergence", step);
+                goto fail;
+            }
+        }
+        if (tl_steps) {
+            const uint32_t ns = step < tl_steps ? step : tl_steps;
+            HIP_TRY(hipMemcpy(o->timeline, d_tl, 16ull * ns * grid, hipMemcpyDeviceToHost));
+            s->timeline_records = ns * grid;
+            s->timeline_grid = grid;
+        }
+        if (conv_n) {
+            uint64_t *h = (uint64_t *)malloc(16ull * conv_n);
+            if (!h) {
+                rc = GGA_ENOMEM;
+                goto fail;
+            }
+            HIP_TRY(hipMemcpy(h, d_conv, 16ull * conv_n, hipMemcpyDeviceToHost));
+            const uint32_t ns = step < conv_n ? step : conv_n;
+            for (uint32_t k = 0; k < ns; k++) {
+                o->conv_rows[k].step = k + 1u;   /* after k + 1 relax steps (bfs_sssp.cu:377, 423) */
+                o->conv_rows[k].diff_count = (uint32_t)h[2 * k];
+                o->conv_rows[k].norm2 = sqrt((double)h[2 * k + 1]);
+            }
+            free(h);
+            s->conv_steps = ns;
+        }
+    } else { /* GGA_MODE_PERSISTENT */
+        const uint32_t maxg = wl_grid(g, 1, bfs, count);
+        /* half the grid once a query on this graph ran more than 1024 steps:
+         * such frontiers are thin, and every step ends in a grid barrier whose
+         * cost grows with the grid (pure 4096^2 grid BFS 102 -> 84 ms, SSSP
+         * 200 -> 194 ms; 1024^2 25 -> 20 ms, 35 -> 29 ms).  A 4096^2 grid
+         * with 0.5% shortcuts (80-125 steps, wide frontiers) needs the full
+         * grid: 10 vs 16 ms BFS. */
+        const int thin = g->last_steps[bfs ? 1 : 0] > 1024u;
+        uint32_t grid = o->grid ? o->grid : (thin && maxg >= 2u ? maxg / 2u : maxg);
+        if (grid > maxg) grid = maxg;
+        if (grid == 0) {
+            rc = GGA_EHIP;
+            gga_set_error("occupancy query failed");
+            goto fail;
+        }
+        HIP_TRY(hipEventRecord(g->ev[2], g->stream));
+        /* every barrier wait is bounded (5 s); a step never comes near that */
+        K_TRY(ggak_persistent(dg, dw, bfs, count, grid, max_iters, 5000u, g->stream));
+        HIP_TRY(hipEventRecord(g->ev[3], g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipEventElapsedTime(&ktime, g->ev[2], g->ev[3]));
+        s->launches = 1;
+    }
+    HIP_TRY(hipEventRecord(g->ev[1], g->stream));              /* bfs_sssp.cu:469 */
+    K_TRY(ggak_reach(dg, dw, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->h_pin, dw.ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(hipEventElapsedTime(&s->time_ms, g->ev[0], g->ev[1]));
+    {
+        const uint32_t *c = g->h_pin;
+        for (uint32_t i = 0; i < 3; i++) {
+            const uint32_t a = c[GGA_CTL_ITER(i)] - p->epoch;
+            if (a > *adv) *adv = a;
+        }
+        if (mode != GGA_MODE_TOPO) {
+            s->iterations = c[GGA_CTL_STEPS];
+            if (c[GGA_CTL_ERR] & GGA_DEVERR_QFULL) {
+                rc = GGA_ERANGE;
+                gga_set_error("worklist overflow (capacity %u)", dw.qcap);
+                goto fail;
+            }
+            if (c[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
+                rc = GGA_ETIMEDOUT;
+                gga_set_error("persistent kernel: grid barrier spin limit");
+                goto fail;
+            }
+            if (mode == GGA_MODE_PERSISTENT) {
+                uint32_t left = c[GGA_CTL_TAIL(s->iterations % 3u)];
+                if (left) {
+                    rc = GGA_ETIMEDOUT;
+                    gga_set_error("persistent kernel: %u steps without convergence", s->iterations);
+                    goto fail;
+                }
+            }
+        }
+        const uint64_t *c64 = (const uint64_t *)(c + GGA_CTL_CNT);
+        s->edges_inspected = c64[0];
+        s->chunks_expanded = c64[1];
+        s->relaxations = c64[2];
+        const uint64_t *r64 = (const uint64_t *)(c + GGA_CTL_RED);
+        s->reached = (uint32_t)r64[0];
+        s->m_reach = r64[1];
+    }
+    s->ktime_ms = ktime;
+    s->sktime_ms = 0.f;
+    g->last_steps[bfs ? 1 : 0] = s->iterations;
+    dfree(g, (void **)&d_tl, tl_bytes);
+    dfree(g, (void **)&d_fin, fin_bytes);
+    dfree(g, (void **)&d_conv, conv_bytes);
+    return GGA_OK;
+fail:
+    hipStreamSynchronize(g->stream);
+    dfree(g, (void **)&d_tl, tl_bytes);
+    dfree(g, (void **)&d_fin, fin_bytes);
+    dfree(g, (void **)&d_conv, conv_bytes);
+    return rc;
+}
+
+/* Mode, near-far width and step cap of a query, from the options and what
+ * the graph has learned (shared by gga_bfs / gga_sssp and the static-scratch
+ * query). */
+static void plan_query(const gga_graph *g, int bfs, const gga_opts *o, query_plan *p, uint32_t *delta)
+{
+    p->bfs = bfs;
+    p->count = o->count ? 1 : 0;
+    int mode = o->mode;
+    if (mode == GGA_MODE_AUTO || mode < 0 || mode > GGA_MODE_AUTO) mode = choose_mode(g, bfs);
+    p->mode = mode;
+    /* Bellman-Ford needs at most n + 1 steps (graph.h:122); near-far adds
+     * bucket changes and one empty step per jump over empty buckets, so its
+     * default cap is only a guard */
+    const uint32_t d = o->delta == 0xFFFFFFFFu ? 0u : (o->delta ? o->delta : auto_delta(g, bfs));
+    const int near_far = !bfs && (mode == GGA_MODE_WORKLIST || mode == GGA_MODE_PERSISTENT) && d != 0u;
+    *delta = near_far ? d : 0u;
+    p->max_iters = o->max_iters ? o->max_iters : (g->n + 1u);
+    if (!o->max_iters && near_far)
+        p->max_iters = (uint64_t)64u * (g->n + 1u) < 0xFFFFFFF0ull ? 64u * (g->n + 1u) : 0xFFFFFFF0u;
+}
+
+static uint32_t dense_default(const gga_graph *g, const gga_opts *o)
+{
+    return o->dense_min ? o->dense_min
+                        : g->tuned_dense ? g->tuned_dense : (g->n / 64u > 4096u ? g->n / 64u : 4096u);
+}
+
+static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gga_opts *user,
+                    gga_stats *st)
+{
+    int rc = GGA_OK;
     gga_opts o;
     gga_stats s;
+    query_plan p;
+    uint32_t delta = 0, adv = 0;
     memset(&s, 0, sizeof s);
     if (user) o = *user;
     else gga_opts_init(&o);
@@ -609,231 +856,43 @@ static int traverse(gga_graph *g, uint32_t src, int bfs, uint32_t *out, const gg
         if (rc) return rc;
     }
     const uint32_t inf = o.inf ? o.inf : GGA_INF;
-    int mode = o.mode;
-    if (mode == GGA_MODE_AUTO || mode < 0 || mode > GGA_MODE_AUTO) mode = choose_mode(g, bfs);
-    /* Bellman-Ford needs at most n + 1 steps (graph.h:122); near-far adds
-     * bucket changes and one empty step per jump over empty buckets, so its
-     * default cap is only a guard */
-    const uint32_t delta = o.delta == 0xFFFFFFFFu ? 0u : (o.delta ? o.delta : auto_delta(g, bfs));
-    const int near_far = !bfs && (mode == GGA_MODE_WORKLIST || mode == GGA_MODE_PERSISTENT) && delta != 0u;
-    uint32_t max_iters = o.max_iters ? o.max_iters : (g->n + 1u);
-    if (!o.max_iters && near_far)
-        max_iters = (uint64_t)64u * (g->n + 1u) < 0xFFFFFFF0ull ? 64u * (g->n + 1u) : 0xFFFFFFF0u;
-    const int count = o.count ? 1 : 0;
+    plan_query(g, bfs, &o, &p, &delta);
+    p.src = src;
+    p.own = 1;
     HIP_TRY(hipSetDevice(g->device));
     if ((rc = ensure_queues(g, g->C))) return rc;
-    /* step stamps keep growing across queries; refill before they wrap */
-    if (g->epoch > 0xF0000000u) {
+    /* Queue-step stamps in mark[] outlive their query: each query's step
+     * numbers start past every stamp written before (g->epoch).  Refill
+     * before the numbers wrap, and after a failed query, whose last step
+     * number is unknown. */
+    if (g->epoch > 0xF0000000u || g->mark_dirty) {
         K_TRY(ggak_fill_u32(g->d_mark, 0xFFFFFFFFu, g->n, g->stream));
         g->epoch = 0;
+        g->mark_dirty = 0;
     }
+    p.epoch = g->epoch;
     gga_dev_graph dg = dev_graph(g);
     gga_dev_work dw = dev_work(g, inf);
-    dw.dense_min = o.dense_min ? o.dense_min
-                               : g->tuned_dense ? g->tuned_dense : (g->n / 64u > 4096u ? g->n / 64u : 4096u);
+    dw.dense_min = dense_default(g, &o);
     dw.bmin = mesh_like(g) ? GGA_MESH_BMIN : 1u;
     /* near-far buckets: SSSP only (BFS levels are already one bucket per level) */
-    if (near_far) {
+    if (delta) {
         if (!g->d_last && (rc = dmalloc(g, (void **)&g->d_last, 4ull * g->n))) return rc;
         if (!g->d_tmin && (rc = dmalloc(g, (void **)&g->d_tmin, 4ull * GGA_TILES((uint64_t)g->n)))) return rc;
         dw.last = g->d_last;
         dw.tmin = g->d_tmin;
         dw.delta = delta;
     }
-    s.mode = mode;
-    float ktime = 0.f;
-
-    HIP_TRY(hipEventRecord(g->ev[0], g->stream));              /* bfs_sssp.cu:350 */
-    K_TRY(ggak_query_init(dg, dw, src, g->epoch, g->stream));
-    if (mode == GGA_MODE_TOPO) {
-        /* the reference's loop: changed = false; relax; read changed (bfs_sssp.cu:372-468) */
-        if ((rc = ensure_kev(g, 2))) goto fail;
-        uint32_t it = 0;
-        for (;;) {
-            it++;
-            g->h_pin[0] = 0;
-            HIP_TRY(hipMemcpyAsync(g->d_ctl + GGA_CTL_CHANGED, g->h_pin, 4, hipMemcpyHostToDevice, g->stream));
-            HIP_TRY(hipEventRecord(g->kev[0], g->stream));
-            K_TRY(ggak_topo_step(dg, dw, bfs, count, g->stream));
-            HIP_TRY(hipEventRecord(g->kev[1], g->stream));
-            HIP_TRY(hipMemcpyAsync(g->h_pin + 1, g->d_ctl + GGA_CTL_CHANGED, 4, hipMemcpyDeviceToHost, g->stream));
-            HIP_TRY(hipStreamSynchronize(g->stream));
-            float t = 0.f;
-            HIP_TRY(hipEventElapsedTime(&t, g->kev[0], g->kev[1]));
-            ktime += t;
-            s.launches++;
-            if (!g->h_pin[1]) break;
-            if (it >= max_iters) {
-                rc = GGA_ETIMEDOUT;
-                gga_set_error("topology mode: %u iterations without convergence", it);
-                goto fail;
-            }
-        }
-        s.iterations = it;
-    } else if (mode == GGA_MODE_WORKLIST) {
-        const uint32_t grid = o.grid ? o.grid : g->tuned_grid ? g->tuned_grid : wl_grid(g, 0, bfs, count);
-        if (grid == 0) {
-            rc = GGA_EHIP;
-            gga_set_error("occupancy query failed");
-            goto fail;
-        }
-        /* steps per host check: sync_every, else 8 -- but the first batch is
-         * one longer than the last query on this graph needed (up to 32; step
-         * counts vary by one from run to run), so that repeated queries end
-         * after one check and at most two empty launches */
-        const uint32_t per = o.sync_every ? o.sync_every : 8u;
-        const uint32_t prev = g->last_steps[bfs ? 1 : 0];
-        uint32_t nb = o.sync_every ? per : (prev ? (prev < 31u ? prev + 1u : 32u) : per);
-        if ((rc = ensure_kev(g, 3 * (nb > per ? nb : per)))) goto fail;
-        /* instrumentation buffers (freed below, also on failure) */
-        const uint32_t tl_steps = o.timeline && o.timeline_cap >= grid ? o.timeline_cap / grid : 0u;
-        const uint32_t conv_n = o.conv_final && o.conv_rows ? o.conv_cap : 0u;
-        tl_bytes = 16ull * tl_steps * grid;
-        if (tl_steps && (rc = dmalloc(g, (void **)&d_tl, tl_bytes))) goto fail;
-        if (conv_n) {
-            fin_bytes = 4ull * g->n;
-            conv_bytes = 16ull * conv_n;
-            if ((rc = dmalloc(g, (void **)&d_fin, fin_bytes))) goto fail;
-            if ((rc = dmalloc(g, (void **)&d_conv, conv_bytes))) goto fail;
-            HIP_TRY(hipMemcpyAsync(d_fin, o.conv_final, 4ull * g->n, hipMemcpyHostToDevice, g->stream));
-            K_TRY(ggak_fill_u32((uint32_t *)d_conv, 0u, 4ull * conv_n, g->stream));
-        }
-        uint32_t step = 0;
-        for (;;) {
-            /* `per` steps back to back: every kernel reads its list length on
-             * the device, so steps after convergence are empty no-ops */
-            for (uint32_t k = 0; k < nb; k++) {
-                const uint32_t abs_step = step + k;
-                gga_wg_record *tl = abs_step < tl_steps ? d_tl + (uint64_t)abs_step * grid : NULL;
-                HIP_TRY(hipEventRecord(g->kev[3 * k], g->stream));
-                K_TRY(ggak_wl_step(dg, dw, abs_step % 6u, bfs, count, grid, tl, g->stream));
-                HIP_TRY(hipEventRecord(g->kev[3 * k + 1], g->stream));
-                K_TRY(ggak_compact(dg, dw, abs_step % 6u, g->stream));
-                K_TRY(ggak_refill(dg, dw, abs_step % 6u, g->stream));
-                HIP_TRY(hipEventRecord(g->kev[3 * k + 2], g->stream));
-                if (abs_step < conv_n) K_TRY(ggak_conv(g->d_dist, d_fin, g->n, d_conv + 2ull * abs_step, g->stream));
-            }
-            step += nb;
-            HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl + GGA_CTL_TAIL(step % 3u), 4, hipMemcpyDeviceToHost,
-                                   g->stream));
-            HIP_TRY(hipMemcpyAsync(g->h_pin + 1, g->d_ctl + GGA_CTL_FAR, 4, hipMemcpyDeviceToHost, g->stream));
-            HIP_TRY(hipStreamSynchronize(g->stream));
-            for (uint32_t k = 0; k < nb; k++) {
-                float t = 0.f, tc = 0.f;
-                HIP_TRY(hipEventElapsedTime(&t, g->kev[3 * k], g->kev[3 * k + 1]));
-                HIP_TRY(hipEventElapsedTime(&tc, g->kev[3 * k + 1], g->kev[3 * k + 2]));
-                ktime += t;
-                s.ctime_ms += tc;
-            }
-            s.launches += nb;
-            nb = per;
-            if (g->h_pin[0] == 0 && (!dw.last || g->h_pin[1] == 0)) break;
-            if (step >= max_iters) {
-                rc = GGA_ETIMEDOUT;
-                gga_set_error("worklist mode: %u steps without convergence", step);
-                goto fail;
-            }
-        }
-        if (tl_steps) {
-            const uint32_t ns = step < tl_steps ? step : tl_steps;
-            HIP_TRY(hipMemcpy(o.timeline, d_tl, 16ull * ns * grid, hipMemcpyDeviceToHost));
-            s.timeline_records = ns * grid;
-            s.timeline_grid = grid;
-        }
-        if (conv_n) {
-            uint64_t *h = (uint64_t *)malloc(16ull * conv_n);
-            if (!h) {
-                rc = GGA_ENOMEM;
-                goto fail;
-            }
-            HIP_TRY(hipMemcpy(h, d_conv, 16ull * conv_n, hipMemcpyDeviceToHost));
-            const uint32_t ns = step < conv_n ? step : conv_n;
-            for (uint32_t k = 0; k < ns; k++) {
-                o.conv_rows[k].step = k + 1u;   /* after k + 1 relax steps (bfs_sssp.cu:377, 423) */
-                o.conv_rows[k].diff_count = (uint32_t)h[2 * k];
-                o.conv_rows[k].norm2 = sqrt((double)h[2 * k + 1]);
-            }
-            free(h);
-            s.conv_steps = ns;
-        }
-    } else { /* GGA_MODE_PERSISTENT */
-        const uint32_t maxg = wl_grid(g, 1, bfs, count);
-        /* half the grid once a query on this graph ran more than 1024 steps:
-         * such frontiers are thin, and every step ends in a grid barrier whose
-         * cost grows with the grid (pure 4096^2 grid BFS 102 -> 84 ms, SSSP
-         * 200 -> 194 ms; 1024^2 25 -> 20 ms, 35 -> 29 ms).  A 4096^2 grid
-         * with 0.5% shortcuts (80-125 steps, wide frontiers) needs the full
-         * grid: 10 vs 16 ms BFS. */
-        const int thin = g->last_steps[bfs ? 1 : 0] > 1024u;
-        uint32_t grid = o.grid ? o.grid : (thin && maxg >= 2u ? maxg / 2u : maxg);
-        if (grid > maxg) grid = maxg;
-        if (grid == 0) {
-            rc = GGA_EHIP;
-            gga_set_error("occupancy query failed");
-            goto fail;
-        }
-        if ((rc = ensure_kev(g, 2))) goto fail;
-        HIP_TRY(hipEventRecord(g->kev[0], g->stream));
-        /* every barrier wait is bounded (5 s); a step never comes near that */
-        K_TRY(ggak_persistent(dg, dw, bfs, count, grid, max_iters, 5000u, g->stream));
-        HIP_TRY(hipEventRecord(g->kev[1], g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
-        HIP_TRY(hipEventElapsedTime(&ktime, g->kev[0], g->kev[1]));
-        s.launches = 1;
-    }
-    HIP_TRY(hipEventRecord(g->ev[1], g->stream));              /* bfs_sssp.cu:469 */
-    K_TRY(ggak_reach(dg, dw, g->stream));
-    HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    HIP_TRY(hipEventElapsedTime(&s.time_ms, g->ev[0], g->ev[1]));
-    {
-        const uint32_t *c = g->h_pin;
-        if (mode != GGA_MODE_TOPO) {
-            s.iterations = c[GGA_CTL_STEPS];
-            if (c[GGA_CTL_ERR] & GGA_DEVERR_QFULL) {
-                rc = GGA_ERANGE;
-                gga_set_error("worklist overflow (capacity %u)", g->qcap);
-                goto fail;
-            }
-            if (c[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
-                rc = GGA_ETIMEDOUT;
-                gga_set_error("persistent kernel: grid barrier spin limit");
-                goto fail;
-            }
-            if (mode == GGA_MODE_PERSISTENT) {
-                uint32_t left = c[GGA_CTL_TAIL(s.iterations % 3u)];
-                if (left) {
-                    rc = GGA_ETIMEDOUT;
-                    gga_set_error("persistent kernel: %u steps without convergence", s.iterations);
-                    goto fail;
-                }
-            }
-        }
-        const uint64_t *c64 = (const uint64_t *)(c + GGA_CTL_CNT);
-        s.edges_inspected = c64[0];
-        s.chunks_expanded = c64[1];
-        s.relaxations = c64[2];
-        const uint64_t *r64 = (const uint64_t *)(c + GGA_CTL_RED);
-        s.reached = (uint32_t)r64[0];
-        s.m_reach = r64[1];
-    }
-    s.ktime_ms = ktime;
-    s.sktime_ms = 0.f;
-    g->epoch += s.iterations + 2u;
+    s.mode = p.mode;
+    rc = run_query(g, dg, dw, &p, &o, &s, &adv);
+    if (rc) goto fail;
+    g->epoch = p.epoch + adv + 1u;
     g->have_result = 1;
-    g->last_steps[bfs ? 1 : 0] = s.iterations;
     if (out) HIP_TRY(hipMemcpy(out, g->d_dist, 4ull * g->n, hipMemcpyDeviceToHost));   /* bfs_sssp.cu:489 */
-    dfree(g, (void **)&d_tl, tl_bytes);
-    dfree(g, (void **)&d_fin, fin_bytes);
-    dfree(g, (void **)&d_conv, conv_bytes);
     if (st) *st = s;
     return GGA_OK;
 fail:
-    hipStreamSynchronize(g->stream);
-    dfree(g, (void **)&d_tl, tl_bytes);
-    dfree(g, (void **)&d_fin, fin_bytes);
-    dfree(g, (void **)&d_conv, conv_bytes);
-    g->epoch += max_iters < 0x10000000u ? max_iters + 2u : 0x10000000u;
+    g->mark_dirty = 1;
     g->have_result = 0;
     if (st) *st = s;
     return rc;

@@ -37,6 +37,8 @@ struct gga_ms {
     uint64_t *d_sum, *h_sum;  /* 32 lane fingerprints */
     float ktime_ms;
     uint32_t iterations;
+    int failed;               /* a launch reported a device error: lists and barrier
+                                 counts are not trusted any more (free the engine) */
 };
 
 #define HIP_TRY(expr)                                                                      \
@@ -224,9 +226,29 @@ uint64_t gga_ms_graph_bytes(gga_graph *g)
     return b;
 }
 
+/* Device error bits of the last read-back -> error code; either error
+ * leaves the engine's lists (a step interrupted by a barrier timeout) and
+ * barrier counts unusable, so the engine refuses further work. */
+static int ms_errbits(gga_ms *e)
+{
+    const uint32_t err = e->h_pin[GGA_CTL_ERR];
+    if (!(err & (GGA_DEVERR_QFULL | GGA_DEVERR_SPIN))) return GGA_OK;
+    e->failed = 1;
+    if (err & GGA_DEVERR_QFULL) {
+        gga_set_error("multi-source list overflow (capacity %u)", e->d.qcap);
+        return GGA_ERANGE;
+    }
+    gga_set_error("multi-source persistent kernel: grid barrier spin limit");
+    return GGA_ETIMEDOUT;
+}
+
 static int ms_check(gga_ms *e)
 {
     gga_graph_view_t cur;
+    if (e->failed) {
+        gga_set_error("engine failed on an earlier device error; free it and create another");
+        return GGA_EINVAL;
+    }
     int rc = gga_graph_view(e->g, &cur);
     if (rc) return rc;
     if (cur.split_gen != e->v.split_gen) {
@@ -313,15 +335,7 @@ static int ms_readback(gga_ms *e)
     int rc = GGA_OK;
     HIP_TRY(hipMemcpyAsync(e->h_pin, e->d.ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, e->v.stream));
     HIP_TRY(hipStreamSynchronize(e->v.stream));
-    if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_QFULL) {
-        gga_set_error("multi-source list overflow (capacity %u)", e->d.qcap);
-        return GGA_ERANGE;
-    }
-    if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
-        gga_set_error("multi-source persistent kernel: grid barrier spin limit");
-        return GGA_ETIMEDOUT;
-    }
-    return GGA_OK;
+    return ms_errbits(e);
 fail:
     return rc;
 }
@@ -389,10 +403,7 @@ int gga_ms_collect(gga_ms *e, uint32_t *live_out)
     HIP_TRY(hipEventSynchronize(e->ev[3]));
     const uint32_t ran = e->pending;
     e->pending = 0;
-    if (e->h_pin[GGA_CTL_ERR] & GGA_DEVERR_QFULL) {
-        gga_set_error("multi-source list overflow (capacity %u)", e->d.qcap);
-        return GGA_ERANGE;
-    }
+    if ((rc = ms_errbits(e))) return rc;
     e->steps_seen = e->h_pin[GGA_CTL_STEPS];
     ms_slot(e, &live, &tail);
     HIP_TRY(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));

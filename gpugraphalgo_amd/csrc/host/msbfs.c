@@ -251,14 +251,21 @@ int gga_multi_bfs(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *out,
     HIP_TRY(hipEventElapsedTime(&r.time_ms, s->ev[0], s->ev[1]));
     {
         const uint32_t *c = s->h_pin;
+        /* a launch that left a barrier early or overflowed leaves state the
+         * next launch must not inherit: drop the cached state (rebuilt on
+         * the next call) */
         if (c[GGA_CTL_ERR] & GGA_DEVERR_SPIN) {
             gga_set_error("multi-source BFS: grid barrier spin limit");
             rc = GGA_ETIMEDOUT;
+            gga_msbfs_release(s);
+            *gga_graph_msbfs_slot(g) = NULL;
             goto fail;
         }
         if (c[GGA_CTL_ERR] & GGA_DEVERR_QFULL) {
             gga_set_error("multi-source BFS: push list overflow");
             rc = GGA_ERANGE;
+            gga_msbfs_release(s);
+            *gga_graph_msbfs_slot(g) = NULL;
             goto fail;
         }
         if (!c[GGA_CTL_DONE]) {

@@ -231,9 +231,12 @@ __global__ __launch_bounds__(BLOCK) void k_msbfs_clear(gga_dev_graph g, gga_dev_
         b.front[i] = 0ull;
         b.next[i] = 0ull;
     }
+    // every control word but the barrier generation (each launch reads that
+    // at its start): error bits and the arrival counts of both barrier levels
+    // start at zero, also after a launch that left a barrier on a timeout
     if (blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < GGA_CTL_WORDS; i += blockDim.x)
-            if (i < GGA_CTL_BAR || i >= GGA_CTL_BARG(0) + GGA_BAR_GROUPS * GGA_CTL_LINE) b.ctl[i] = 0u;
+            if (i != GGA_CTL_BARGEN) b.ctl[i] = 0u;
 }
 
 // lane i starts at src[i]: level 0 there; reached / out-edge totals seeded

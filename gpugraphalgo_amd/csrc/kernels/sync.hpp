@@ -56,6 +56,7 @@ __device__ __forceinline__ void grid_barrier(uint32_t *ctl, uint32_t nblocks, ui
             __hip_atomic_store(&ctl[GGA_CTL_BARGEN], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t t0 = wall_clock64();
+            uint32_t polls = 0;
             while (__hip_atomic_load(&ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 __builtin_amdgcn_s_sleep(2);
                 if (wall_clock64() - t0 > spin_ticks) {
@@ -63,6 +64,12 @@ __device__ __forceinline__ void grid_barrier(uint32_t *ctl, uint32_t nblocks, ui
                                           __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
+                // once any workgroup has given up (error bit set), this
+                // barrier will not be released: leave now instead of
+                // waiting out the full spin limit at every later barrier
+                if ((++polls & 63u) == 0u &&
+                    __hip_atomic_load(&ctl[GGA_CTL_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }

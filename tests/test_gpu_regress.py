@@ -1,0 +1,128 @@
+"""GPU regression tests: state that outlives one query, and the device verifier.
+
+1. Queue-step stamps (mark[]) stay in the graph's workspace between queries;
+   a near-far query with a narrow bucket runs empty steps at every threshold
+   jump, so its step numbers run ahead of its non-empty step count.  Every
+   later query on the same handle (plain worklist, persistent, near-far
+   again) must still equal the oracle.
+2. The device verifier (k_verify_edges / k_verify_vertices behind
+   gga_graph_verify, replacing verifysolution, myutils.h:1-18) must count
+   exactly what the oracle's restatement counts on distances with planted
+   errors: too small (tightness), too large (fixpoint), a wrong source.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import gpugraphalgo_amd as gga
+from gpugraphalgo_amd import _lib
+import oracle
+
+pytestmark = pytest.mark.gpu
+INF = gga.INF
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _h2d(dptr: int, arr: np.ndarray) -> None:
+    """Host -> device copy into a library-owned device buffer (test only)."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+    arr = np.ascontiguousarray(arr, dtype=np.uint32)
+    assert hip.hipMemcpy(dptr, arr.ctypes.data, arr.nbytes, 1) == 0  # hipMemcpyHostToDevice
+
+
+def test_stamps_do_not_leak_across_queries(gpu):
+    g = gga.gen_grid(40, 50, shortcut_frac=0.0, seed=5, wlo=1, whi=1000)
+    srcs = [0, 777, 1999, 1234, 3]
+    want = {s: oracle.dijkstra(g.row_ptr, g.dst, g.w, s) for s in srcs}
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=4)
+        for rnd in range(2):
+            for s in srcs:
+                # delta = 1 below every weight: a threshold jump (an empty
+                # step) after nearly every bucket
+                announce(f"round {rnd} src={s} near-far delta=1")
+                d, st = dg.sssp(s, mode="worklist", delta=1, dense_min=0xFFFFFFFF)
+                np.testing.assert_array_equal(d, want[s], err_msg=f"near-far src={s}")
+                assert st["iterations"] > 2
+                for t in srcs:
+                    for mode in ("worklist", "persistent"):
+                        announce(f"  then {mode} src={t}")
+                        d, _ = dg.sssp(t, mode=mode, delta=0xFFFFFFFF, dense_min=0xFFFFFFFF)
+                        np.testing.assert_array_equal(d, want[t], err_msg=f"{mode} src={t} after near-far")
+                    d, _ = dg.bfs(t, mode="worklist", dense_min=0xFFFFFFFF)
+                    np.testing.assert_array_equal(d, oracle.bfs_queue(g.row_ptr, g.dst, t))
+
+
+def test_failed_query_does_not_poison_the_next(gpu):
+    """A query stopped by its step cap (GGA_ETIMEDOUT) leaves stamps of
+    unknown step numbers; the next query on the handle is still exact."""
+    g = gga.gen_grid(30, 30, shortcut_frac=0.0, seed=2, wlo=1, whi=50)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=4)
+        for mode in ("worklist", "persistent"):
+            announce(f"capped {mode}")
+            with pytest.raises(_lib.GgaError, match="GGA_ETIMEDOUT"):
+                dg.sssp(0, mode=mode, max_iters=3, delta=0xFFFFFFFF, dense_min=0xFFFFFFFF)
+            for s in (0, 450, 899):
+                d, _ = dg.sssp(s, mode="worklist", delta=0xFFFFFFFF, dense_min=0xFFFFFFFF)
+                np.testing.assert_array_equal(d, oracle.dijkstra(g.row_ptr, g.dst, g.w, s))
+
+
+def _plant(rng, dist, inf, k_small, k_large):
+    d = dist.copy()
+    reached = np.flatnonzero((d != inf) & (d > 1))
+    pick = rng.choice(reached, size=k_small + k_large, replace=False)
+    d[pick[:k_small]] -= 1                       # too small: no tight predecessor
+    d[pick[k_small:]] += 5                       # too large: an edge relaxes it
+    return d
+
+
+@pytest.mark.parametrize("bfs", [False, True])
+def test_device_verifier_counts_planted_errors(gpu, bfs):
+    g = gga.gen_rmat(12, 8, seed=21, wlo=1, whi=100)
+    w = None if bfs else g.w
+    src = g.maxdeg()[1]
+    rng = np.random.default_rng(4)
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=16)
+        for case in ("clean", "small", "large", "mixed", "source", "unreached"):
+            run = dg.bfs if bfs else dg.sssp
+            d, _ = run(src, mode="worklist")
+            if case == "small":
+                d = _plant(rng, d, INF, 7, 0)
+            elif case == "large":
+                d = _plant(rng, d, INF, 0, 9)
+            elif case == "mixed":
+                d = _plant(rng, d, INF, 5, 11)
+            elif case == "source":
+                d = d.copy()
+                d[src] = 3
+            elif case == "unreached":
+                d = d.copy()
+                reached = np.flatnonzero((d != INF) & (np.arange(g.n) != src))
+                d[rng.choice(reached, size=6, replace=False)] = INF
+            ptr = _lib.load().gga_graph_dist_device(dg._h)
+            assert ptr, "no device result"
+            _h2d(ptr, d)
+            announce(f"verify {'bfs' if bfs else 'sssp'} case={case}")
+            r = dg.verify(src, bfs=bfs)
+            nerr = oracle.verify(g.row_ptr, g.dst, w, d)
+            bad = oracle.tight(g.row_ptr, g.dst, w, src, d)
+            assert r["nerr"] == nerr, (case, r, nerr)
+            assert r["src_ok"] == bool(d[src] == 0), case
+            assert r["untight"] + (0 if r["src_ok"] else 1) == bad, (case, r, bad)
+            assert r["reached"] == int((d != INF).sum())
+            host = gga.verify(g, src, d, bfs=bfs)
+            assert (host["nerr"], host["untight"], host["src_ok"]) == (r["nerr"], r["untight"], r["src_ok"])
+            if case == "clean":
+                assert nerr == 0 and bad == 0
+            else:
+                assert nerr + bad > 0, f"planted case {case} went unnoticed by the oracle"
