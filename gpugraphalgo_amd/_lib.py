@@ -41,7 +41,7 @@ class Csr(ctypes.Structure):
 class SplitStats(ctypes.Structure):
     _fields_ = [("nnode", u32), ("nedge", u32), ("ns_nnode", u32), ("ns_nedge", u32),
                 ("extranode", u32), ("maxdeg", u32), ("level", i32), ("bucket", u32), ("mda", u32),
-                ("max_chunk", u32), ("split_ms", ctypes.c_float)]
+                ("max_chunk", u32), ("split_ms", ctypes.c_float), ("exec_chunks", u32)]
 
 
 class WgRecord(ctypes.Structure):
@@ -89,7 +89,8 @@ class QueryResult(ctypes.Structure):
 class ServerCounters(ctypes.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("rejected", u64), ("rounds", u64),
                 ("evictions", u64), ("engines_created", u64), ("bytes_in_use", u64),
-                ("peak_bytes", u64), ("engines_resident", u32), ("pad", u32)]
+                ("peak_bytes", u64), ("engines_resident", u32), ("peak_running", u32),
+                ("joined", u64)]
 
 
 class TuneResult(ctypes.Structure):
@@ -139,6 +140,7 @@ SIGNATURES = [
     ("gga_bfs", ctypes.c_int, [vp, u32, vp, P(Opts), P(Stats)]),
     ("gga_sssp", ctypes.c_int, [vp, u32, vp, P(Opts), P(Stats)]),
     ("gga_graph_copy_dist", ctypes.c_int, [vp, vp]),
+    ("gga_write_degree_dist", ctypes.c_int, [ctypes.c_char_p, u32, vp]),
     ("gga_graph_dist_device", vp, [vp]),
     ("gga_verify", ctypes.c_int, [u32, vp, vp, vp, u32, vp, u32, P(VerifyResult)]),
     ("gga_graph_verify", ctypes.c_int, [vp, u32, ctypes.c_int, u32, P(VerifyResult)]),

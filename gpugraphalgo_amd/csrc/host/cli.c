@@ -15,6 +15,8 @@
  *   --out FILE          distance file (same as the positional outfile)
  *   --result FILE       CSV row file (default result.vi, myutils.h:89)
  *   --device d          HIP device (default 0)
+ *   --dd                also write DD_<graph>.txt, the out-degree distribution
+ *                       (printfDegreeDistribution, graph.h:98-114; bfs_sssp.cu:196-200)
  * Output: the reference's stdout lines, the Ary2File distance file, and one
  * 17-field writeResult row.
  */
@@ -44,7 +46,7 @@ int main(int argc, char **argv)
     int npos = 0;
     const char *algo = NULL, *mode_s = "auto", *outfile = NULL, *result = "result.vi";
     uint32_t mda = 0, src = 0, inf = GGA_INF;
-    int device = 0;
+    int device = 0, want_dd = 0;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         if (!strcmp(a, "--algo") && i + 1 < argc) algo = argv[++i];
@@ -55,6 +57,7 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--out") && i + 1 < argc) outfile = argv[++i];
         else if (!strcmp(a, "--result") && i + 1 < argc) result = argv[++i];
         else if (!strcmp(a, "--device") && i + 1 < argc) device = atoi(argv[++i]);
+        else if (!strcmp(a, "--dd")) want_dd = 1;
         else if (npos < 3) pos[npos++] = a;
     }
     if (!algo) algo = strstr(argv[0], "bfs") ? "bfs" : "sssp";
@@ -84,7 +87,22 @@ int main(int argc, char **argv)
     printf("%s\n%s\n", gname, pos[0]);
 
     gga_graph *g = NULL;
-    int rc = gga_graph_load_gr(pos[0], device, &g);
+    int rc;
+    if (want_dd) {
+        /* the histogram needs the host CSR: read it here, write DD_<graph>.txt
+         * (before the split, as bfs_sssp.cu:196-200), then upload it */
+        gga_csr c;
+        rc = gga_csr_read_gr(pos[0], &c);
+        if (!rc) {
+            char ddname[256];
+            snprintf(ddname, sizeof ddname, "DD_%s.txt", gname);
+            rc = gga_write_degree_dist(ddname, c.n, c.row_ptr);
+            if (!rc) rc = gga_graph_from_csr32(c.n, c.m, c.row_ptr, c.dst, c.has_weights ? c.w : NULL, device, &g);
+            gga_csr_free(&c);
+        }
+    } else {
+        rc = gga_graph_load_gr(pos[0], device, &g);
+    }
     if (rc) {
         fprintf(stderr, "load %s: %s (%s)\n", pos[0], gga_strerror(rc), gga_last_error());
         return 2;

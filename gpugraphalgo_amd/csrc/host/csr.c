@@ -271,6 +271,31 @@ int gga_write_result_row(const char *path, const gga_result_row *r)
     return fclose(f) == 0 ? GGA_OK : GGA_EIO;
 }
 
+int gga_write_degree_dist(const char *path, uint32_t n, const uint32_t *row_ptr)
+{
+    if (!path || !row_ptr) return GGA_EINVAL;
+    uint32_t maxd = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const uint32_t d = row_ptr[v + 1] - row_ptr[v];
+        if (d > maxd) maxd = d;
+    }
+    /* dd[deg] = vertices with that out-degree, deg = 0 .. maxdeg (graph.h:98-103) */
+    uint32_t *dd = (uint32_t *)calloc((size_t)maxd + 1u, sizeof *dd);
+    if (!dd) return GGA_ENOMEM;
+    for (uint32_t v = 0; v < n; v++) dd[row_ptr[v + 1] - row_ptr[v]]++;
+    FILE *f = fopen(path, "w");
+    if (!f) {
+        gga_set_error("fopen %s: %s", path, strerror(errno));
+        free(dd);
+        return GGA_EIO;
+    }
+    /* "deg,count" for every degree that occurs, ascending (graph.h:105-110) */
+    for (uint64_t d = 0; d <= maxd; d++)
+        if (dd[d]) fprintf(f, "%llu,%u\n", (unsigned long long)d, dd[d]);
+    free(dd);
+    return fclose(f) == 0 ? GGA_OK : GGA_EIO;
+}
+
 void gga_graph_name(const char *path, char *out, size_t cap)
 {
     if (!out || cap == 0) return;

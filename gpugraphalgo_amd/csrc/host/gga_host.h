@@ -37,6 +37,13 @@ uint32_t gga_graph_default_grid(gga_graph *g, int bfs);
 void **gga_graph_msbfs_slot(gga_graph *g);
 void gga_msbfs_release(void *state);
 
+/* Cooperative launches (persistent kernels) of one process run one at a time:
+ * each holds this lock from its launch until its stream has drained.  Two
+ * such grids at once could each hold execution slots the other's grid
+ * barrier waits for. */
+void gga_coop_lock(void);
+void gga_coop_unlock(void);
+
 /* thread-local detail for gga_last_error() */
 void gga_set_error(const char *fmt, ...);
 void gga_clear_error(void);

@@ -11,6 +11,7 @@
  */
 #define _GNU_SOURCE
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -32,8 +33,10 @@ struct gga_graph {
     uint32_t maxdeg, maxdeg_idx;
     /* split */
     int split;
-    uint32_t C, mda;
+    uint32_t C, mda;              /* C: chunks of the table the kernels walk (d_cstart/d_cbeg/d_cown) */
     uint32_t *d_cstart, *d_cbeg, *d_cown;
+    uint32_t Cref;                /* chunks of the reference's NodeSplit (ns_nnode) */
+    uint32_t *d_rcbeg, *d_rcown;  /* its descriptors when they differ from the walked table, else NULL */
     gga_split_stats sst;
     /* query workspace */
     uint32_t *d_dist, *d_mark, *d_q0, *d_q1, *d_ctl, *d_tiles;
@@ -62,6 +65,10 @@ struct gga_graph {
 };
 
 const char *gga_version(void) { return GGA_VERSION; }
+
+static pthread_mutex_t coop_mu = PTHREAD_MUTEX_INITIALIZER;
+void gga_coop_lock(void) { pthread_mutex_lock(&coop_mu); }
+void gga_coop_unlock(void) { pthread_mutex_unlock(&coop_mu); }
 
 int gga_device_count(void)
 {
@@ -162,6 +169,8 @@ static void free_split(gga_graph *g)
     dfree(g, (void **)&g->d_cstart, 4ull * ((uint64_t)g->n + 1));
     dfree(g, (void **)&g->d_cbeg, 4ull * ((uint64_t)g->C + 1));
     dfree(g, (void **)&g->d_cown, 4ull * g->C);
+    dfree(g, (void **)&g->d_rcbeg, 4ull * ((uint64_t)g->Cref + 1));
+    dfree(g, (void **)&g->d_rcown, 4ull * g->Cref);
     g->split = 0;
 }
 
@@ -434,30 +443,55 @@ int gga_graph_split(gga_graph *g, uint32_t mda, int32_t level, uint32_t buckets,
     g->C = g->h_pin[0];
     if ((rc = dmalloc(g, (void **)&g->d_cbeg, 4ull * ((uint64_t)g->C + 1)))) goto fail;
     if ((rc = dmalloc(g, (void **)&g->d_cown, 4ull * g->C))) goto fail;
+    uint32_t max_chunk = 0;
     {
         uint32_t *d_max = d_tmp;  /* scan scratch is free again */
         K_TRY(ggak_fill_u32(d_max, 0u, GGA_MAXCHUNK_SLOTS, g->stream));
         K_TRY(ggak_split_fill(g->d_row, g->d_cstart, g->n, g->m, g->d_cbeg, g->d_cown, d_max, g->stream));
-        HIP_TRY(hipEventRecord(g->ev[1], g->stream));
         HIP_TRY(hipMemcpyAsync(g->h_pin, d_max, 4ull * GGA_MAXCHUNK_SLOTS, hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
-        for (uint32_t i = 1; i < GGA_MAXCHUNK_SLOTS; i++)
-            if (g->h_pin[i] > g->h_pin[0]) g->h_pin[0] = g->h_pin[i];
+        for (uint32_t i = 0; i < GGA_MAXCHUNK_SLOTS; i++)
+            if (g->h_pin[i] > max_chunk) max_chunk = g->h_pin[i];
     }
+    g->Cref = g->C;
+    /* A chunk longer than GGA_PIECE_MAX edges (a threshold above it that
+     * leaves hubs whole: L = 0 / 1, or an explicit large T) would be walked
+     * by one wave while the rest of the grid idles -- the load imbalance the
+     * reference answers with a child grid per heavy vertex (cudaKernels.h:
+     * 466-487).  The kernels then walk a second table whose chunks are the
+     * same vertices cut at min(T, GGA_PIECE_MAX) edges, so every heavy
+     * vertex is spread over many waves; the reference's table stays for the
+     * split statistics and gga_graph_chunks. */
+    if (max_chunk > GGA_PIECE_MAX) {
+        g->d_rcbeg = g->d_cbeg;
+        g->d_rcown = g->d_cown;
+        g->d_cbeg = g->d_cown = NULL;
+        K_TRY(ggak_split_scan(g->d_row, g->n, GGA_PIECE_MAX, g->d_cstart, d_tmp, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->h_pin, g->d_cstart + g->n, 4, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        g->C = g->h_pin[0];
+        if ((rc = dmalloc(g, (void **)&g->d_cbeg, 4ull * ((uint64_t)g->C + 1)))) goto fail;
+        if ((rc = dmalloc(g, (void **)&g->d_cown, 4ull * g->C))) goto fail;
+        K_TRY(ggak_fill_u32(d_tmp, 0u, GGA_MAXCHUNK_SLOTS, g->stream));
+        K_TRY(ggak_split_fill(g->d_row, g->d_cstart, g->n, g->m, g->d_cbeg, g->d_cown, d_tmp, g->stream));
+    }
+    HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
     dfree(g, (void **)&d_tmp, 4ull * tmp_words);
     g->mda = T;
     g->split = 1;
     memset(&g->sst, 0, sizeof g->sst);
     g->sst.nnode = g->n;
     g->sst.nedge = g->m;
-    g->sst.ns_nnode = g->C;
+    g->sst.ns_nnode = g->Cref;
     g->sst.ns_nedge = g->m;
-    g->sst.extranode = g->C - g->n;
+    g->sst.extranode = g->Cref - g->n;
     g->sst.maxdeg = g->maxdeg;
     g->sst.level = level_rec;
     g->sst.bucket = bucket_rec;
     g->sst.mda = T;
-    g->sst.max_chunk = g->h_pin[0];
+    g->sst.max_chunk = max_chunk;
+    g->sst.exec_chunks = g->C;
     HIP_TRY(hipEventElapsedTime(&g->sst.split_ms, g->ev[0], g->ev[1]));
     if ((rc = ensure_queues(g, g->C))) goto fail;
     if (st) *st = g->sst;
@@ -474,10 +508,11 @@ int gga_graph_chunks(gga_graph *g, uint32_t *chunk_begin, uint32_t *owner)
     if (!g) return GGA_EINVAL;
     if (!g->split) return GGA_ENOSPLIT;
     HIP_TRY(hipSetDevice(g->device));
+    /* the reference's table (NodeSplit's layout), also where the kernels walk a cut one */
+    const uint32_t *cb = g->d_rcbeg ? g->d_rcbeg : g->d_cbeg, *co = g->d_rcown ? g->d_rcown : g->d_cown;
     if (chunk_begin)
-        HIP_TRY(hipMemcpyAsync(chunk_begin, g->d_cbeg, 4ull * ((uint64_t)g->C + 1), hipMemcpyDeviceToHost,
-                               g->stream));
-    if (owner) HIP_TRY(hipMemcpyAsync(owner, g->d_cown, 4ull * g->C, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(chunk_begin, cb, 4ull * ((uint64_t)g->Cref + 1), hipMemcpyDeviceToHost, g->stream));
+    if (owner) HIP_TRY(hipMemcpyAsync(owner, co, 4ull * g->Cref, hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     return GGA_OK;
 fail:
@@ -744,11 +779,16 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
             gga_set_error("occupancy query failed");
             goto fail;
         }
-        HIP_TRY(hipEventRecord(g->ev[2], g->stream));
         /* every barrier wait is bounded (5 s); a step never comes near that */
-        K_TRY(ggak_persistent(dg, dw, bfs, count, grid, max_iters, 5000u, g->stream));
-        HIP_TRY(hipEventRecord(g->ev[3], g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
+        gga_coop_lock();
+        hipError_t he = hipEventRecord(g->ev[2], g->stream);
+        int kr = he == hipSuccess ? ggak_persistent(dg, dw, bfs, count, grid, max_iters, 5000u, g->stream) : 0;
+        if (he == hipSuccess && kr == 0) he = hipEventRecord(g->ev[3], g->stream);
+        const hipError_t hs = hipStreamSynchronize(g->stream);
+        gga_coop_unlock();
+        HIP_TRY(he);
+        K_TRY(kr);
+        HIP_TRY(hs);
         HIP_TRY(hipEventElapsedTime(&ktime, g->ev[2], g->ev[3]));
         s->launches = 1;
     }

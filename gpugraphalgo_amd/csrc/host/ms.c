@@ -492,9 +492,17 @@ int gga_ms_run(gga_ms *e, uint32_t max_steps, const gga_opts *o, uint32_t *live_
             gga_set_error("occupancy query failed");
             return GGA_EHIP;
         }
-        HIP_TRY(hipEventRecord(e->ev[0], e->v.stream));
-        K_TRY(ggak_ms_persistent(e->v.dg, e->d, e->step % 6u, cap, count, grid, 5000u, e->v.stream));
-        HIP_TRY(hipEventRecord(e->ev[1], e->v.stream));
+        gga_coop_lock();
+        hipError_t he = hipEventRecord(e->ev[0], e->v.stream);
+        int kr = he == hipSuccess ? ggak_ms_persistent(e->v.dg, e->d, e->step % 6u, cap, count, grid, 5000u,
+                                                       e->v.stream)
+                                  : 0;
+        if (he == hipSuccess && kr == 0) he = hipEventRecord(e->ev[1], e->v.stream);
+        const hipError_t hs = hipStreamSynchronize(e->v.stream);
+        gga_coop_unlock();
+        HIP_TRY(he);
+        K_TRY(kr);
+        HIP_TRY(hs);
         rc = ms_readback(e);
         /* the kernel ran exactly the non-empty steps it counted */
         ran = e->h_pin[GGA_CTL_STEPS] - e->steps_seen;

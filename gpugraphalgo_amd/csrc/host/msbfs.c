@@ -243,11 +243,19 @@ int gga_multi_bfs(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *out,
     HIP_TRY(hipMemcpyAsync(s->d_src, srcs, 4ull * k, hipMemcpyHostToDevice, v.stream));
     HIP_TRY(hipEventRecord(s->ev[0], v.stream));
     K_TRY(ggak_msbfs_init(v.dg, b, s->d_src, v.stream));
-    /* every barrier wait is bounded (5 s) */
-    K_TRY(ggak_msbfs_run(v.dg, t, b, v.dg.n + 1u, s->grid, 5000u, v.stream));
-    HIP_TRY(hipEventRecord(s->ev[1], v.stream));
-    HIP_TRY(hipMemcpyAsync(s->h_pin, s->ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, v.stream));
-    HIP_TRY(hipStreamSynchronize(v.stream));
+    {
+        /* every barrier wait is bounded (5 s) */
+        gga_coop_lock();
+        int kr = ggak_msbfs_run(v.dg, t, b, v.dg.n + 1u, s->grid, 5000u, v.stream);
+        hipError_t he = kr == 0 ? hipEventRecord(s->ev[1], v.stream) : hipSuccess;
+        if (kr == 0 && he == hipSuccess)
+            he = hipMemcpyAsync(s->h_pin, s->ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, v.stream);
+        const hipError_t hs = hipStreamSynchronize(v.stream);
+        gga_coop_unlock();
+        K_TRY(kr);
+        HIP_TRY(he);
+        HIP_TRY(hs);
+    }
     HIP_TRY(hipEventElapsedTime(&r.time_ms, s->ev[0], s->ev[1]));
     {
         const uint32_t *c = s->h_pin;

@@ -282,6 +282,7 @@ static int admit(gga_server *s, double now)
             continue;
         }
         const uint32_t lane = (uint32_t)__builtin_ctz(lanes_all & ~(E->busy | E->start_mask));
+        if (E->busy) s->st.joined++;   /* lanes of this engine are running: the new one joins them */
         E->start_mask |= 1u << lane;
         E->start_src[lane] = x->q.src;
         if (x->q.bfs) E->start_unit |= 1u << lane;
@@ -292,6 +293,7 @@ static int admit(gga_server *s, double now)
         x->r.engine = (uint32_t)(E - G->eng);
         x->r.status = 2;   /* running */
         s->running++;
+        if (s->running > s->st.peak_running) s->st.peak_running = (uint32_t)s->running;
     }
     s->npend = keep;
     pthread_mutex_unlock(&s->mu);

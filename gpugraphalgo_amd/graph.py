@@ -129,6 +129,12 @@ def write_dist(path: str, dist: np.ndarray) -> None:
     check(load().gga_write_dist(os.fsencode(path), _ptr(dist), dist.size), "write_dist")
 
 
+def write_degree_dist(path: str, g: CsrGraph) -> None:
+    """DD_<graph>.txt: "deg,count" per occurring out-degree (printfDegreeDistribution,
+    graph.h:98-114)."""
+    check(load().gga_write_degree_dist(path.encode(), g.n, _ptr(g.row_ptr)), "write_degree_dist")
+
+
 def write_result_row(path: str, **fields) -> None:
     """writeResult (myutils.h:86-113)."""
     r = _lib.ResultRow()

@@ -152,7 +152,7 @@ class Server:
     def stats(self) -> dict:
         st = _lib.ServerCounters()
         check(load().gga_server_stats(self._h, ctypes.byref(st)), "gga_server_stats")
-        return {f: getattr(st, f) for f, _ in st._fields_ if f != "pad"}
+        return {f: getattr(st, f) for f, _ in st._fields_}
 
 
 def latency_summary(res: dict) -> dict:

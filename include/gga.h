@@ -128,7 +128,14 @@ typedef struct gga_split_stats {
     uint32_t mda;        /* MAX_EDGES_ALLOWED (the split threshold T) */
     uint32_t max_chunk;  /* largest chunk in edges (<= mda) */
     float    split_ms;   /* device time of the split pass */
+    uint32_t exec_chunks;/* chunks the relax kernels walk: the split's chunks, with every chunk
+                            longer than GGA_PIECE_MAX edges cut into pieces of at most that
+                            many, so that a vertex the threshold leaves whole (L = 0 / 1, the
+                            reference's dynamic-parallelism case) is still expanded by many
+                            waves at once */
 } gga_split_stats;
+/* Longest chunk a wave expands: 4 rounds of 64 lanes x 4 edge slots. */
+#define GGA_PIECE_MAX 1024u
 
 /* Node splitting as a GPU prefix-sum pass (replaces Graph::NodeSplit,
  * graph.h:415-522, getSizeafterLoadBalance 386-414 and
@@ -269,6 +276,10 @@ int gga_write_conv(const char *path, int32_t level, uint32_t maxdeg, const char 
 /* gga_checksum of the last result, computed on the device. */
 int gga_graph_checksum(gga_graph *g, uint64_t *sum);
 
+/* Degree distribution file (printfDegreeDistribution, graph.h:98-114, written
+ * as DD_<graph>.txt when bfs_sssp.cu:196-200 is enabled): one "deg,count"
+ * line per out-degree that occurs, ascending, from the host CSR row_ptr[n + 1]. */
+int gga_write_degree_dist(const char *path, uint32_t n, const uint32_t *row_ptr);
 /* Distance file: one "%u\n" per original vertex (Ary2File, myutils.h:51-62). */
 int gga_write_dist(const char *path, const uint32_t *dist, uint32_t n);
 
@@ -461,7 +472,11 @@ typedef struct gga_query_result {
 typedef struct gga_server_counters {
     uint64_t submitted, completed, rejected, rounds, evictions, engines_created;
     uint64_t bytes_in_use, peak_bytes;
-    uint32_t engines_resident, pad;
+    uint32_t engines_resident;
+    uint32_t peak_running;    /* most queries running at once */
+    uint64_t joined;          /* queries admitted into an engine while other lanes of it were
+                                 running (continuous batching: work of a new query overlaps
+                                 work already on the device) */
 } gga_server_counters;
 
 /* Threading: gga_server_submit, gga_server_poll and gga_server_now may be

@@ -123,3 +123,47 @@ def test_cli_no_split_row(grfile, tmp_path):
     run_cli(_lib.CLI_PATH, [path, "--algo", "bfs", "--result", "r.vi"], str(tmp_path))
     (r,) = read_row(tmp_path / "r.vi")
     assert int(r["mda"]) == 0 and int(r["ns_nnode"]) == g.n and int(r["extranode"]) == 0
+
+
+def test_mysc_sweep_reproduces_published_rows(tmp_path, split_kat):
+    """The reference's sweep script (mysc:20-51): L = 1..40, then L = -1 with
+    buckets {2, 5, 10, 15}, each a separate run of the executable.  Run
+    through gga_run (linked as na_bfs.exe) on a .gr with webs' published
+    degree sequence, the 44 result.vi rows must carry the published split
+    fields of na_bfs.exe on webs (Kepler result.vi) -- MDA, node count after
+    the split, maxdeg, n, m -- and the first run's --dd file the published
+    DD_webs.gr.txt."""
+    dd = np.asarray(split_kat["dd"]["webs"], dtype=np.int64)
+    deg = np.repeat(dd[:, 0], dd[:, 1]).astype(np.uint32)
+    rng = np.random.default_rng(11)
+    rng.shuffle(deg)
+    rp = np.concatenate([[0], np.cumsum(deg, dtype=np.int64)]).astype(np.uint32)
+    dst = rng.integers(0, deg.size, int(rp[-1]), dtype=np.int64).astype(np.uint32)
+    g = gga.csr_from_arrays(rp, dst, None)
+    gga.write_gr(str(tmp_path / "webs.gr"), g)
+    exe = str(tmp_path / "na_bfs.exe")
+    os.symlink(_lib.CLI_PATH, exe)
+    published = {(r["level"], r["bucket"]): r for r in split_kat["rows"]
+                 if r["graph"] == "webs" and r["exe"] == "na_bfs.exe" and r["file"].startswith("Kepler")}
+    runs = [[str(L)] for L in range(1, 41)] + [["-1", str(b)] for b in (2, 5, 10, 15)]
+    assert len(runs) == len(published) == 44
+    for k, args in enumerate(runs):
+        extra = ["--dd", "--mode", "worklist"] if k == 0 else ["--mode", "worklist"]
+        out = run_cli(exe, ["webs.gr"] + args + extra, str(tmp_path))
+        assert "BFS" in out.splitlines()[0]
+    with open(tmp_path / "DD_webs.gr.txt") as f:
+        assert f.read() == "".join(f"{d},{c}\n" for d, c in dd.tolist())
+    rows = read_row(str(tmp_path / "result.vi"))
+    assert len(rows) == 44
+    degs = g.degrees()
+    for args, r in zip(runs, rows):
+        L = int(args[0])
+        b = int(args[1]) if L == -1 else 0
+        want = published[(L, b)]
+        assert r["exe"].endswith("na_bfs.exe") and r["algo"] == "BFS" and r["graph"] == "webs.gr"
+        got = {k: int(r[k]) for k in ("bucket", "nnode", "nedge", "maxdeg", "mda", "ns_nnode")}
+        got["level"] = int(np.uint32(int(r["level"])).astype(np.int32))
+        exp = {k: want[k] for k in ("bucket", "nnode", "nedge", "maxdeg", "mda", "ns_nnode", "level")}
+        assert got == exp, (args, got, exp)
+        assert int(r["ns_nnode"]) == oracle.split_count(degs, int(r["mda"]))
+        assert int(r["extranode"]) == int(r["ns_nnode"]) - int(r["nnode"])

@@ -257,14 +257,14 @@ def test_near_far_sssp_equals_oracle(gpu, golden, delta):
 @pytest.mark.slow
 def test_full_size_config3_grid_sssp_bfs_equal_oracle(gpu):
     """Config 3 at full size (4096^2 grid + 0.5% shortcuts): the default
-    (AUTO: persistent kernel, near-far SSSP) and worklist modes against the
-    oracle's Dijkstra / queue BFS."""
+    (AUTO), worklist and persistent (one cooperative launch, near-far SSSP)
+    modes against the oracle's Dijkstra / queue BFS."""
     g = gga.gen_grid(4096, 4096, 0.005, seed=11, wlo=1, whi=1000)
     want_s = oracle.dijkstra(g.row_ptr, g.dst, g.w, 0)
     want_b = oracle.bfs_queue(g.row_ptr, g.dst, 0)
     with gga.DeviceGraph(g) as dg:
         dg.split()
-        for mode in ("auto", "worklist"):
+        for mode in ("auto", "worklist", "persistent"):
             announce(f"config3 full size sssp mode={mode}")
             np.testing.assert_array_equal(dg.sssp(0, mode=mode)[0], want_s)
             announce(f"config3 full size bfs mode={mode}")

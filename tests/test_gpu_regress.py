@@ -126,3 +126,75 @@ def test_device_verifier_counts_planted_errors(gpu, bfs):
                 assert nerr == 0 and bad == 0
             else:
                 assert nerr + bad > 0, f"planted case {case} went unnoticed by the oracle"
+
+
+def test_concurrent_persistent_queries_on_two_graphs(gpu):
+    """Two threads, each driving its own graph, run cooperative-launch
+    queries (persistent single-source, 256-lane bitmask BFS) at the same
+    time: every call succeeds and equals the oracle (the library runs one
+    cooperative grid at a time)."""
+    import threading
+    gs = [gga.gen_grid(60, 70, 0.01, seed=31, wlo=1, whi=1000), gga.gen_rmat(13, 16, seed=32, wlo=1, whi=100)]
+    srcs = [[0, 1000, 4199], [0, 77, 4000]]
+    want = [{s: oracle.dijkstra(g.row_ptr, g.dst, g.w, s) for s in ss} for g, ss in zip(gs, srcs)]
+    want_b = [{s: oracle.bfs_queue(g.row_ptr, g.dst, s) for s in ss} for g, ss in zip(gs, srcs)]
+    errors = []
+    dgs = [gga.DeviceGraph(g) for g in gs]
+    try:
+        for dg in dgs:
+            dg.split(mda=16)
+
+        def worker(i):
+            try:
+                for rep in range(4):
+                    for s in srcs[i]:
+                        d, _ = dgs[i].sssp(s, mode="persistent")
+                        if not np.array_equal(d, want[i][s]):
+                            errors.append(f"graph {i} src {s} rep {rep}: persistent sssp differs")
+                    lv, _ = dgs[i].multi_bfs(np.array(srcs[i], dtype=np.uint32))
+                    for k, s in enumerate(srcs[i]):
+                        if not np.array_equal(lv[k], want_b[i][s]):
+                            errors.append(f"graph {i} src {s} rep {rep}: bitmask bfs differs")
+            except Exception as e:  # noqa: BLE001 - reported below
+                errors.append(f"graph {i}: {e!r}")
+
+        announce("two threads, cooperative launches on two graphs")
+        th = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join(timeout=300)
+        assert not any(t.is_alive() for t in th), "worker still running"
+    finally:
+        for dg in dgs:
+            dg.close()
+    assert not errors, errors
+
+
+def test_unsplit_hubs_walked_in_pieces(gpu):
+    """L = 0 (no split, the reference's baseline and dynamic-parallelism
+    case): the split statistics and gga_graph_chunks keep the reference's
+    one-chunk-per-vertex table, while the kernels walk hubs cut into pieces
+    of at most GGA_PIECE_MAX edges; every mode is still exact."""
+    g = gga.gen_rmat(16, 16, seed=7, wlo=1, whi=100)
+    maxdeg, src = g.maxdeg()
+    assert maxdeg > 1024
+    deg = g.degrees().astype(np.int64)
+    want = oracle.dijkstra(g.row_ptr, g.dst, g.w, src)
+    want_b = oracle.bfs_queue(g.row_ptr, g.dst, src)
+    with gga.DeviceGraph(g) as dg:
+        st = dg.split()
+        assert st["ns_nnode"] == g.n and st["mda"] == maxdeg and st["max_chunk"] == maxdeg
+        assert st["exec_chunks"] == int(np.maximum(1, -(-deg // 1024)).sum())
+        cb, own = dg.chunks()
+        assert cb.size == g.n + 1 and own.size == g.n
+        ref_cb, ref_own = oracle.node_split(g.row_ptr, maxdeg)[:2]
+        np.testing.assert_array_equal(cb, ref_cb)
+        np.testing.assert_array_equal(own, ref_own)
+        for mode in ("topo", "worklist", "persistent"):
+            announce(f"unsplit rmat16 mode={mode}")
+            np.testing.assert_array_equal(dg.sssp(src, mode=mode)[0], want)
+            np.testing.assert_array_equal(dg.bfs(src, mode=mode)[0], want_b)
+        st = dg.split(mda=4096)
+        assert st["exec_chunks"] > st["ns_nnode"]
+        np.testing.assert_array_equal(dg.sssp(src, mode="worklist")[0], want)

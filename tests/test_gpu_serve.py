@@ -58,8 +58,11 @@ def test_served_results_equal_single_source(gpu, use_graph):
         want = want_for(g, r["src"], bool(r["bfs"]))
         np.testing.assert_array_equal(outs[i], want, err_msg=f"query {i} graph {r['graph']} src {r['src']}")
         assert int(res["checksum"][i]) == checksum(want)
-    # queries overlapped: more than one lane busy at some point
-    assert st["completed"] == 150 and st["rounds"] < 150 * 20
+    # continuous batching: queries were admitted into engines whose other
+    # lanes were still running, and many ran at once
+    assert st["completed"] == 150
+    assert st["joined"] > 0, st
+    assert st["peak_running"] > 1, st
     for dg in dgs:
         dg.close()
 

@@ -225,3 +225,31 @@ def test_unknown_option_names_raise():
     assert (o.dense_min, o.delta, o.mode) == (7, 3, 1)
     with pytest.raises(TypeError, match="dens_min"):
         _opts("worklist", INF, False, {"dens_min": 7})
+
+
+@pytest.mark.parametrize("graph", ["webs", "rmat20"])
+def test_degree_distribution_matches_published_file(tmp_path, split_kat, graph):
+    """A graph with the published degree sequence writes back the published
+    DD_<graph>.gr.txt histogram (printfDegreeDistribution, graph.h:98-114):
+    one "deg,count" line per occurring degree, ascending."""
+    dd = np.asarray(split_kat["dd"][graph], dtype=np.int64)
+    deg = np.repeat(dd[:, 0], dd[:, 1]).astype(np.uint32)
+    np.random.default_rng(3).shuffle(deg)
+    rp = np.concatenate([[0], np.cumsum(deg, dtype=np.int64)]).astype(np.uint32)
+    g = gga.csr_from_arrays(rp, np.zeros(int(rp[-1]), dtype=np.uint32), None)
+    p = str(tmp_path / f"DD_{graph}.gr.txt")
+    gga.write_degree_dist(p, g)
+    with open(p) as f:
+        text = f.read()
+    assert text == "".join(f"{d},{c}\n" for d, c in dd.tolist())
+
+
+def test_degree_distribution_small_cases(tmp_path):
+    p = str(tmp_path / "dd.txt")
+    # degrees 2, 0, 2, 5: zero-degree vertices are listed too, gaps are skipped
+    rp = np.array([0, 2, 2, 4, 9], dtype=np.uint32)
+    gga.write_degree_dist(p, gga.csr_from_arrays(rp, np.zeros(9, dtype=np.uint32), None))
+    assert open(p).read() == "0,1\n2,2\n5,1\n"
+    rp = np.zeros(4, dtype=np.uint32)   # no edges at all
+    gga.write_degree_dist(p, gga.csr_from_arrays(rp, np.zeros(0, dtype=np.uint32), None))
+    assert open(p).read() == "0,3\n"

@@ -214,3 +214,22 @@ def test_submit_and_poll_from_other_threads_while_pumping():
     assert set(submitted) <= set(polled)
     assert set(polled) - set(submitted) <= {int(x) for x in res["id"]}
     assert st["completed"] == 2 * n_each + len(rows)
+
+
+def test_joined_counts_queries_admitted_beside_running_lanes():
+    """Two queries start; two more arrive while their lanes run and join the
+    same engine (continuous batching), so four run at once."""
+    with Server(lanes=8, dry_run=True, dry_steps=3, engines_per_graph=1) as s:
+        g = s.add_graph_dry(100, 100)
+        s.submit(g, 1, True)
+        s.submit(g, 2, False)
+        s.pump()                       # admitted and launched
+        st = s.stats()
+        assert st["joined"] == 0 and st["peak_running"] == 2
+        s.submit(g, 3, True)
+        s.submit(g, 4, False)
+        s.pump()                       # the first two are still running
+        st = s.stats()
+        assert st["joined"] == 2 and st["peak_running"] == 4
+        s.drain()
+        assert s.stats()["completed"] == 4
