@@ -140,6 +140,7 @@ SIGNATURES = [
     ("gga_bfs", ctypes.c_int, [vp, u32, vp, P(Opts), P(Stats)]),
     ("gga_sssp", ctypes.c_int, [vp, u32, vp, P(Opts), P(Stats)]),
     ("gga_graph_copy_dist", ctypes.c_int, [vp, vp]),
+    ("gga_sssp_noalloc", ctypes.c_int, [vp, u32, vp, P(Opts), P(Stats)]),
     ("gga_write_degree_dist", ctypes.c_int, [ctypes.c_char_p, u32, vp]),
     ("gga_graph_dist_device", vp, [vp]),
     ("gga_verify", ctypes.c_int, [u32, vp, vp, vp, u32, vp, u32, P(VerifyResult)]),

@@ -37,6 +37,13 @@ int ggak_split_fill(const uint32_t *row, const uint32_t *cstart, uint32_t n, uin
  * ggak_split_tmp_words(len) words. */
 int ggak_scan_u32(const uint32_t *in, uint32_t len, uint32_t *out, uint32_t *d_tmp, hipStream_t s);
 
+/* ---- static workspace of the allocation-free query (relax.hip) ---- */
+#define GGA_SCRATCH_BYTES (128ull << 20)
+/* *ptr = this device's address of the static scratch array (looked up once
+ * per device with hipGetSymbolAddress, that device current); returns 0 or a
+ * hipError_t. */
+int ggak_scratch(int device, void **ptr);
+
 /* ---- traversal (relax.hip) ---- */
 int ggak_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t s);
 int ggak_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src, uint32_t epoch, hipStream_t s);

@@ -948,6 +948,126 @@ int gga_sssp(gga_graph *g, uint32_t src, uint32_t *out, const gga_opts *o, gga_s
     return traverse(g, src, 0, out, o, st);
 }
 
+/* ------------------------------------------------ allocation-free SSSP */
+/* Byte offsets of a query's workspace inside the static scratch array
+ * (every part 256-byte aligned; the control words keep their 128-byte
+ * lines). */
+typedef struct scratch_layout {
+    uint64_t ctl, cnt, tiles, mark, flag, q0, q1, last, tmin, total;
+} scratch_layout;
+
+static uint64_t up256(uint64_t x) { return (x + 255u) & ~255ull; }
+
+static void scratch_plan(uint32_t n, uint32_t C, int near_far, scratch_layout *L)
+{
+    const uint64_t ntiles = GGA_TILES((uint64_t)n);
+    uint64_t o = 0;
+    L->ctl = o;
+    o = up256(o + 4ull * GGA_CTL_WORDS);
+    L->cnt = o;
+    o = up256(o + 32ull * GGA_CNT_SLOTS);
+    L->tiles = o;
+    o = up256(o + 4ull * (ntiles + 1));
+    L->mark = o;
+    o = up256(o + 4ull * n);
+    L->flag = o;
+    o = up256(o + (uint64_t)n);
+    L->q0 = o;
+    o = up256(o + 4ull * C);
+    L->q1 = o;
+    o = up256(o + 4ull * C);
+    L->last = L->tmin = 0;
+    if (near_far) {
+        L->last = o;
+        o = up256(o + 4ull * n);
+        L->tmin = o;
+        o = up256(o + 4ull * ntiles);
+    }
+    L->total = o;
+}
+
+/* One query at a time uses a device's scratch array. */
+static pthread_mutex_t noalloc_mu = PTHREAD_MUTEX_INITIALIZER;
+
+int gga_sssp_noalloc(gga_graph *g, uint32_t src, uint32_t *d_out, const gga_opts *user, gga_stats *st)
+{
+    int rc = GGA_OK;
+    gga_opts o;
+    gga_stats s;
+    query_plan p;
+    scratch_layout L;
+    uint32_t delta = 0, adv = 0;
+    void *base = NULL;
+    memset(&s, 0, sizeof s);
+    if (user) o = *user;
+    else gga_opts_init(&o);
+    if (!g || !d_out) return GGA_EINVAL;
+    gga_clear_error();
+    if (src >= g->n) {
+        gga_set_error("source %u >= n %u", src, g->n);
+        return GGA_EINVAL;
+    }
+    if (!g->split) {
+        gga_set_error("gga_sssp_noalloc needs a split graph (gga_graph_split allocates the chunk table)");
+        return GGA_ENOSPLIT;
+    }
+    if (o.timeline || o.conv_rows) {
+        gga_set_error("instrumented queries allocate their buffers: not available in gga_sssp_noalloc");
+        return GGA_EINVAL;
+    }
+    const uint32_t inf = o.inf ? o.inf : GGA_INF;
+    plan_query(g, 0, &o, &p, &delta);
+    p.src = src;
+    p.own = 0;
+    p.epoch = 0;
+    scratch_plan(g->n, g->C, delta != 0, &L);
+    if (L.total > GGA_SCRATCH_BYTES) {
+        gga_set_error("query workspace of %llu bytes (n = %u, %u chunks%s) exceeds the static scratch of %llu bytes",
+                      (unsigned long long)L.total, g->n, g->C, delta ? ", near-far" : "",
+                      (unsigned long long)GGA_SCRATCH_BYTES);
+        return GGA_ERANGE;
+    }
+    pthread_mutex_lock(&noalloc_mu);
+    HIP_TRY(hipSetDevice(g->device));
+    K_TRY(ggak_scratch(g->device, &base));
+    {
+        uint8_t *b = (uint8_t *)base;
+        gga_dev_graph dg = dev_graph(g);
+        gga_dev_work dw = dev_work(g, inf);
+        dw.dist = d_out;
+        dw.mark = (uint32_t *)(b + L.mark);
+        dw.flag = b + L.flag;
+        dw.q0 = (uint32_t *)(b + L.q0);
+        dw.q1 = (uint32_t *)(b + L.q1);
+        dw.ctl = (uint32_t *)(b + L.ctl);
+        dw.cnt = (uint64_t *)(b + L.cnt);
+        dw.tiles = (uint32_t *)(b + L.tiles);
+        dw.qcap = g->C;
+        dw.dense_min = dense_default(g, &o);
+        dw.bmin = mesh_like(g) ? GGA_MESH_BMIN : 1u;
+        if (delta) {
+            dw.last = (uint32_t *)(b + L.last);
+            dw.tmin = (uint32_t *)(b + L.tmin);
+            dw.delta = delta;
+        }
+        /* the array holds whatever the previous call (of any graph) left:
+         * the step stamps and every control word start over (fill kernels,
+         * no memset nodes) */
+        K_TRY(ggak_fill_u32(dw.mark, 0xFFFFFFFFu, g->n, g->stream));
+        K_TRY(ggak_fill_u32(dw.ctl, 0u, GGA_CTL_WORDS, g->stream));
+        s.mode = p.mode;
+        rc = run_query(g, dg, dw, &p, &o, &s, &adv);
+    }
+    pthread_mutex_unlock(&noalloc_mu);
+    if (st) *st = s;
+    return rc;
+fail:
+    hipStreamSynchronize(g->stream);
+    pthread_mutex_unlock(&noalloc_mu);
+    if (st) *st = s;
+    return rc;
+}
+
 int gga_graph_copy_dist(gga_graph *g, uint32_t *out)
 {
     int rc = GGA_OK;

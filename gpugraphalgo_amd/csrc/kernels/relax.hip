@@ -44,7 +44,17 @@
 #include "sync.hpp"
 #include "compact.hpp"
 
+#include <mutex>
+
 using namespace gga;
+
+// Workspace of the allocation-free query (gga_sssp_noalloc): step stamps,
+// flag bytes, the two chunk lists, compaction tiles, control words, work
+// counters and the near-far arrays, carved by the host out of one static
+// array.  Each device that runs the query holds its own copy from module
+// load to process end.  Host code never names `scratch` in a kernel
+// argument or a copy: it uses the device address ggak_scratch looks up.
+static __device__ __align__(256) uint8_t scratch[GGA_SCRATCH_BYTES];
 
 namespace {
 
@@ -712,6 +722,33 @@ inline uint32_t grid_for(uint64_t items, uint32_t cap) {
 }
 
 }  // namespace
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_addr[SCRATCH_DEVICES];   // device address of `scratch`, per device
+}  // namespace
+
+extern "C" int ggak_scratch(int device, void **ptr) {
+    *ptr = nullptr;
+    if (device < 0 || device >= SCRATCH_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_addr[device]) {
+        // once per device, with that device current
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return (int)e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return (int)e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return (int)e;
+        if (!p) return (int)hipErrorInvalidSymbol;
+        scratch_addr[device] = p;
+    }
+    *ptr = scratch_addr[device];
+    return 0;
+}
 
 extern "C" int ggak_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t s) {
     if (count == 0) return 0;

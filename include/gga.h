@@ -241,6 +241,21 @@ typedef struct gga_stats {
 int gga_bfs(gga_graph *g, uint32_t src, uint32_t *out, const gga_opts *o, gga_stats *st);
 /* SSSP distances from src with the stored uint32 weights (the na_sssp path). */
 int gga_sssp(gga_graph *g, uint32_t src, uint32_t *out, const gga_opts *o, gga_stats *st);
+/* SSSP that allocates nothing (no hipMalloc / hipFree / event or stream
+ * creation).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory once set up -- tight serving loops, harnesses that
+ * forbid allocation after initialisation.  The graph must be uploaded and
+ * split beforehand (GGA_ENOSPLIT otherwise); d_out is the caller's device
+ * buffer of n uint32 and receives the distances.  The query's workspace
+ * (step stamps, flag bytes, chunk lists, control words, counters, near-far
+ * arrays) lives in one static device array of GGA_SCRATCH_BYTES (128 MiB)
+ * per device, held from module load to process end; a graph whose workspace
+ * exceeds it -- 13 n + 8 C bytes plus ~300 KB, + 4 n with near-far -- is
+ * GGA_ERANGE.  Calls share that array: the library serialises them
+ * (one at a time per process), and each call starts its contents over.
+ * Instrumentation (opts.timeline / conv_rows) is GGA_EINVAL; ktime_ms and
+ * ctime_ms stay 0 outside PERSISTENT mode. */
+int gga_sssp_noalloc(gga_graph *g, uint32_t src, uint32_t *d_out, const gga_opts *o, gga_stats *st);
 /* Copies the last result (n uint32) to the host (bfs_sssp.cu:489). */
 int gga_graph_copy_dist(gga_graph *g, uint32_t *out);
 /* Device pointer of the last result (n uint32), valid until the next query. */
