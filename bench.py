@@ -121,6 +121,27 @@ def distinct_sources(dg, g, mode, k=20):
             "gteps": round(reach / sum(walls) / 1e9, 3)}
 
 
+RATES = os.path.join(ROOT, "gpugraphalgo_amd", "bin", "gga_rates")
+
+
+def random_access_ceiling(scale: int, pass_pct: int) -> dict | None:
+    """Runs the memory-rate probe (gpugraphalgo_amd/csrc/probes/rates.hip) with
+    the relax kernel's access pattern at this workload's scale and guard-pass
+    share: streamed col/w, a gather of dist[v] per edge from an R-MAT column
+    distribution, a device-scope atomicMin where the guard passes."""
+    import subprocess
+    if not os.path.exists(RATES):
+        return None
+    cmd = [RATES, "--scale", str(scale), "--pass", str(pass_pct), "--medges", "32"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        log(f"gga_rates failed: {r.stderr.strip()}")
+        return None
+    out = json.loads(r.stdout)
+    out["command"] = " ".join(["gpugraphalgo_amd/bin/gga_rates"] + cmd[1:])
+    return out
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -406,6 +427,21 @@ def main() -> int:
         serving = serving_leg(gga, np, full=not args.serve_reduced)
         log(f"serving: {json.dumps(serving)}")
 
+    ceiling = None
+    if not args.quick and r["edges_inspected"] > 0:
+        pass_pct = max(1, min(100, round(100.0 * r["relaxations"] / r["edges_inspected"])))
+        ceiling = random_access_ceiling(args.scale, pass_pct)
+    random_access = None
+    if ceiling:
+        relax = {(x["layout"], x["atomics"]): x for x in ceiling["relax"]}
+        achieved = r["edges_inspected"] / (r["ktime_ms"] * 1e-3) / 1e9 if r["ktime_ms"] > 0 else 0.0
+        cap = relax[("unsliced", True)]["g_edges_per_s"]
+        random_access = {
+            "achieved_g_edges_per_s": round(achieved, 1),
+            "probe_g_edges_per_s": cap,
+            "frac": round(achieved / cap, 3) if cap else None,
+            "probe": ceiling,
+        }
     traffic = None
     if os.path.exists(TRAFFIC) and r["mode"] == 1:
         from gpugraphalgo_amd.profiling import read_summary
@@ -452,6 +488,7 @@ def main() -> int:
             "avg_launch_ms": round(r["ktime_ms"] / max(r["launches"], 1), 5),
             "work": {"edges_inspected": r["edges_inspected"], "chunks_expanded": r["chunks_expanded"],
                      "relaxations": r["relaxations"]},
+            "random_access": random_access,
         },
         "cpu_baseline": cpu,
         "grid": grid,

@@ -97,7 +97,8 @@ class TuneResult(ctypes.Structure):
     _fields_ = [("mda", u32), ("grid", u32), ("dense_min", u32), ("layout", u32), ("mode", i32), ("delta", u32),
                 ("seed_mda", u32),
                 ("ntried", u32), ("seed_ms", ctypes.c_float), ("best_ms", ctypes.c_float),
-                ("layout_ms", ctypes.c_float * 2), ("tried_mda", u32 * 16), ("tried_ms", ctypes.c_float * 16)]
+                ("layout_ms", ctypes.c_float * 2), ("tried_mda", u32 * 16), ("tried_ms", ctypes.c_float * 16),
+                ("block", u32), ("block_ms", ctypes.c_float * 3)]
 
 
 class VerifyResult(ctypes.Structure):

@@ -48,9 +48,10 @@ int ggak_scratch(int device, void **ptr);
 int ggak_fill_u32(uint32_t *p, uint32_t value, uint64_t count, hipStream_t s);
 int ggak_query_init(gga_dev_graph g, gga_dev_work w, uint32_t src, uint32_t epoch, hipStream_t s);
 int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int count, hipStream_t s);
-/* tl != NULL: instrumented (counting) variant; workgroup b writes tl[b]. */
+/* tl != NULL: instrumented (counting) variant; workgroup b writes tl[b].
+ * block: workgroup size, 128, 256 or 512 threads. */
 int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t phase, int bfs, int count,
-                 uint32_t grid, gga_wg_record *tl, hipStream_t s);
+                 uint32_t grid, uint32_t block, gga_wg_record *tl, hipStream_t s);
 /* rows[0] += #v with dist != fin, rows[1] += sum of (fin - dist)^2 in uint32. */
 int ggak_conv(const uint32_t *dist, const uint32_t *fin, uint32_t n, uint64_t *rows, hipStream_t s);
 /* After ggak_wl_step(phase): builds the next list from step stamps when that
@@ -59,8 +60,9 @@ int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t phase, hipStream_t s)
 /* Near-far (w.last != NULL): after ggak_compact(phase), when the next list is
  * empty, lists the deferred vertices under the next threshold. */
 int ggak_refill(gga_dev_graph g, gga_dev_work w, uint32_t phase, hipStream_t s);
-/* Occupancy-bound grid (workgroups) for the worklist / persistent kernels. */
-uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count);
+/* Occupancy-bound grid (workgroups of `block` threads) for the worklist
+ * kernel, or for the persistent kernel (always 256 threads); 0 on error. */
+uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count, uint32_t block);
 /* One cooperative launch running worklist steps until the list is empty or
  * max_steps; grid must not exceed ggak_wl_grid(dev, 1, ...). */
 int ggak_persistent(gga_dev_graph g, gga_dev_work w, int bfs, int count, uint32_t grid,

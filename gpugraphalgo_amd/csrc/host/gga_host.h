@@ -32,7 +32,9 @@ uint32_t gga_graph_wmean(const gga_graph *g);
 uint32_t gga_graph_auto_delta(const gga_graph *g, int bfs);
 int gga_graph_mesh_like(const gga_graph *g);   /* max degree within 4x the mean (+8) */
 uint32_t gga_graph_maxdeg(const gga_graph *g);
-uint32_t gga_graph_default_grid(gga_graph *g, int bfs);
+/* Occupancy grid of the worklist kernel at `block` threads (0: the graph's size). */
+uint32_t gga_graph_default_grid(gga_graph *g, int bfs, uint32_t block);
+void gga_graph_set_tuned_block(gga_graph *g, uint32_t block);
 /* State of gga_multi_bfs (msbfs.c), owned by the graph. */
 void **gga_graph_msbfs_slot(gga_graph *g);
 void gga_msbfs_release(void *state);

@@ -47,11 +47,12 @@ struct gga_graph {
     uint32_t epoch;           /* step number of the next query's first step (mark[] stamps are below it) */
     int mark_dirty;           /* mark[] holds stamps of unknown step numbers: refill before the next query */
     int have_result;
-    uint32_t wl_grid[2][2];   /* [bfs][count] */
+    uint32_t wl_grid[3][2][2];/* [workgroup size 128 / 256 / 512][bfs][count] */
     uint32_t p_grid[2][2];
     uint32_t split_gen;       /* bumped by every split; engines built on an older table are stale */
     gga_ms *ms_cache;         /* engine of gga_multi_sp */
     uint32_t tuned_grid, tuned_dense, tuned_layout;   /* gga_tune results (0: defaults) */
+    uint32_t tuned_block;        /* worklist workgroup size from gga_tune (0: 256) */
     uint32_t tuned_delta;
     int32_t tuned_mode;          /* 0: none; else GGA_MODE_* + 1 */
     uint32_t last_steps[2];      /* [bfs] relax steps of the last query (AUTO's evidence) */
@@ -579,14 +580,27 @@ static int ensure_kev(gga_graph *g, uint32_t need)
     return GGA_OK;
 }
 
-static uint32_t wl_grid(gga_graph *g, int persistent, int bfs, int count)
+static int block_index(uint32_t block) { return block == 128 ? 0 : block == 512 ? 2 : 1; }
+
+static uint32_t wl_grid(gga_graph *g, int persistent, int bfs, int count, uint32_t block)
 {
-    uint32_t *slot = persistent ? &g->p_grid[bfs][count] : &g->wl_grid[bfs][count];
-    if (*slot == 0) *slot = ggak_wl_grid(g->device, persistent, bfs, count);
+    uint32_t *slot = persistent ? &g->p_grid[bfs][count] : &g->wl_grid[block_index(block)][bfs][count];
+    if (*slot == 0) *slot = ggak_wl_grid(g->device, persistent, bfs, count, persistent ? 256u : block);
     return *slot;
 }
 
-uint32_t gga_graph_default_grid(gga_graph *g, int bfs) { return wl_grid(g, 0, bfs ? 1 : 0, 0); }
+/* Worklist workgroup size: opts.block, else the tuned one, else 256. */
+static uint32_t wl_block(const gga_graph *g, const gga_opts *o)
+{
+    return o && o->block ? o->block : g->tuned_block ? g->tuned_block : 256u;
+}
+
+uint32_t gga_graph_default_grid(gga_graph *g, int bfs, uint32_t block)
+{
+    return wl_grid(g, 0, bfs ? 1 : 0, 0, block ? block : wl_block(g, NULL));
+}
+
+void gga_graph_set_tuned_block(gga_graph *g, uint32_t block) { g->tuned_block = block; }
 
 /* Mesh-like: maximum degree within a few times the mean.  Such graphs have
  * long shortest paths and thin frontiers, so steps are many and small. */
@@ -677,7 +691,16 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
         }
         s->iterations = it;
     } else if (mode == GGA_MODE_WORKLIST) {
-        const uint32_t grid = o->grid ? o->grid : g->tuned_grid ? g->tuned_grid : wl_grid(g, 0, bfs, count);
+        const uint32_t block = wl_block(g, o);
+        if (block != 128u && block != 256u && block != 512u) {
+            rc = GGA_EINVAL;
+            gga_set_error("workgroup size %u: the worklist kernel runs 128, 256 or 512 threads", block);
+            goto fail;
+        }
+        /* a tuned grid belongs to the tuned workgroup size */
+        const uint32_t grid = o->grid ? o->grid
+                              : (g->tuned_grid && !o->block) ? g->tuned_grid
+                                                              : wl_grid(g, 0, bfs, count, block);
         if (grid == 0) {
             rc = GGA_EHIP;
             gga_set_error("occupancy query failed");
@@ -712,7 +735,7 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
                 const uint32_t abs_step = step + k;
                 gga_wg_record *tl = abs_step < tl_steps ? d_tl + (uint64_t)abs_step * grid : NULL;
                 if (own) HIP_TRY(hipEventRecord(g->kev[3 * k], g->stream));
-                K_TRY(ggak_wl_step(dg, dw, abs_step % 6u, bfs, count, grid, tl, g->stream));
+                K_TRY(ggak_wl_step(dg, dw, abs_step % 6u, bfs, count, grid, block, tl, g->stream));
                 if (own) HIP_TRY(hipEventRecord(g->kev[3 * k + 1], g->stream));
                 K_TRY(ggak_compact(dg, dw, abs_step % 6u, g->stream));
                 K_TRY(ggak_refill(dg, dw, abs_step % 6u, g->stream));
@@ -764,7 +787,7 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
             s->conv_steps = ns;
         }
     } else { /* GGA_MODE_PERSISTENT */
-        const uint32_t maxg = wl_grid(g, 1, bfs, count);
+        const uint32_t maxg = wl_grid(g, 1, bfs, count, 256u);
         /* half the grid once a query on this graph ran more than 1024 steps:
          * such frontiers are thin, and every step ends in a grid barrier whose
          * cost grows with the grid (pure 4096^2 grid BFS 102 -> 84 ms, SSSP

@@ -7,10 +7,12 @@
  * 10 buckets of bfs_sssp.cu:166) and the hand sweep over NodeSplitLevel in
  * mysc (mysc:20-51).  The histogram threshold is the tuner's first candidate
  * (its seed); every candidate is timed on one query and the fastest kept.
- * The choice never changes a result -- every threshold, grid and switch gives
- * the same distances -- only the time.  Workgroups stay at 256 threads (four
- * waves; the per-wave LDS staging is sized for it); the grid, i.e. the
- * workgroups resident per CU, is tuned instead.
+ * The choice never changes a result -- every threshold, workgroup size, grid
+ * and switch gives the same distances -- only the time.  The worklist kernel
+ * is built for workgroups of 128, 256 and 512 threads (2, 4, 8 waves with a
+ * per-wave LDS push buffer each); the tuner times the three at their
+ * occupancy grids, then the grid (workgroups resident per CU) for the one
+ * it keeps.
  */
 #define _GNU_SOURCE
 #include <stdlib.h>
@@ -77,6 +79,8 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
     const int bfs = (flags & GGA_TUNE_BFS) ? 1 : 0;
     gga_graph_set_tuned(g, 0, 0, 0);
     gga_graph_set_tuned_sssp(g, -1, 0);
+    gga_graph_set_tuned_block(g, 0);
+    r->block = 256u;
     gga_opts o;
     gga_opts_init(&o);
     o.mode = GGA_MODE_WORKLIST;
@@ -123,7 +127,21 @@ int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r)
         r->layout = gga_tune_layout(g);
         goto layout;
     }
-    const uint32_t g0 = gga_graph_default_grid(g, bfs);
+    /* workgroup size at its occupancy grid: 128, 256 (timed already: the
+     * threshold sweep ran it), 512 threads */
+    r->block_ms[1] = best;
+    for (uint32_t bi = 0; bi < 3; bi += 2) {
+        float ms;
+        o.block = bi == 0 ? 128u : 512u;
+        if ((rc = time_query(g, src, bfs, flags, &o, &ms))) return rc;
+        r->block_ms[bi] = ms;
+        if (ms < best) {
+            best = ms;
+            r->block = o.block;
+        }
+    }
+    o.block = r->block;
+    const uint32_t g0 = gga_graph_default_grid(g, bfs, r->block);
     r->grid = g0;
     for (uint32_t div = 2; div <= 4 && g0 / div >= 64u; div *= 2) {
         float ms;
@@ -204,5 +222,6 @@ layout:
         r->layout_ms[1] = lt[1];
     }
     gga_graph_set_tuned(g, r->grid, r->dense_min, (flags & GGA_TUNE_LAYOUT) ? r->layout : 0u);
+    gga_graph_set_tuned_block(g, r->block);
     return GGA_OK;
 }

@@ -99,8 +99,20 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
         bool again = false;
         if (!BFS && OUT == OUT_FLAG) again = w.flag[own] == (uint8_t)it;
         if (!BFS && OUT == OUT_QUEUE) again = w.mark[own] == it;
-        n = du < inf && !again ? ee - eb : 0u;
-        if (w.last && du < inf && !again) w.last[own] = du;
+        // Near-far: only an owner below the threshold is expanded.  Every
+        // vertex listed by a stamp or a refill is; a flag byte repeats every
+        // 256 steps, so a deferred vertex (dist >= thr) can also match the
+        // step's byte and be listed.  Expanding it could go wrong: a
+        // concurrent improvement of it would be >= thr, deferred rather than
+        // stamped, and found later only through dist < last -- but with
+        // several chunks reading different distances, the chunk that saw the
+        // newer one may write last[] after the others, leaving dist == last
+        // while edges expanded with the older distance are never relaxed
+        // again.  Skipped, it is still deferred (dist < last) and a refill
+        // lists it when the threshold passes it.
+        const bool go = du < inf && !again && du < thr;
+        n = go ? ee - eb : 0u;
+        if (w.last && go) w.last[own] = du;
     }
     const uint32_t incl = wave_incl_scan(n);
     const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
@@ -177,7 +189,7 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
 template <bool COUNT>
 __device__ __forceinline__ void publish_counters(const gga_dev_work &w, const Counters &c) {
     if (COUNT && lane_id() == 0) {
-        const uint32_t slot = (blockIdx.x * WAVES + threadIdx.x / GGA_WAVE) % GGA_CNT_SLOTS;
+        const uint32_t slot = (blockIdx.x * (blockDim.x / GGA_WAVE) + threadIdx.x / GGA_WAVE) % GGA_CNT_SLOTS;
         uint64_t *s = w.cnt + 4u * slot;
         if (c.edges) atomic_add_u64(&s[0], c.edges);
         if (c.chunks) atomic_add_u64(&s[1], c.chunks);
@@ -268,8 +280,9 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     // 2.17 / 2.14 / 2.26 ms).  Uniform small chunks (meshes, bmin > 1): few
     // large batches -- every batch is a chain of dependent loads -- with one
     // batch per 1/1024 of the list, but at least bmin chunks
-    const uint32_t waves = gridDim.x * WAVES;
-    const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
+    const uint32_t wpb = blockDim.x / GGA_WAVE;   // waves per workgroup (128, 256 or 512 threads)
+    const uint32_t waves = gridDim.x * wpb;
+    const uint32_t wid = blockIdx.x * wpb + threadIdx.x / GGA_WAVE;
     uint32_t bsz = w.bmin > 1u ? nq >> 10 : nq / (16u * waves);
     bsz = bsz < w.bmin ? w.bmin : bsz;
     bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
@@ -295,7 +308,8 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
 // One byte per vertex, so the two passes over the stamps read n bytes, not
 // 4n.  The byte repeats every 256 steps of a query: a vertex last stamped 256
 // steps before is listed again -- one extra expansion at an unchanged
-// distance, no effect on any result.
+// distance (none at all for a near-far vertex at or above the threshold,
+// see expand_batch), no effect on any result.
 struct StampMarked {
     const uint8_t *flag;
     uint8_t it8;
@@ -343,10 +357,11 @@ __device__ __forceinline__ uint32_t hw_place() {
 // duration and placement to tl[blockIdx.x].  Every variant keeps eight waves
 // per SIMD (at most 64 VGPRs; the counting variants would otherwise take 62-65
 // and drop to seven): the relax is bound by the latency of its gathers.
-template <bool BFS, bool COUNT, bool TL>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
+// BS: workgroup size, 128, 256 or 512 threads (the tuner times all three).
+template <bool BFS, bool COUNT, bool TL, uint32_t BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl) {
-    __shared__ uint32_t s_pbuf[WAVES][PBUF];
+    __shared__ uint32_t s_pbuf[BS / GGA_WAVE][PBUF];
     uint64_t t0 = 0;
     if (TL && threadIdx.x == 0) t0 = wall_clock64();
     const uint32_t nq = step_len(w, step);
@@ -776,33 +791,37 @@ extern "C" int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int coun
     return launch_status();
 }
 
-// Kernel variants by (bfs, count).
-template <bool BFS, bool COUNT>
-static const void *wl_fn() {
-    return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, false>);
+// Kernel variants by (bfs, count, instrumented) and workgroup size.
+template <bool BFS, bool COUNT, bool TL>
+static const void *wl_fn(uint32_t block) {
+    switch (block) {
+    case 128: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 128>);
+    case 512: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 512>);
+    default: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 256>);
+    }
 }
 template <bool BFS, bool COUNT>
 static const void *persistent_fn() {
     return reinterpret_cast<const void *>(&k_persistent<BFS, COUNT>);
 }
-static const void *pick_kernel(int persistent, int bfs, int count) {
-    static const void *const wl[4] = {wl_fn<false, false>(), wl_fn<false, true>(), wl_fn<true, false>(),
-                                      wl_fn<true, true>()};
-    static const void *const pk[4] = {persistent_fn<false, false>(), persistent_fn<false, true>(),
-                                      persistent_fn<true, false>(), persistent_fn<true, true>()};
-    const int i = (bfs ? 2 : 0) | (count ? 1 : 0);
-    return persistent ? pk[i] : wl[i];
+static const void *pick_kernel(int persistent, int bfs, int count, uint32_t block) {
+    if (persistent) {
+        static const void *const pk[4] = {persistent_fn<false, false>(), persistent_fn<false, true>(),
+                                          persistent_fn<true, false>(), persistent_fn<true, true>()};
+        return pk[(bfs ? 2 : 0) | (count ? 1 : 0)];
+    }
+    if (bfs) return count ? wl_fn<true, true, false>(block) : wl_fn<true, false, false>(block);
+    return count ? wl_fn<false, true, false>(block) : wl_fn<false, false, false>(block);
 }
 
 extern "C" int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, int bfs, int count,
-                            uint32_t grid, gga_wg_record *tl, hipStream_t s) {
-    if (grid == 0) return (int)hipErrorInvalidValue;
+                            uint32_t grid, uint32_t block, gga_wg_record *tl, hipStream_t s) {
+    if (grid == 0 || (block != 128 && block != 256 && block != 512)) return (int)hipErrorInvalidValue;
     void *args[] = {&g, &w, &step, &tl};
     // the instrumented variant always counts (its timeline goes with the work counters)
-    const void *fn = tl ? (bfs ? reinterpret_cast<const void *>(&k_wl_step<true, true, true>)
-                               : reinterpret_cast<const void *>(&k_wl_step<false, true, true>))
-                        : pick_kernel(0, bfs, count);
-    return (int)hipLaunchKernel(fn, dim3(grid), dim3(BLOCK), args, 0, s);
+    const void *fn = tl ? (bfs ? wl_fn<true, true, true>(block) : wl_fn<false, true, true>(block))
+                        : pick_kernel(0, bfs, count, block);
+    return (int)hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
 }
 
 // Convergence of one step against the final distances (printConv_rtime,
@@ -852,11 +871,13 @@ extern "C" int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t step, hipS
     return launch_status();
 }
 
-extern "C" uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count) {
+extern "C" uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count, uint32_t block) {
     int cus = 0, per = 0;
+    if (persistent) block = BLOCK;
+    if (block != 128 && block != 256 && block != 512) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    const void *fn = pick_kernel(persistent, bfs, count);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, 0) != hipSuccess) return 0;
+    const void *fn = pick_kernel(persistent, bfs, count, block);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, (int)block, 0) != hipSuccess) return 0;
     if (per <= 0) return 0;
     // A grid barrier needs every workgroup resident at once.  On gfx950 the
     // occupancy query (7-8 per CU for these kernels) overstates what the
@@ -877,7 +898,7 @@ extern "C" int ggak_persistent(gga_dev_graph g, gga_dev_work w, int bfs, int cou
         khz = 100000;  // 100 MHz
     uint64_t spin_ticks = (uint64_t)spin_limit_ms * (uint64_t)khz;
     void *args[] = {&g, &w, &max_steps, &spin_ticks};
-    hipError_t e = hipLaunchCooperativeKernel(pick_kernel(1, bfs, count), dim3(grid), dim3(BLOCK),
+    hipError_t e = hipLaunchCooperativeKernel(pick_kernel(1, bfs, count, BLOCK), dim3(grid), dim3(BLOCK),
                                               args, 0, s);
     if (e != hipSuccess) return (int)e;
     return launch_status();

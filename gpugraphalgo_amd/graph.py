@@ -295,12 +295,14 @@ class DeviceGraph:
 
     def tune(self, src: int, bfs: bool = False, layout: bool = False, multi: bool = False) -> dict:
         """gga_tune: times split thresholds (seeded by the histogram heuristic),
-        grid and push/compaction switch on queries from src (multi: 32-lane
-        multi-source runs instead); keeps the best."""
+        workgroup size, grid and push/compaction switch on queries from src
+        (multi: 32-lane multi-source runs instead); keeps the best."""
         r = _lib.TuneResult()
         flags = (1 if bfs else 0) | (2 if layout else 0) | (4 if multi else 0)
         check(load().gga_tune(self._h, src, flags, ctypes.byref(r)), "gga_tune")
-        out = {f: getattr(r, f) for f, _ in r._fields_ if f not in ("tried_mda", "tried_ms", "layout_ms")}
+        out = {f: getattr(r, f) for f, _ in r._fields_
+               if f not in ("tried_mda", "tried_ms", "layout_ms", "block_ms")}
+        out["block_ms"] = {128: float(r.block_ms[0]), 256: float(r.block_ms[1]), 512: float(r.block_ms[2])}
         k = min(r.ntried, 16)
         out["tried"] = {int(r.tried_mda[i]): float(r.tried_ms[i]) for i in range(k)}
         out["layout_ms"] = (float(r.layout_ms[0]), float(r.layout_ms[1]))

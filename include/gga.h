@@ -186,7 +186,8 @@ typedef struct gga_opts {
     int32_t  mode;        /* GGA_MODE_* (default GGA_MODE_AUTO) */
     uint32_t inf;         /* unreached value: GGA_INF (0 -> default) or GGA_REF_INF */
     uint32_t max_iters;   /* 0 -> n + 1 (the serial bound, graph.h:122) */
-    uint32_t block;       /* workgroup size, 0 -> default (256) */
+    uint32_t block;       /* WORKLIST: workgroup size, 128, 256 or 512 threads (0 -> the tuned
+                             size, else 256; GGA_EINVAL otherwise); PERSISTENT runs 256 */
     uint32_t sync_every;  /* WORKLIST: steps launched per host convergence check (0 -> 8, the
                              first check after one step more than the graph's last query
                              needed, at most 32) */
@@ -421,7 +422,7 @@ int gga_multi_bfs(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *out,
 #define GGA_TUNE_MAX     16
 typedef struct gga_tune_result {
     uint32_t mda;             /* chosen split threshold (the graph is left split with it) */
-    uint32_t grid;            /* chosen worklist grid (workgroups of 256 threads) */
+    uint32_t grid;            /* chosen worklist grid (workgroups of `block` threads) */
     uint32_t dense_min;       /* chosen push / compaction switch */
     uint32_t layout;          /* GGA_MS_VERTEX_MAJOR or GGA_MS_LANE_MAJOR */
     int32_t  mode;            /* chosen GGA_MODE_WORKLIST / GGA_MODE_PERSISTENT (what AUTO then uses) */
@@ -432,6 +433,8 @@ typedef struct gga_tune_result {
     float    layout_ms[2];    /* vertex-major, lane-major (GGA_TUNE_LAYOUT) */
     uint32_t tried_mda[GGA_TUNE_MAX];
     float    tried_ms[GGA_TUNE_MAX];
+    uint32_t block;           /* chosen worklist workgroup size (128, 256 or 512 threads) */
+    float    block_ms[3];     /* best query time at 128, 256, 512 threads (single-source objective) */
 } gga_tune_result;
 int gga_tune(gga_graph *g, uint32_t src, uint32_t flags, gga_tune_result *r);
 /* Engine layout for the graph: the measured one after gga_tune with

@@ -198,3 +198,62 @@ def test_unsplit_hubs_walked_in_pieces(gpu):
         st = dg.split(mda=4096)
         assert st["exec_chunks"] > st["ns_nnode"]
         np.testing.assert_array_equal(dg.sssp(src, mode="worklist")[0], want)
+
+
+@pytest.mark.parametrize("block", [128, 256, 512])
+def test_worklist_workgroup_sizes_equal_oracle(gpu, block):
+    """The worklist kernel at 128 / 256 / 512 threads per workgroup: same
+    distances in every frontier form; an unsupported size is an error."""
+    cases = [(gga.gen_rmat(14, 16, seed=5, wlo=1, whi=100), 64),
+             (gga.gen_grid(90, 80, 0.005, seed=6, wlo=1, whi=1000), 4)]
+    for g, T in cases:
+        src = g.maxdeg()[1]
+        want = oracle.dijkstra(g.row_ptr, g.dst, g.w, src)
+        want_b = oracle.bfs_queue(g.row_ptr, g.dst, src)
+        with gga.DeviceGraph(g) as dg:
+            dg.split(mda=T)
+            for dense in (0, 1, 0xFFFFFFFF):
+                announce(f"block={block} n={g.n} dense_min={dense}")
+                d, _ = dg.sssp(src, mode="worklist", block=block, dense_min=dense, count=True)
+                np.testing.assert_array_equal(d, want)
+                d, _ = dg.bfs(src, mode="worklist", block=block, dense_min=dense)
+                np.testing.assert_array_equal(d, want_b)
+            with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+                dg.sssp(src, mode="worklist", block=192)
+
+
+def test_tuner_picks_a_workgroup_size(gpu):
+    g = gga.gen_rmat(15, 16, seed=9, wlo=1, whi=100)
+    src = g.maxdeg()[1]
+    with gga.DeviceGraph(g) as dg:
+        announce("tune rmat15")
+        tr = dg.tune(src)
+        assert tr["block"] in (128, 256, 512)
+        assert all(ms > 0 for ms in tr["block_ms"].values()), tr["block_ms"]
+        d, _ = dg.sssp(src)   # the tuned size and grid
+        np.testing.assert_array_equal(d, oracle.dijkstra(g.row_ptr, g.dst, g.w, src))
+
+
+def test_near_far_flag_steps_past_the_byte_wrap(gpu):
+    """Near-far SSSP with every step a flag step and one chunk per edge
+    (T = 1), on queries of more than 256 steps: the step stamps' byte wraps,
+    listing vertices whose stamp matches from 256 steps before -- including
+    deferred ones above the threshold, which must not be expanded (a
+    concurrent deferred improvement of such a vertex could otherwise be
+    lost).  The loss depends on timing, so many queries run."""
+    g = gga.gen_rmat(13, 8, seed=77, wlo=1, whi=100000)
+    srcs = [int(s) for s in np.flatnonzero(g.degrees() > 8)[:6]]
+    wrapped = 0
+    with gga.DeviceGraph(g) as dg:
+        dg.split(mda=1)
+        for s in srcs:
+            for inf in (INF, gga.REF_INF):
+                want = oracle.dijkstra(g.row_ptr, g.dst, g.w, s, inf=inf)
+                for delta in (7, 100):
+                    announce(f"near-far flags T=1 src={s} delta={delta} inf={inf}")
+                    d, st = dg.sssp(s, mode="worklist", inf=inf, dense_min=1, delta=delta)
+                    np.testing.assert_array_equal(d, want, err_msg=f"src={s} delta={delta} inf={inf}")
+                    wrapped += st["iterations"] > 256
+            d, st = dg.sssp(s, mode="persistent", dense_min=1, delta=7)
+            np.testing.assert_array_equal(d, oracle.dijkstra(g.row_ptr, g.dst, g.w, s))
+    assert wrapped >= 4, f"only {wrapped} queries ran past 256 steps"
