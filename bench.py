@@ -13,10 +13,11 @@ A step is one full SSSP query (distance init -> convergence) with the graph
 already resident in HBM and the result left in HBM.  value = GTEPS =
 steps * m_reach / wall time of the K timed steps.  roofline.achieved =
 algorithmic bytes of the relax kernel (per launch) / its average launch
-duration, both from the timed steps: bytes from the device work counters
-(12 B per edge inspected, 16 B per chunk expanded, 8 B per successful
-relaxation), durations from HIP events recorded by the library on the stream
-the kernel runs on.  cpu_baseline = the oracle's restatement of the
+duration, both from instrumented queries run right after the timed ones:
+bytes from the device work counters (12 B per edge inspected, 16 B per chunk
+expanded, 8 B per successful relaxation), durations from HIP events the
+library records around each relax launch on the stream the kernel runs on
+(left out of the timed steps: each event adds a dispatch gap).  cpu_baseline = the oracle's restatement of the
 reference's serial SSSP (graph.h:115-148), one thread, full run on the same
 graph and source.
 
@@ -56,38 +57,45 @@ def algorithmic_bytes(st: dict, mode: str, bfs: bool) -> int:
 
 
 def run_config(dg, src, T, mode, steps, warmup, bfs=False):
+    """W warmup + K timed queries, uninstrumented; then instrumented ones
+    (work counters + a HIP event pair around every relax launch, on the
+    graph's stream) for the byte model and the relax kernel's durations --
+    the events add dispatch gaps, so they stay out of the timed K."""
     dg.split(mda=T)
     run = dg.bfs if bfs else dg.sssp
     for _ in range(warmup):
-        run(src, mode=mode, keep_on_device=True, count=True)
+        run(src, mode=mode, keep_on_device=True)
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        _, st = run(src, mode=mode, keep_on_device=True, count=True)
+        _, st = run(src, mode=mode, keep_on_device=True)
         stats.append(st)
     wall = time.perf_counter() - t0
-    return wall, stats
+    counted = [run(src, mode=mode, keep_on_device=True, count=True)[1] for _ in range(max(3, steps // 4))]
+    return wall, stats, counted
 
 
-def summarize(wall, stats, mode, bfs):
-    k = len(stats)
+def summarize(wall, stats, counted, mode, bfs):
+    k, kc = len(stats), len(counted)
     m_reach = stats[0]["m_reach"]
-    ktime_ms = sum(s["ktime_ms"] for s in stats) / k
-    launches = sum(s["launches"] for s in stats) / k
-    bytes_q = sum(algorithmic_bytes(s, mode, bfs) for s in stats) / k
+    ktime_ms = sum(s["ktime_ms"] for s in counted) / kc
+    launches = sum(s["launches"] for s in counted) / kc
+    bytes_q = sum(algorithmic_bytes(s, mode, bfs) for s in counted) / kc
     return {
         "ms_per_step": wall / k * 1e3,
         "gteps": k * m_reach / wall / 1e9,
         "m_reach": m_reach,
         "device_ms": sum(s["time_ms"] for s in stats) / k,
+        "counted_queries": kc,
+        "counted_device_ms": sum(s["time_ms"] for s in counted) / kc,
         "ktime_ms": ktime_ms,
         "launches": launches,
         "bytes_per_query": bytes_q,
         "achieved_gbs": bytes_q / (ktime_ms * 1e-3) / 1e9 if ktime_ms > 0 else 0.0,
         "iterations": stats[0]["iterations"],
-        "edges_inspected": sum(s["edges_inspected"] for s in stats) / k,
-        "chunks_expanded": sum(s["chunks_expanded"] for s in stats) / k,
-        "relaxations": sum(s["relaxations"] for s in stats) / k,
+        "edges_inspected": sum(s["edges_inspected"] for s in counted) / kc,
+        "chunks_expanded": sum(s["chunks_expanded"] for s in counted) / kc,
+        "relaxations": sum(s["relaxations"] for s in counted) / kc,
         "mode": stats[0]["mode"],
     }
 
@@ -102,6 +110,21 @@ def cold_first_query(gga, g, src, T, mode):
         wall = time.perf_counter() - t0
     return {"ms": round(wall * 1e3, 3), "device_ms": round(st["time_ms"], 3),
             "gteps": round(st["m_reach"] / wall / 1e9, 3), "steps": st["iterations"]}
+
+
+def unsplit_query(dg, src, mode, split_ms, reps=10):
+    """L = 0, the reference's no-split baseline (one chunk per vertex, the
+    97 670-edge hub included): the kernels walk its hubs in pieces of at most
+    GGA_PIECE_MAX edges, spread over many waves."""
+    st = dg.split()
+    for _ in range(2):
+        dg.sssp(src, mode=mode, keep_on_device=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dg.sssp(src, mode=mode, keep_on_device=True)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    return {"ms": round(ms, 3), "vs_split": round(ms / split_ms, 3), "chunks": st["ns_nnode"],
+            "walked_chunks": st["exec_chunks"], "max_chunk": st["max_chunk"]}
 
 
 def distinct_sources(dg, g, mode, k=20):
@@ -392,8 +415,8 @@ def main() -> int:
     with gga.DeviceGraph(g) as dg:
         for T in sweep:
             log(f"T={T} mode={args.mode}: {args.warmup} warmup + {args.steps} timed steps")
-            wall, stats = run_config(dg, src, T, args.mode, args.steps, args.warmup)
-            results[T] = summarize(wall, stats, "topo" if stats[0]["mode"] == 0 else "worklist", False)
+            wall, stats, counted = run_config(dg, src, T, args.mode, args.steps, args.warmup)
+            results[T] = summarize(wall, stats, counted, "topo" if stats[0]["mode"] == 0 else "worklist", False)
             results[T]["split"] = dict(dg.split_stats)
             log(f"T={T}: {results[T]['ms_per_step']:.3f} ms/step, {results[T]['gteps']:.2f} GTEPS, "
                 f"{results[T]['achieved_gbs']:.0f} GB/s algorithmic, {results[T]['iterations']} steps")
@@ -404,6 +427,7 @@ def main() -> int:
         ver = dg.verify(src, bfs=False)
         assert ver["nerr"] == 0 and ver["untight"] == 0 and ver["src_ok"], ver
         distinct = None if args.quick else distinct_sources(dg, g, args.mode)
+        unsplit = None if args.quick else unsplit_query(dg, src, args.mode, r["ms_per_step"])
     cold = None if args.quick else cold_first_query(gga, g, src, best, args.mode)
 
     cpu = None
@@ -471,6 +495,7 @@ def main() -> int:
             "sweep_gteps": {str(t): round(results[t]["gteps"], 3) for t in results},
             "cold_first_query": cold,
             "distinct_sources": distinct,
+            "unsplit": unsplit,
         },
         "roofline": {
             "bound": "hbm",
@@ -486,6 +511,9 @@ def main() -> int:
             "bytes_per_launch": round(r["bytes_per_query"] / max(r["launches"], 1)),
             "launches_per_step": r["launches"],
             "avg_launch_ms": round(r["ktime_ms"] / max(r["launches"], 1), 5),
+            "measured_over": (f"{r['counted_queries']} instrumented queries after the timed ones (work "
+                              f"counters; HIP events around each relax launch on the graph's stream), "
+                              f"{r['counted_device_ms']:.3f} ms each"),
             "work": {"edges_inspected": r["edges_inspected"], "chunks_expanded": r["chunks_expanded"],
                      "relaxations": r["relaxations"]},
             "random_access": random_access,

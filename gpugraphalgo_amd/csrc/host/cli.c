@@ -126,6 +126,7 @@ int main(int argc, char **argv)
     gga_opts o;
     gga_opts_init(&o);
     o.inf = inf;
+    o.count = 1;   /* the reference's ktime line needs the per-launch times */
     if (!strcmp(mode_s, "topo")) o.mode = GGA_MODE_TOPO;
     else if (!strcmp(mode_s, "worklist")) o.mode = GGA_MODE_WORKLIST;
     else if (!strcmp(mode_s, "persistent")) o.mode = GGA_MODE_PERSISTENT;

@@ -657,6 +657,10 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
     uint64_t *d_conv = NULL;
     uint64_t tl_bytes = 0, fin_bytes = 0, conv_bytes = 0;
     const int bfs = p->bfs, mode = p->mode, count = p->count, own = p->own;
+    /* per-launch timing events (the relax time ktime_ms, the list time
+     * ctime_ms) only in counted runs: each event is a barrier packet in the
+     * stream, ~5 us of dispatch gap, three per step */
+    const int tev = own && count;
     const uint32_t max_iters = p->max_iters;
     float ktime = 0.f;
     *adv = 0;
@@ -665,18 +669,18 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
     K_TRY(ggak_query_init(dg, dw, p->src, p->epoch, g->stream));
     if (mode == GGA_MODE_TOPO) {
         /* the reference's loop: changed = false; relax; read changed (bfs_sssp.cu:372-468) */
-        if (own && (rc = ensure_kev(g, 2))) goto fail;
+        if (tev && (rc = ensure_kev(g, 2))) goto fail;
         uint32_t it = 0;
         for (;;) {
             it++;
             g->h_pin[0] = 0;
             HIP_TRY(hipMemcpyAsync(dw.ctl + GGA_CTL_CHANGED, g->h_pin, 4, hipMemcpyHostToDevice, g->stream));
-            if (own) HIP_TRY(hipEventRecord(g->kev[0], g->stream));
+            if (tev) HIP_TRY(hipEventRecord(g->kev[0], g->stream));
             K_TRY(ggak_topo_step(dg, dw, bfs, count, g->stream));
-            if (own) HIP_TRY(hipEventRecord(g->kev[1], g->stream));
+            if (tev) HIP_TRY(hipEventRecord(g->kev[1], g->stream));
             HIP_TRY(hipMemcpyAsync(g->h_pin + 1, dw.ctl + GGA_CTL_CHANGED, 4, hipMemcpyDeviceToHost, g->stream));
             HIP_TRY(hipStreamSynchronize(g->stream));
-            if (own) {
+            if (tev) {
                 float t = 0.f;
                 HIP_TRY(hipEventElapsedTime(&t, g->kev[0], g->kev[1]));
                 ktime += t;
@@ -713,7 +717,7 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
         const uint32_t per = o->sync_every ? o->sync_every : 8u;
         const uint32_t prev = g->last_steps[bfs ? 1 : 0];
         uint32_t nb = o->sync_every ? per : (prev ? (prev < 31u ? prev + 1u : 32u) : per);
-        if (own && (rc = ensure_kev(g, 3 * (nb > per ? nb : per)))) goto fail;
+        if (tev && (rc = ensure_kev(g, 3 * (nb > per ? nb : per)))) goto fail;
         /* instrumentation buffers (freed below, also on failure) */
         const uint32_t tl_steps = own && o->timeline && o->timeline_cap >= grid ? o->timeline_cap / grid : 0u;
         const uint32_t conv_n = own && o->conv_final && o->conv_rows ? o->conv_cap : 0u;
@@ -734,12 +738,12 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
             for (uint32_t k = 0; k < nb; k++) {
                 const uint32_t abs_step = step + k;
                 gga_wg_record *tl = abs_step < tl_steps ? d_tl + (uint64_t)abs_step * grid : NULL;
-                if (own) HIP_TRY(hipEventRecord(g->kev[3 * k], g->stream));
+                if (tev) HIP_TRY(hipEventRecord(g->kev[3 * k], g->stream));
                 K_TRY(ggak_wl_step(dg, dw, abs_step % 6u, bfs, count, grid, block, tl, g->stream));
-                if (own) HIP_TRY(hipEventRecord(g->kev[3 * k + 1], g->stream));
+                if (tev) HIP_TRY(hipEventRecord(g->kev[3 * k + 1], g->stream));
                 K_TRY(ggak_compact(dg, dw, abs_step % 6u, g->stream));
                 K_TRY(ggak_refill(dg, dw, abs_step % 6u, g->stream));
-                if (own) HIP_TRY(hipEventRecord(g->kev[3 * k + 2], g->stream));
+                if (tev) HIP_TRY(hipEventRecord(g->kev[3 * k + 2], g->stream));
                 if (abs_step < conv_n) K_TRY(ggak_conv(dw.dist, d_fin, g->n, d_conv + 2ull * abs_step, g->stream));
             }
             step += nb;
@@ -747,7 +751,7 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
                                    g->stream));
             HIP_TRY(hipMemcpyAsync(g->h_pin + 1, dw.ctl + GGA_CTL_FAR, 4, hipMemcpyDeviceToHost, g->stream));
             HIP_TRY(hipStreamSynchronize(g->stream));
-            if (own)
+            if (tev)
                 for (uint32_t k = 0; k < nb; k++) {
                     float t = 0.f, tc = 0.f;
                     HIP_TRY(hipEventElapsedTime(&t, g->kev[3 * k], g->kev[3 * k + 1]));

@@ -130,28 +130,57 @@ __device__ __forceinline__ void compact_scan(const ListOut &o, uint32_t n, uint3
 }
 
 // The chunk ids of one thread's vertices at list position pos.  A vertex with
-// few chunks is written by its thread; a hub's hundreds of chunk ids are
-// written by the whole wave, 64 per round.
+// few chunks is written by its thread.  Longer runs (hubs' hundreds of chunk
+// ids) are staged in LDS and written by every wave of the workgroup, 64 ids
+// per round: in R-MAT the hubs sit at the lowest ids, so one tile -- and
+// within it the first wave's vertices -- holds most of them, and written by
+// their own wave they made that wave the whole pass's long pole (62 of the
+// 154 us of a config-2 query's list writes, in one pass).  All threads of
+// the workgroup must call this (it synchronises).
+constexpr uint32_t CBIG = 256;   // long runs staged per round
 __device__ __forceinline__ void write_items(const ListOut &o, const uint32_t *cs, const uint32_t *nch,
                                             uint32_t pos) {
+    __shared__ uint32_t s_bpos[CBIG], s_bcs[CBIG], s_bcnt[CBIG];
+    __shared__ uint32_t s_nb;
     uint32_t bpos[CITEMS];
+    uint32_t pending = 0;   // bit i: item i is a long run not staged yet
 #pragma unroll
     for (uint32_t i = 0; i < CITEMS; i++) {
         bpos[i] = pos;
         if (nch[i] <= CSMALL)
             for (uint32_t k = 0; k < nch[i]; k++) queue_store(o.q, pos + k, cs[i] + k, o.qcap, o.ctl);
+        else
+            pending |= 1u << i;
         pos += nch[i];
     }
+    const uint32_t wv = threadIdx.x / GGA_WAVE, nw = blockDim.x / GGA_WAVE;
+    for (;;) {
+        if (threadIdx.x == 0) s_nb = 0;
+        __syncthreads();
+        while (pending) {
+            const uint32_t i = (uint32_t)__builtin_ctz(pending);
+            const uint32_t e = atomicAdd(&s_nb, 1u);
+            if (e >= CBIG) break;   // full: stays pending for the next round
+            uint32_t p = 0, c = 0, k = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < CITEMS; i++) {
-        uint64_t big = __ballot(nch[i] > CSMALL);
-        while (big) {
-            const uint32_t j = (uint32_t)__builtin_ctzll(big);
-            big &= big - 1;
-            const uint32_t p = wave_bcast(bpos[i], j), c0 = wave_bcast(cs[i], j);
-            const uint32_t cnt = wave_bcast(nch[i], j);
+            for (uint32_t j = 0; j < CITEMS; j++)
+                if (j == i) {
+                    p = bpos[j];
+                    c = cs[j];
+                    k = nch[j];
+                }
+            s_bpos[e] = p;
+            s_bcs[e] = c;
+            s_bcnt[e] = k;
+            pending &= pending - 1u;
+        }
+        __syncthreads();
+        const uint32_t nb = s_nb < CBIG ? s_nb : CBIG;
+        for (uint32_t e = wv; e < nb; e += nw) {
+            const uint32_t p = s_bpos[e], c0 = s_bcs[e], cnt = s_bcnt[e];
             for (uint32_t k = lane_id(); k < cnt; k += GGA_WAVE) queue_store(o.q, p + k, c0 + k, o.qcap, o.ctl);
         }
+        if (!__syncthreads_or(pending != 0u)) break;
     }
 }
 

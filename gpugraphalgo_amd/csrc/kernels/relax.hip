@@ -68,11 +68,31 @@ constexpr uint32_t UNROLL = GGA_UNROLL; // edge slots per lane per round in expa
 
 enum { OUT_CHANGED = 0, OUT_QUEUE = 1, OUT_FLAG = 2 };
 
+// Work counters of one wave: wave-uniform values, kept in scalar registers
+// (32 bits: one wave's share of one step), added to the striped 64-bit
+// counters once per wave at the end.
 struct Counters {
-    uint64_t edges = 0, chunks = 0, relax = 0;
+    uint32_t edges = 0, chunks = 0, relax = 0;
 };
 
-// Expand one batch of chunks (this lane's chunk c when valid).
+// A lane's chunk in a batch: its edge range [eb, ee) and owner.
+struct ChunkDesc {
+    uint32_t eb = 0, ee = 0, own = 0;
+    bool valid = false;
+};
+
+__device__ __forceinline__ ChunkDesc load_chunk(const gga_dev_graph &g, uint32_t c, bool valid) {
+    ChunkDesc d;
+    d.valid = valid;
+    if (valid) {
+        d.eb = stream_load(&g.cbeg[c]);
+        d.ee = stream_load(&g.cbeg[c + 1]);
+        d.own = stream_load(&g.cown[c]);
+    }
+    return d;
+}
+
+// Expand one batch of chunks (this lane's chunk d when d.valid).
 // thr (near-far): a destination whose new distance is >= thr is lowered but
 // not listed -- it is deferred until the threshold passes it, and w.last
 // (when set) records the distance each owner was expanded at, so that a
@@ -80,17 +100,17 @@ struct Counters {
 // worklist (every improvement is listed).
 template <bool BFS, bool COUNT, int OUT>
 __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_dev_work &w,
-                                             uint32_t c, bool valid, uint32_t it, uint32_t *pbuf,
+                                             const ChunkDesc &d, uint32_t it, uint32_t *pbuf,
                                              uint32_t &pcount, uint32_t *qout, uint32_t *tail,
                                              bool &improved, Counters &cnt, uint32_t thr = 0xFFFFFFFFu) {
     const uint32_t l = lane_id();
     uint32_t *__restrict__ dist = w.dist;
     const uint32_t inf = w.inf;
     uint32_t eb = 0, n = 0, du = inf;
-    if (valid) {
-        eb = stream_load(&g.cbeg[c]);
-        const uint32_t ee = stream_load(&g.cbeg[c + 1]);
-        const uint32_t own = stream_load(&g.cown[c]);
+    if (d.valid) {
+        eb = d.eb;
+        const uint32_t ee = d.ee;
+        const uint32_t own = d.own;
         du = dist[own];
         // SSSP: an owner already lowered and listed again during this step is
         // expanded in the next one with its newer distance, which relaxes
@@ -117,8 +137,8 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
     const uint32_t incl = wave_incl_scan(n);
     const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
     if (COUNT) {
-        cnt.edges += total;
-        cnt.chunks += (uint32_t)__popcll(__ballot(n > 0));
+        cnt.edges += __builtin_amdgcn_readfirstlane(total);
+        cnt.chunks += __builtin_amdgcn_readfirstlane((uint32_t)__popcll(__ballot(n > 0)));
     }
     const uint32_t excl = incl - n;
     // UNROLL slots per lane per round.  Slots past the end are clamped to the
@@ -157,7 +177,7 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
                     imp = alt < old;
                 }
             }
-            if (COUNT) cnt.relax += (uint32_t)__popcll(__ballot(imp));
+            if (COUNT) cnt.relax += __builtin_amdgcn_readfirstlane((uint32_t)__popcll(__ballot(imp)));
             if (OUT == OUT_QUEUE) {
                 bool want = imp && alt < thr;
                 if (!BFS && want)
@@ -252,8 +272,8 @@ __global__ __launch_bounds__(BLOCK) void k_topo_step(gga_dev_graph g, gga_dev_wo
     bool improved = false;
     uint32_t pcount = 0;
     Counters cnt;
-    expand_batch<BFS, COUNT, OUT_CHANGED>(g, w, valid ? (uint32_t)c : 0u, valid, 0u, nullptr, pcount,
-                                          nullptr, nullptr, improved, cnt);
+    expand_batch<BFS, COUNT, OUT_CHANGED>(g, w, load_chunk(g, valid ? (uint32_t)c : 0u, valid), 0u, nullptr,
+                                          pcount, nullptr, nullptr, improved, cnt);
     if (__ballot(improved) && lane_id() == 0) w.ctl[GGA_CTL_CHANGED] = 1u;  // *changed = true
     publish_counters<COUNT>(w, cnt);
 }
@@ -292,14 +312,36 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     const uint32_t thr = w.ctl[GGA_CTL_THR(step % 3u)];
     uint32_t pcount = 0;
     bool dummy = false;
-    for (uint32_t b = wid; b < nbatch; b += waves) {
-        const uint32_t idx = b * bsz + l;
-        const bool valid = l < bsz && idx < nq;
-        const uint32_t c = valid ? stream_load(&qin[idx]) : 0u;
+    // Software pipeline over the wave's batches b, b + waves, b + 2 waves...:
+    // while a batch is expanded, the next batch's chunk descriptors and the
+    // list entries of the one after it are already in flight, so a batch
+    // starts at its owners' distances instead of two dependent loads
+    // earlier.  The list and the chunk table do not change during a step.
+    auto entry = [&](uint64_t bb, bool &ok) -> uint32_t {
+        const uint64_t idx = bb * bsz + l;
+        ok = bb < nbatch && l < bsz && idx < nq;
+        return ok ? stream_load(&qin[idx]) : 0u;
+    };
+    bool ok_next;
+    uint32_t c_next;
+    ChunkDesc cur;
+    {
+        bool ok0;
+        const uint32_t c0 = entry(wid, ok0);
+        c_next = entry((uint64_t)wid + waves, ok_next);
+        cur = load_chunk(g, c0, ok0);
+    }
+    for (uint64_t b = wid; b < nbatch; b += waves) {
+        bool ok_after;
+        const uint32_t c_after = entry(b + 2ull * waves, ok_after);
+        const ChunkDesc nxt = load_chunk(g, c_next, ok_next);
         if (flags)
-            expand_batch<BFS, COUNT, OUT_FLAG>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt, thr);
+            expand_batch<BFS, COUNT, OUT_FLAG>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, thr);
         else
-            expand_batch<BFS, COUNT, OUT_QUEUE>(g, w, c, valid, it, pbuf, pcount, qout, tail, dummy, cnt, thr);
+            expand_batch<BFS, COUNT, OUT_QUEUE>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, thr);
+        cur = nxt;
+        c_next = c_after;
+        ok_next = ok_after;
     }
     if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
 }

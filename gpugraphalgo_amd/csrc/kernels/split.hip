@@ -127,8 +127,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_split_scan_down(const uint32_t
 // Chunk descriptors: each wave covers 64 consecutive vertices and writes their
 // chunks (a contiguous id range) with all 64 lanes, so a hub's thousands of
 // chunks cost its wave a few coalesced rounds rather than one lane a loop.
-// owner_mod != 0: item i is vertex i % owner_mod (the part-major table of the
-// XCD-partitioned layout, where items are (part, vertex) pairs).
 __global__ __launch_bounds__(256) void k_split_fill(const uint32_t *__restrict__ row,
                                                     const uint32_t *__restrict__ cstart, uint32_t n,
                                                     uint32_t m, uint32_t *__restrict__ cbeg,

@@ -193,7 +193,8 @@ typedef struct gga_opts {
                              needed, at most 32) */
     uint32_t use_graph;   /* multi-source engines: replay steps from captured hipGraphs (six
                              steps per graph) */
-    uint32_t count;       /* 1 -> collect edges_inspected / chunks_expanded / relaxations */
+    uint32_t count;       /* 1 -> collect edges_inspected / chunks_expanded / relaxations, and
+                             time each relax / list launch (ktime_ms, ctime_ms) */
     uint32_t grid;        /* PERSISTENT / WORKLIST grid in workgroups, 0 -> occupancy bound */
     uint32_t dense_min;   /* WORKLIST / PERSISTENT: a step whose list holds >= dense_min chunks
                              records the next frontier as step stamps + compaction instead of
@@ -221,7 +222,9 @@ typedef struct gga_stats {
     uint32_t iterations;      /* relax steps run (reference `iterations`) */
     uint32_t reached;         /* vertices with dist < inf */
     float    time_ms;         /* device time, dist init -> convergence (reference `time`, bfs_sssp.cu:347-469) */
-    float    ktime_ms;        /* relax-kernel device time (reference `ktime`) */
+    float    ktime_ms;        /* relax-kernel device time (reference `ktime`); measured with
+                                 per-launch events, so only when opts.count is set (they cost
+                                 a few us of dispatch gap each) or in PERSISTENT mode */
     float    sktime_ms;       /* child-sync time: always 0 (no child copies; reference `sktime`) */
     float    ctime_ms;        /* WORKLIST: list-compaction kernel time (after flag steps) */
     int32_t  mode;            /* mode actually used */
