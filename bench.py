@@ -40,7 +40,7 @@ METRIC = "GTEPS + % HBM roofline (1 GPU); served BFS/SSSP queries/s with p50/p99
 HBM_PEAK_GBS = 8000.0
 SWEEP = (32, 64, 128, 256, 1024)
 DEFAULT_T = 128  # best of SWEEP on MI355X (profiles/sweep_rmat22_sssp.txt)
-TRAFFIC = os.path.join(ROOT, "profiles", "k_wl_step_rmat22_sssp_traffic.txt")
+TRAFFIC = os.path.join(ROOT, "profiles", "k_wl_step_rmat22_sssp_counters.txt")
 
 
 def log(msg: str) -> None:
@@ -505,9 +505,11 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_source": (os.path.relpath(TRAFFIC, ROOT) + " (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of "
-                               "k_wl_step on this workload, a committed profile, not measured in this run)")
+            "traffic_source": (os.path.relpath(TRAFFIC, ROOT) + " (rocprofv3 --pmc L2 request counters of "
+                               "k_wl_step on this workload: reads by request size, writes, atomics; a "
+                               "committed profile, not measured in this run)")
                               if traffic is not None else None,
+            "l2_hit_rate": float(read_summary(TRAFFIC)["l2_hit_rate"]) if traffic is not None else None,
             "bytes_per_launch": round(r["bytes_per_query"] / max(r["launches"], 1)),
             "launches_per_step": r["launches"],
             "avg_launch_ms": round(r["ktime_ms"] / max(r["launches"], 1), 5),

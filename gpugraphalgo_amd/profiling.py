@@ -10,6 +10,16 @@
         each counter scaled by what it reads for the copy (bytes / (KiB * 1024)).
         --last N: only the last N dispatches of NAME (one run at the end of the workload).
 
+    python -m gpugraphalgo_amd.profiling counters --kernel NAME --hitmiss D1 --rdsize D2 \
+            --wratom D3 --dram D4 --copy-rd D5 --copy-wr D6 --copy-bytes B --cmd "..."
+        Di/*counter_collection.csv of rocprofv3 --pmc passes (at most four TCC
+        counters each): TCC_HIT/MISS/REQ; TCC_EA0_RDREQ by request size (32,
+        64, 128 B); TCC_EA0_WRREQ (32 / 64 B) and TCC_ATOMIC / TCC_EA0_ATOMIC;
+        DRAM-destined requests -> L2 hit rate, bytes read and written past L2
+        per dispatch of exactly NAME (request counts times sizes: no scale
+        factor), atomics per dispatch; the same size passes over a copy of B
+        bytes check the byte accounting.
+
 Summaries are `key = value` text; read_summary() parses them back.
 """
 from __future__ import annotations
@@ -79,6 +89,74 @@ def summarize_traffic(kernel: str, fetch: str, write: str, copy_fetch: str, copy
     return "\n".join(lines) + "\n"
 
 
+def _per_dispatch(d: str, kernel: str, exact: bool = True) -> dict:
+    """Mean over the dispatches of `kernel` of each counter in d's collection."""
+    path = _one(os.path.join(d, "**", "*counter_collection.csv"))
+    per: dict = {}
+    for r in csv.DictReader(open(path)):
+        name = r["Kernel_Name"].replace("(anonymous namespace)::", "")
+        if (name.split("(")[0].strip() == kernel) if exact else (kernel in name):
+            per.setdefault(r["Dispatch_Id"], {}).setdefault(r["Counter_Name"], 0.0)
+            per[r["Dispatch_Id"]][r["Counter_Name"]] += float(r["Counter_Value"])
+    if not per:
+        raise ValueError(f"no dispatch of {kernel} in {path}")
+    keys = {k for v in per.values() for k in v}
+    return {"_dispatches": len(per), **{k: sum(v.get(k, 0.0) for v in per.values()) / len(per) for k in keys}}
+
+
+def _read_bytes(c: dict) -> float:
+    return (32.0 * c["TCC_EA0_RDREQ_32B_sum"] + 64.0 * c["TCC_EA0_RDREQ_64B_sum"]
+            + 128.0 * c["TCC_EA0_RDREQ_128B_sum"])
+
+
+def _write_bytes(c: dict) -> float:
+    # WRREQ counts 32- and 64-byte transactions; atomics travel on the same
+    # interface as 32-byte ones
+    n64 = c["TCC_EA0_WRREQ_64B_sum"]
+    n32 = c["TCC_EA0_WRREQ_sum"] - n64 - c.get("TCC_EA0_ATOMIC_sum", 0.0)
+    return 64.0 * n64 + 32.0 * n32
+
+
+def summarize_counters(kernel: str, hitmiss: str, rdsize: str, wratom: str, dram: str, copy_rd: str,
+                       copy_wr: str, copy_bytes: int, cmd: str) -> str:
+    hm = _per_dispatch(hitmiss, kernel)
+    rd = _per_dispatch(rdsize, kernel)
+    wa = _per_dispatch(wratom, kernel)
+    dr = _per_dispatch(dram, kernel)
+    cr = _per_dispatch(copy_rd, "copyBuffer", exact=False)
+    cw = _per_dispatch(copy_wr, "copyBuffer", exact=False)
+    # the copy: its largest dispatches are the timed 1 GiB copies
+    rb, wb = _read_bytes(rd), _write_bytes(wa)
+    ab = 32.0 * wa["TCC_EA0_ATOMIC_sum"]   # 32-byte requests on the write interface
+    lines = [
+        "# L2 (TCC) counters per dispatch: hit rate, bytes past L2 by request size, atomics",
+        f"command = {cmd}",
+        f"kernel = {kernel}",
+        f"dispatches = {rd['_dispatches']}",
+        f"l2_requests = {hm['TCC_REQ_sum']:.0f}",
+        f"l2_hits = {hm['TCC_HIT_sum']:.0f}",
+        f"l2_misses = {hm['TCC_MISS_sum']:.0f}",
+        f"l2_hit_rate = {hm['TCC_HIT_sum'] / max(hm['TCC_HIT_sum'] + hm['TCC_MISS_sum'], 1.0):.4f}",
+        f"read_requests_32B = {rd['TCC_EA0_RDREQ_32B_sum']:.0f}",
+        f"read_requests_64B = {rd['TCC_EA0_RDREQ_64B_sum']:.0f}",
+        f"read_requests_128B = {rd['TCC_EA0_RDREQ_128B_sum']:.0f}",
+        f"read_requests_dram = {dr['TCC_EA0_RDREQ_DRAM_sum']:.0f}",
+        f"write_requests = {wa['TCC_EA0_WRREQ_sum']:.0f}",
+        f"write_requests_64B = {wa['TCC_EA0_WRREQ_64B_sum']:.0f}",
+        f"write_requests_dram = {dr['TCC_EA0_WRREQ_DRAM_sum']:.0f}",
+        f"atomics = {wa['TCC_ATOMIC_sum']:.0f}",
+        f"atomics_past_l2 = {wa['TCC_EA0_ATOMIC_sum']:.0f}",
+        f"read_bytes_per_dispatch = {rb:.0f}",
+        f"write_bytes_per_dispatch = {wb:.0f}",
+        f"atomic_bytes_per_dispatch = {ab:.0f}",
+        f"traffic_bytes_per_dispatch = {rb + wb + ab:.0f}",
+        f"copy_bytes = {copy_bytes}",
+        f"copy_read_bytes_counted = {_read_bytes(cr):.0f}",
+        f"copy_write_bytes_counted = {64.0 * cw['TCC_EA0_WRREQ_64B_sum'] + 32.0 * (cw['TCC_EA0_WRREQ_sum'] - cw['TCC_EA0_WRREQ_64B_sum']):.0f}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
 def read_summary(path: str) -> dict:
     out = {}
     for line in open(path):
@@ -99,9 +177,16 @@ def main(argv=None) -> int:
         t.add_argument("--" + k, required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
     t.add_argument("--last", type=int, default=0)
+    c = sub.add_parser("counters")
+    for k in ("kernel", "hitmiss", "rdsize", "wratom", "dram", "copy-rd", "copy-wr", "cmd"):
+        c.add_argument("--" + k, required=True)
+    c.add_argument("--copy-bytes", type=int, required=True)
     a = ap.parse_args(argv)
     if a.what == "stats":
         sys.stdout.write(summarize_stats(a.dir, a.cmd))
+    elif a.what == "counters":
+        sys.stdout.write(summarize_counters(a.kernel, a.hitmiss, a.rdsize, a.wratom, a.dram, a.copy_rd,
+                                            a.copy_wr, a.copy_bytes, a.cmd))
     else:
         sys.stdout.write(summarize_traffic(a.kernel, a.fetch, a.write, a.copy_fetch, a.copy_write,
                                            a.copy_bytes, a.cmd, a.last))
