@@ -280,6 +280,29 @@ def grid_leg(gga, np, quick=False, coop=True):
     return out
 
 
+def engine_leg(gga, np, coop=True, side=4096, steps=600):
+    """One 32-lane engine (16 BFS + 16 SSSP lanes from random sources, seed 5)
+    on the side x side grid of the serving set (weights [1, 1000], seed 17),
+    `steps` steps launched one kernel at a time (no captured graphs): the
+    per-step cost that sets the serving latency of long-diameter graphs."""
+    g = gga.gen_grid(side, side, 0.0, seed=17, wlo=1, whi=1000)
+    rng = np.random.default_rng(5)
+    srcs = rng.choice(g.n, 32, replace=False)
+    with gga.DeviceGraph(g) as dg:
+        dg.split()
+        eng = gga.MultiSourceEngine(dg)
+        try:
+            eng.start({i: int(s) for i, s in enumerate(srcs)}, bfs=set(range(16)))
+            t0 = time.perf_counter()
+            live = eng.run(max_steps=steps, mode="worklist", sync_every=12)
+            wall = time.perf_counter() - t0
+        finally:
+            eng.close()
+    return {"workload": f"32-lane engine on the {side}x{side} grid (16 BFS + 16 SSSP lanes), {steps} steps",
+            "ms": round(wall * 1e3, 3), "us_per_step": round(wall / steps * 1e6, 2),
+            "lanes_still_running": bin(live).count("1")}
+
+
 def serving_leg(gga, np, full=True, quick=False):
     """Config 5 serving: resident R-MAT graphs (s16.. ef16, weights [1,100])
     and 4-neighbour grids (weights [1,1000]), seed 17; BFS/SSSP 50/50, uniform
@@ -373,7 +396,7 @@ def main() -> int:
     ap.add_argument("--serve-reduced", action="store_true",
                     help="serving leg on a reduced config 5 set (R-MAT s16-s20, grids 512^2 and 1024^2) "
                          "instead of the full one (R-MAT s16-s23, grids 512^2 to 4096^2)")
-    ap.add_argument("--leg", choices=("grid", "batched", "serve"),
+    ap.add_argument("--leg", choices=("grid", "batched", "serve", "engine"),
                     help="profiling run: only this leg (configs 3, 4, 5), its JSON object on stdout")
     ap.add_argument("--no-coop", action="store_true",
                     help="with --leg: leave out cooperative launches (rocprofv3 can lose its output at "
@@ -401,7 +424,7 @@ def main() -> int:
     import gpugraphalgo_amd as gga
 
     if args.leg:
-        leg = {"grid": grid_leg, "batched": batched_leg}.get(args.leg)
+        leg = {"grid": grid_leg, "batched": batched_leg, "engine": engine_leg}.get(args.leg)
         res = leg(gga, np, coop=not args.no_coop) if leg else serving_leg(gga, np, full=not args.serve_reduced)
         print(json.dumps({"leg": args.leg, **res}))
         return 0

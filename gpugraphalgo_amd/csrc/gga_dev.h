@@ -81,6 +81,7 @@ typedef struct gga_wg_record {
  *   q0, q1      chunk lists (step s reads q[s & 1])
  *   tiles[GGA_TILES(n) + 1]  compaction counts / offsets */
 #define GGA_MS_RBLOCKS 512u   /* workgroups of the near-far refill (rows of lpart) */
+#define GGA_MS_FAR_TILES(n) (((n) + 63u) / 64u)
 #ifndef GGA_MS_LANES
 #define GGA_MS_LANES 32   /* same value as include/gga.h */
 #endif
@@ -101,6 +102,9 @@ typedef struct gga_dev_ms {
     /* near-far per lane (NULL / 0: plain Bellman-Ford lanes) */
     uint32_t *far;       /* per vertex: lanes whose distance there dropped by a deferred
                             improvement and was not listed or expanded since (n words) */
+    uint32_t *tfar;      /* per 64-vertex tile: non-zero when a vertex of the tile may hold
+                            far bits (set with every far bit, cleared by the refill when it
+                            finds the tile empty; GGA_MS_FAR_TILES(n) words) */
     uint32_t *lpart;     /* refill: per-workgroup per-lane minimum deferred distance */
     uint32_t delta;      /* bucket width */
     uint32_t bmin;       /* fewest chunks per wave batch (more on meshes, whose lanes rarely meet) */

@@ -25,10 +25,12 @@ struct gga_ms {
     uint32_t grid[2][2];      /* [persistent][count] */
     uint32_t *h_pin;          /* GGA_CTL_WORDS */
     uint32_t *d_stage;        /* 32 x n staging for host fetches (lazy) */
-    hipGraphExec_t gx[6];     /* six-step groups captured per phase */
+    hipGraphExec_t gx[12];    /* six-step groups captured per phase; [6 + p]: push-only steps */
     hipStream_t own;          /* the engine's stream (v.stream points to it) */
-    uint32_t gx_grid[6], gx_dense[6];
-    int gx_count[6];
+    uint32_t gx_grid[12], gx_dense[12];
+    int gx_count[12];
+    int mesh;                 /* graph is mesh-like: lists grow a little per step */
+    uint32_t last_tail;       /* list length at the last read-back */
     uint32_t dense_default;
     uint64_t dev_bytes;
     hipEvent_t ev[4];         /* [0],[1] around steps; [2] gga_multi_sp's start; [3] read-back done */
@@ -87,7 +89,7 @@ int gga_ms_free(gga_ms *e)
     if (!e) return GGA_OK;
     hipSetDevice(e->v.device);
     if (e->v.stream) hipStreamSynchronize(e->v.stream);
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < 12; i++)
         if (e->gx[i]) hipGraphExecDestroy(e->gx[i]);
     ms_release(e->d.D);
     ms_release(e->d.M0);
@@ -104,6 +106,7 @@ int gga_ms_free(gga_ms *e)
     ms_release(e->d_sum);
     ms_release(e->d.far);
     ms_release(e->d.lpart);
+    ms_release(e->d.tfar);
     if (e->h_sum) hipHostFree(e->h_sum);
     if (e->own) hipStreamDestroy(e->own);
     free(e);
@@ -132,6 +135,7 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
     e->d.lane_major = layout == GGA_MS_LANE_MAJOR ? 1u : 0u;
     e->dense_default = e->n / 64u > 4096u ? e->n / 64u : 4096u;
     e->d.dense_min = e->dense_default;
+    e->mesh = gga_graph_mesh_like(g);
     /* meshes: lanes rarely share a vertex, batches flatten (edge, lane) pairs,
      * so each wave takes at least 16 low-degree chunks per batch */
     e->d.bmin = gga_graph_mesh_like(g) ? 16u : 1u;
@@ -176,8 +180,13 @@ static void ms_drop_far(gga_ms *e)
         ms_release(e->d.lpart);
         e->dev_bytes -= 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS;
     }
+    if (e->d.tfar) {
+        ms_release(e->d.tfar);
+        e->dev_bytes -= 4ull * GGA_MS_FAR_TILES(e->n);
+    }
     e->d.far = NULL;
     e->d.lpart = NULL;
+    e->d.tfar = NULL;
 }
 
 int gga_ms_set_delta(gga_ms *e, uint32_t delta)
@@ -195,7 +204,9 @@ int gga_ms_set_delta(gga_ms *e, uint32_t delta)
          * neither, so any failure below turns near-far off again */
         if ((rc = ms_alloc(e, (void **)&e->d.far, 4ull * e->n))) goto fail;
         if ((rc = ms_alloc(e, (void **)&e->d.lpart, 4ull * GGA_MS_LANES * GGA_MS_RBLOCKS))) goto fail;
+        if ((rc = ms_alloc(e, (void **)&e->d.tfar, 4ull * GGA_MS_FAR_TILES(e->n)))) goto fail;
         K_TRY(ggak_fill_u32(e->d.far, 0u, e->n, e->v.stream));
+        K_TRY(ggak_fill_u32(e->d.tfar, 0u, GGA_MS_FAR_TILES(e->n), e->v.stream));
         HIP_TRY(hipStreamSynchronize(e->v.stream));
     } else if (!delta && e->d.far) {
         HIP_TRY(hipStreamSynchronize(e->v.stream));
@@ -203,7 +214,7 @@ int gga_ms_set_delta(gga_ms *e, uint32_t delta)
     }
     e->d.delta = delta;
     /* captured steps hold the old engine arguments */
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < 12; i++)
         if (e->gx[i]) {
             hipGraphExecDestroy(e->gx[i]);
             e->gx[i] = NULL;
@@ -296,10 +307,14 @@ static uint32_t ms_grid(gga_ms *e, int persistent, int count)
     return *slot;
 }
 
-/* Six steps starting at phase p as one captured graph (no memset nodes). */
-static int ms_graph(gga_ms *e, uint32_t p, int count, uint32_t grid)
+/* Six steps starting at phase p as one captured graph (no memset nodes);
+ * push-only steps (no list rebuilds) keep graphs of their own. */
+static uint32_t ms_graph_slot(const gga_ms *e, uint32_t p) { return p + (e->d.dense_min == 0xFFFFFFFFu ? 6u : 0u); }
+
+static int ms_graph(gga_ms *e, uint32_t phase, int count, uint32_t grid)
 {
     int rc = GGA_OK;
+    const uint32_t p = ms_graph_slot(e, phase);
     if (e->gx[p] && e->gx_grid[p] == grid && e->gx_count[p] == count && e->gx_dense[p] == e->d.dense_min)
         return GGA_OK;
     if (e->gx[p]) {
@@ -309,7 +324,7 @@ static int ms_graph(gga_ms *e, uint32_t p, int count, uint32_t grid)
     hipGraph_t gr = NULL;
     HIP_TRY(hipStreamBeginCapture(e->v.stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t k = 0; k < 6; k++) {
-        int r = ggak_ms_step(e->v.dg, e->d, (p + k) % 6u, count, grid, e->v.stream);
+        int r = ggak_ms_step(e->v.dg, e->d, (phase + k) % 6u, count, grid, e->v.stream);
         if (r) {
             hipStreamEndCapture(e->v.stream, &gr);
             if (gr) hipGraphDestroy(gr);
@@ -364,6 +379,12 @@ int gga_ms_launch(gga_ms *e, uint32_t nsteps, const gga_opts *o)
     else gga_opts_init(&oo);
     const int count = oo.count ? 1 : 0;
     e->d.dense_min = oo.dense_min ? oo.dense_min : e->dense_default;
+    /* Mesh-like graphs grow their lists by a few vertices per lane and step:
+     * while the last list was far below the compaction switch, this launch's
+     * steps only push, which saves the three list-rebuild launches of every
+     * step (a list that does grow past the switch meanwhile is still pushed
+     * correctly, only with more appends) */
+    if (!oo.dense_min && e->mesh && e->last_tail < e->dense_default / 8u) e->d.dense_min = 0xFFFFFFFFu;
     const uint32_t grid = oo.grid ? oo.grid : ms_grid(e, 0, count);
     if (grid == 0) {
         gga_set_error("occupancy query failed");
@@ -375,7 +396,7 @@ int gga_ms_launch(gga_ms *e, uint32_t nsteps, const gga_opts *o)
     if (oo.use_graph)
         for (; k + 6u <= nsteps; k += 6u) {
             if ((rc = ms_graph(e, (e->step + k) % 6u, count, grid))) return rc;
-            HIP_TRY(hipGraphLaunch(e->gx[(e->step + k) % 6u], e->v.stream));
+            HIP_TRY(hipGraphLaunch(e->gx[ms_graph_slot(e, (e->step + k) % 6u)], e->v.stream));
         }
     for (; k < nsteps; k++) K_TRY(ggak_ms_step(e->v.dg, e->d, (e->step + k) % 6u, count, grid, e->v.stream));
     HIP_TRY(hipEventRecord(e->ev[1], e->v.stream));
@@ -406,6 +427,7 @@ int gga_ms_collect(gga_ms *e, uint32_t *live_out)
     if ((rc = ms_errbits(e))) return rc;
     e->steps_seen = e->h_pin[GGA_CTL_STEPS];
     ms_slot(e, &live, &tail);
+    e->last_tail = e->h_pin[GGA_CTL_TAIL(e->step % 3u)];
     HIP_TRY(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));
     e->ktime_ms += t;
     e->iterations += ran;

@@ -209,8 +209,10 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                     if (imp && a < thr)
                         first = __hip_atomic_fetch_or(&mnxt[v[i]], 1u << sl[i], __ATOMIC_RELAXED,
                                                       __HIP_MEMORY_SCOPE_AGENT) == 0u;
-                    else if (imp && ms.far)
+                    else if (imp && ms.far) {
                         atomicOr(&ms.far[v[i]], 1u << sl[i]);
+                        ms.tfar[v[i] >> 6] = 1u;
+                    }
                     if (COUNT) {
                         cnt.edges += (uint32_t)__popcll(__ballot(on));
                         cnt.relax += (uint32_t)__popcll(__ballot(imp));
@@ -280,7 +282,10 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                     first = __hip_atomic_fetch_or(&mnxt[v[i]], hm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
                 if (ms.far) {
                     const uint32_t fm = (uint32_t)(__ballot(imp && a >= thr_s) >> (32u * half));
-                    if (s == 0 && fm) atomicOr(&ms.far[v[i]], fm);
+                    if (s == 0 && fm) {
+                        atomicOr(&ms.far[v[i]], fm);
+                        ms.tfar[v[i] >> 6] = 1u;
+                    }
                 }
                 if (COUNT) {
                     cnt.edges += (uint32_t)__popcll(__ballot(on));
@@ -493,8 +498,13 @@ __global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms
          base += stride) {
         const uint64_t v = base + lane_id();
         bool first = false;
-        if (v < g.n) {
+        // the wave's 64 vertices are one far tile: skip it unless marked
+        const uint32_t tile = (uint32_t)(base >> 6);
+        const uint32_t marked = ms.tfar[tile];
+        uint32_t left = 0;   // far bits this lane keeps
+        if (v < g.n && marked) {
             const uint32_t fw = ms.far[v];
+            left = fw;
             uint32_t m = fw & cand, bits = 0;
             while (m) {
                 const uint32_t s = (uint32_t)__builtin_ctz(m);
@@ -505,10 +515,14 @@ __global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms
             }
             if (bits) {
                 ms.far[v] = fw & ~bits;
+                left = fw & ~bits;
                 listed |= bits;
                 first = __hip_atomic_fetch_or(&mnxt[v], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
             }
         }
+        // nothing deferred left in the tile (no other kernel writes far bits
+        // meanwhile: the step that sets them ran before this one)
+        if (marked && !__ballot(left != 0u) && lane_id() == 0) ms.tfar[tile] = 0u;
         const uint64_t pm = __ballot(first);
         if (pm) {
             const uint32_t np = (uint32_t)__popcll(pm);
