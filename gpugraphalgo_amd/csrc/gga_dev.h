@@ -156,6 +156,7 @@ typedef struct gga_dev_msbfs {
 #define GGA_CTL_LTHR(s)   (3u * GGA_CTL_LINE + (s))     /* lane s lists distances < LTHR(s) */
 #define GGA_CTL_RUN       (4u * GGA_CTL_LINE)           /* lanes started and not converged */
 #define GGA_CTL_RLISTED   (4u * GGA_CTL_LINE + 1u)      /* lanes a refill listed vertices for */
+#define GGA_CTL_TDONE     (5u * GGA_CTL_LINE)           /* workgroups done with a step's tail pass */
 #define GGA_CTL_ITER(s)   (6u * GGA_CTL_LINE + (s))     /* step number of a ring slot */
 #define GGA_CTL_CHANGED   (7u * GGA_CTL_LINE)           /* topology-mode `changed` flag */
 #define GGA_CTL_STEPS     (7u * GGA_CTL_LINE + 1u)      /* steps that had a non-empty list */

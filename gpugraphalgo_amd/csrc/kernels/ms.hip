@@ -125,7 +125,7 @@ __device__ __forceinline__ bool ms_flag_step(const gga_dev_ms &ms, uint32_t nq) 
 // unconditional full-line loads)
 template <bool COUNT, bool LM>
 __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
-                                         uint32_t nq, uint32_t *pbuf, Counters &cnt) {
+                                         uint32_t nq, uint32_t *pbuf, uint32_t *nl, Counters &cnt) {
     const bool push = !ms_flag_step(ms, nq);
     uint32_t *qout = (step & 1u) ? ms.q0 : ms.q1;
     uint32_t *tail = &ms.ctl[GGA_CTL_TAIL((step + 1u) % 3u)];
@@ -148,6 +148,9 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     // distance but is deferred (no mask bit, a far bit instead) until a refill
     // reaches it
     const uint32_t thr_s = ms.far ? ms.ctl[GGA_CTL_LTHR(s)] : 0xFFFFFFFFu;
+    // lanes the wave sets in next masks, ORed into its LDS word (no register
+    // stays live across the loop for it)
+    if (lane_id() == 0) *nl = 0u;
     for (uint32_t b = wid; b < nbatch; b += waves) {
         const uint32_t idx = b * bsz + l;
         const bool valid = l < bsz && idx < nq;
@@ -206,9 +209,11 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                             &D[LM ? (uint64_t)sl[i] * g.n + v[i] : (uint64_t)v[i] * LANES + sl[i]], a);
                         imp = a < old;
                     }
-                    if (imp && a < thr)
+                    const bool near = imp && a < thr;
+                    if (near)
                         first = __hip_atomic_fetch_or(&mnxt[v[i]], 1u << sl[i], __ATOMIC_RELAXED,
                                                       __HIP_MEMORY_SCOPE_AGENT) == 0u;
+                    if (near) __hip_atomic_fetch_or(nl, 1u << sl[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                     else if (imp && ms.far) {
                         atomicOr(&ms.far[v[i]], 1u << sl[i]);
                         ms.tfar[v[i] >> 6] = 1u;
@@ -278,8 +283,10 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                 const uint64_t bal = __ballot(imp && a < thr_s);
                 const uint32_t hm = (uint32_t)(bal >> (32u * half));
                 bool first = false;
-                if (s == 0 && hm)
+                if (s == 0 && hm) {
                     first = __hip_atomic_fetch_or(&mnxt[v[i]], hm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+                    __hip_atomic_fetch_or(nl, hm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                }
                 if (ms.far) {
                     const uint32_t fm = (uint32_t)(__ballot(imp && a >= thr_s) >> (32u * half));
                     if (s == 0 && fm) {
@@ -314,59 +321,43 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
         }
     }
     if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
+    // the next list's lane word: every lane with a bit in a next mask
+    __builtin_amdgcn_wave_barrier();
+    if (lane_id() == 0) {
+        const uint32_t x = __hip_atomic_load(nl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        if (x)
+            __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_LANES((step + 1u) % 3u)], x, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // ------------------------------------------------------- finish pass
-// Clears the masks step `step` consumed (owners of its list) and ORs the next
-// list's masks into LANES((step + 1) % 3); clears the slot the step after
-// next fills (its length and lanes), which held the list of step - 1.
-__device__ __forceinline__ void ms_finish(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
-                                          uint32_t *s_or) {
-    const uint32_t nin = ms_len(ms, step), nout = ms_len(ms, step + 1u);
-    const uint32_t *qin = ms_qin(ms, step), *qout = ms_qin(ms, step + 1u);
+// Clears the masks step `step` consumed (owners of its list) and the slot the
+// step after next fills (its length and lanes), which held the list of
+// step - 1.  (The relax itself ORs the next list's lanes into its slot.)
+__device__ __forceinline__ void ms_finish(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step) {
+    const uint32_t nin = ms_len(ms, step);
+    const uint32_t *qin = ms_qin(ms, step);
     uint32_t *mcur = ms_mcur(ms, step);
-    const uint32_t *mnxt = ms_mnxt(ms, step);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // the thread holding an owner's first chunk retires the owner: clears its
     // mask and (near-far) the far bits of the lanes it expanded.  No expanded
     // lane can have been deferred at the owner during the step: its distance
-    // was under the threshold, so every drop during the step was too.
-    // One loop over both lists, so that the two chains of dependent loads
-    // (list -> owner -> first chunk -> mask / far word; list -> owner -> next
-    // mask) are in flight together.
-    uint32_t x = 0;
-    const uint32_t nmax = nin > nout ? nin : nout;
-    for (uint64_t i = tid; i < nmax; i += stride) {
-        const bool in_ok = i < nin, out_ok = i < nout;
-        const uint32_t c = in_ok ? qin[i] : 0u, co = out_ok ? qout[i] : 0u;
-        const uint32_t u = in_ok ? g.cown[c] : 0u, uo = out_ok ? g.cown[co] : 0u;
-        const bool first = in_ok && c == g.cstart[u];
-        if (out_ok) x |= mnxt[uo];
-        if (first) {
-            const uint32_t m = mcur[u];
-            if (ms.far) {
-                const uint32_t fw = ms.far[u];
-                if (fw & m) ms.far[u] = fw & ~m;
-            }
-            mcur[u] = 0u;
-        }
+    // was under the threshold, so every drop during the step was too.  (The
+    // far word is cleared with an atomic: a refill may update it meanwhile.)
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nin; i += stride) {
+        const uint32_t c = qin[i];
+        const uint32_t u = g.cown[c];
+        if (c != g.cstart[u]) continue;
+        const uint32_t m = mcur[u];
+        if (ms.far && (ms.far[u] & m)) atomicAnd(&ms.far[u], ~m);
+        mcur[u] = 0u;
     }
-    x = wave_or(x);
-    if (lane_id() == 0) s_or[threadIdx.x / GGA_WAVE] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t y = 0;
-        for (uint32_t i = 0; i < blockDim.x / GGA_WAVE; i++) y |= s_or[i];
-        if (y) __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_LANES((step + 1u) % 3u)], y, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-        if (blockIdx.x == 0) {
-            ms.ctl[GGA_CTL_TAIL((step + 2u) % 3u)] = 0u;
-            ms.ctl[GGA_CTL_LANES((step + 2u) % 3u)] = 0u;
-            if (nin) ms.ctl[GGA_CTL_STEPS] += 1u;
-        }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ms.ctl[GGA_CTL_TAIL((step + 2u) % 3u)] = 0u;
+        ms.ctl[GGA_CTL_LANES((step + 2u) % 3u)] = 0u;
+        if (nin) ms.ctl[GGA_CTL_STEPS] += 1u;
     }
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------- kernels
@@ -427,11 +418,12 @@ template <bool COUNT, bool LM>
 // from 7.4 to 5.9 ms (grids unchanged).
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_pbuf[WAVES][PBUF];
+    __shared__ uint32_t s_pbuf[WAVES][PBUF], s_nl[WAVES];
     const uint32_t nq = ms_len(ms, step);
     if (nq == 0) return;
     Counters cnt;
-    ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);   // wave-uniform
+    ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[wv], &s_nl[wv], cnt);
     publish_counters<COUNT>(ms, cnt);
 }
 
@@ -461,29 +453,34 @@ __global__ __launch_bounds__(CBLOCK) void k_ms_write(gga_dev_graph g, gga_dev_ms
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_ms_finish(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_or[WAVES];
-    ms_finish(g, ms, step, s_or);
-}
-
-// ---- near-far refill, after the finish pass of step `step`: lanes still
-// running but without listed work (candidates) list their deferred
-// (vertex, lane) pairs -- the far bits -- whose distance is under threshold +
-// delta, and clear those bits; a vertex whose next mask was empty is appended
-// to the next list (wave-staged pushes, as in a push step).  Per-lane minima
-// of the deferred distances beyond it go to lpart (one row of 32 per
-// workgroup) for the reduction.  The scan reads one far word per vertex and D
-// only where a candidate's bit is set.
+// ---- the tail of a step: the finish pass, then (near-far) the refill of
+// lanes still running but without listed work (candidates): they list their
+// deferred (vertex, lane) pairs -- the far bits -- whose distance is under
+// threshold + delta, and clear those bits; a vertex whose next mask was empty
+// is appended to the next list (wave-staged pushes, as in a push step).
+// Per-lane minima of the deferred distances beyond it go to lpart (one row of
+// 32 per workgroup); the last workgroup to finish reduces them (a counter,
+// no waiting): a lane whose refill listed pairs raises its threshold by
+// delta; with nothing under the new threshold but deferred pairs left, the
+// threshold jumps to the smallest deferred distance (listed by the next
+// step's refill); with nothing deferred the lane has converged.  The scan
+// reads one far-tile word per 64 vertices, the far word of each vertex in a
+// marked tile, and D only where a candidate's bit is set.
 constexpr uint32_t RBLOCKS = GGA_MS_RBLOCKS;
-__global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_t1[LANES], s_min[LANES], s_listed;
+__global__ __launch_bounds__(BLOCK) void k_ms_tail(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
+    __shared__ uint32_t s_t1[LANES], s_min[LANES], s_listed, s_last, s_done;
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
+    __shared__ uint32_t s_red[BLOCK / LANES][LANES];
+    ms_finish(g, ms, step);
     if (!ms.far) return;
-    const uint32_t cand = ms.ctl[GGA_CTL_RUN] & ~ms.ctl[GGA_CTL_LANES((step + 1u) % 3u)];
+    const uint32_t sout = (step + 1u) % 3u;
+    // RUN and LTHR change only in the last workgroup's reduction, after every
+    // workgroup has read them
+    const uint32_t cand = ms.ctl[GGA_CTL_RUN] & ~ms.ctl[GGA_CTL_LANES(sout)];
     if (!cand) return;
     uint32_t *mnxt = ms_mnxt(ms, step);
     uint32_t *qout = (step & 1u) ? ms.q0 : ms.q1;
-    uint32_t *tail = &ms.ctl[GGA_CTL_TAIL((step + 1u) % 3u)];
+    uint32_t *tail = &ms.ctl[GGA_CTL_TAIL(sout)];
     uint32_t *pbuf = s_pbuf[threadIdx.x / GGA_WAVE];
     if (threadIdx.x < LANES) {
         s_t1[threadIdx.x] = sat_add(ms.ctl[GGA_CTL_LTHR(threadIdx.x)], ms.delta);
@@ -492,49 +489,54 @@ __global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms
     if (threadIdx.x == 0) s_listed = 0;
     __syncthreads();
     uint32_t listed = 0, pcount = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    // wave-uniform loop (the pushes below use whole-wave operations)
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(GGA_WAVE - 1u)); base < g.n;
-         base += stride) {
-        const uint64_t v = base + lane_id();
-        bool first = false;
-        // the wave's 64 vertices are one far tile: skip it unless marked
-        const uint32_t tile = (uint32_t)(base >> 6);
-        const uint32_t marked = ms.tfar[tile];
-        uint32_t left = 0;   // far bits this lane keeps
-        if (v < g.n && marked) {
-            const uint32_t fw = ms.far[v];
-            left = fw;
-            uint32_t m = fw & cand, bits = 0;
-            while (m) {
-                const uint32_t s = (uint32_t)__builtin_ctz(m);
-                m &= m - 1u;
-                const uint32_t d = ms.D[didx(ms, g.n, (uint32_t)v, s)];
-                if (d < s_t1[s]) bits |= 1u << s;
-                else atomicMin(&s_min[s], d);
+    // Each wave reads 64 consecutive far-tile words in one load, then walks
+    // only the marked tiles, one vertex per lane (wave-uniform: the pushes
+    // below use whole-wave operations).
+    const uint64_t ntiles = GGA_MS_FAR_TILES(g.n);
+    const uint64_t wstride = (uint64_t)gridDim.x * WAVES * GGA_WAVE;
+    for (uint64_t tb = ((uint64_t)blockIdx.x * WAVES + threadIdx.x / GGA_WAVE) * GGA_WAVE; tb < ntiles;
+         tb += wstride) {
+        const uint64_t tl = tb + lane_id();
+        uint64_t tm = __ballot(tl < ntiles && ms.tfar[tl] != 0u);
+        while (tm) {
+            const uint64_t tile = tb + (uint64_t)__builtin_ctzll(tm);
+            tm &= tm - 1u;
+            const uint64_t v = tile * GGA_WAVE + lane_id();
+            bool first = false;
+            uint32_t left = 0;   // far bits this lane keeps
+            if (v < g.n) {
+                const uint32_t fw = ms.far[v];
+                left = fw;
+                uint32_t m = fw & cand, bits = 0;
+                while (m) {
+                    const uint32_t s = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1u;
+                    const uint32_t d = ms.D[didx(ms, g.n, (uint32_t)v, s)];
+                    if (d < s_t1[s]) bits |= 1u << s;
+                    else atomicMin(&s_min[s], d);
+                }
+                if (bits) {
+                    left = atomicAnd(&ms.far[v], ~bits) & ~bits;
+                    listed |= bits;
+                    first = __hip_atomic_fetch_or(&mnxt[v], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+                }
             }
-            if (bits) {
-                ms.far[v] = fw & ~bits;
-                left = fw & ~bits;
-                listed |= bits;
-                first = __hip_atomic_fetch_or(&mnxt[v], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+            // nothing deferred left in the tile (far bits are only cleared
+            // while this kernel runs; the step that sets them ran before)
+            if (!__ballot(left != 0u) && lane_id() == 0) ms.tfar[tile] = 0u;
+            const uint64_t pm = __ballot(first);
+            if (pm) {
+                const uint32_t np = (uint32_t)__popcll(pm);
+                if (pcount + np > PBUF) {
+                    flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
+                    pcount = 0;
+                }
+                if (first)
+                    pbuf[pcount + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] = (uint32_t)v;
+                pcount += np;
+                __builtin_amdgcn_wave_barrier();
             }
-        }
-        // nothing deferred left in the tile (no other kernel writes far bits
-        // meanwhile: the step that sets them ran before this one)
-        if (marked && !__ballot(left != 0u) && lane_id() == 0) ms.tfar[tile] = 0u;
-        const uint64_t pm = __ballot(first);
-        if (pm) {
-            const uint32_t np = (uint32_t)__popcll(pm);
-            if (pcount + np > PBUF) {
-                flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
-                pcount = 0;
-            }
-            if (first)
-                pbuf[pcount + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] =
-                    (uint32_t)v;
-            pcount += np;
-            __builtin_amdgcn_wave_barrier();
         }
     }
     if (pcount) flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
@@ -542,41 +544,44 @@ __global__ __launch_bounds__(BLOCK) void k_ms_refill(gga_dev_graph g, gga_dev_ms
     if (lane_id() == 0 && listed) atomicOr(&s_listed, listed);
     __syncthreads();
     if (threadIdx.x < LANES) ms.lpart[(uint64_t)blockIdx.x * LANES + threadIdx.x] = s_min[threadIdx.x];
-    if (threadIdx.x == 0 && s_listed)
-        __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_RLISTED], s_listed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// One workgroup: per candidate lane, a listing refill raises its threshold by
-// delta; with nothing under the new threshold but deferred pairs left, the
-// threshold jumps to the smallest deferred distance (listed by the next
-// step's refill); with nothing deferred the lane has converged.
-__global__ __launch_bounds__(BLOCK) void k_ms_refill_reduce(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_done;
-    if (!ms.far) return;
-    const uint32_t sout = (step + 1u) % 3u;
-    const uint32_t cand = ms.ctl[GGA_CTL_RUN] & ~ms.ctl[GGA_CTL_LANES(sout)];
-    if (!cand) return;
-    if (threadIdx.x == 0) s_done = 0;
+    __threadfence();   // this workgroup's lpart row before its arrival
     __syncthreads();
-    const uint32_t listed = ms.ctl[GGA_CTL_RLISTED];
-    const uint32_t s = threadIdx.x;
-    if (s < LANES && ((cand >> s) & 1u)) {
+    if (threadIdx.x == 0) {
+        if (s_listed)
+            __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_RLISTED], s_listed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = __hip_atomic_fetch_add(&ms.ctl[GGA_CTL_TDONE], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+                 gridDim.x - 1u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();   // every row, as its workgroup left it
+    {
+        const uint32_t s = threadIdx.x & (LANES - 1u), grp = threadIdx.x / LANES;
         uint32_t gmin = 0xFFFFFFFFu;
-        for (uint32_t b = 0; b < RBLOCKS; b++) {
+        for (uint32_t b = grp; b < gridDim.x; b += BLOCK / LANES) {
             const uint32_t x = ms.lpart[(uint64_t)b * LANES + s];
             gmin = x < gmin ? x : gmin;
         }
-        if ((listed >> s) & 1u) ms.ctl[GGA_CTL_LTHR(s)] = sat_add(ms.ctl[GGA_CTL_LTHR(s)], ms.delta);
+        s_red[grp][s] = gmin;
+    }
+    if (threadIdx.x == 0) s_done = 0;
+    __syncthreads();
+    const uint32_t rl = __hip_atomic_load(&ms.ctl[GGA_CTL_RLISTED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < LANES && ((cand >> threadIdx.x) & 1u)) {
+        const uint32_t s = threadIdx.x;
+        uint32_t gmin = 0xFFFFFFFFu;
+        for (uint32_t k = 0; k < BLOCK / LANES; k++) gmin = s_red[k][s] < gmin ? s_red[k][s] : gmin;
+        if ((rl >> s) & 1u) ms.ctl[GGA_CTL_LTHR(s)] = sat_add(ms.ctl[GGA_CTL_LTHR(s)], ms.delta);
         else if (gmin != 0xFFFFFFFFu) ms.ctl[GGA_CTL_LTHR(s)] = gmin;
         else atomicOr(&s_done, 1u << s);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         ms.ctl[GGA_CTL_RUN] &= ~s_done;
-        ms.ctl[GGA_CTL_LANES(sout)] |= listed;
+        ms.ctl[GGA_CTL_LANES(sout)] |= rl;
         ms.ctl[GGA_CTL_RLISTED] = 0u;
+        ms.ctl[GGA_CTL_TDONE] = 0u;
     }
-    (void)g;
 }
 
 // All steps in one cooperative launch: relax, then the three compaction
@@ -585,7 +590,7 @@ template <bool COUNT, bool LM>
 __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0,
                                                          uint32_t max_steps, uint64_t spin_ticks) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
-    __shared__ uint32_t s_wave[WAVES], s_or[WAVES], s_state[2];
+    __shared__ uint32_t s_wave[WAVES], s_state[2], s_nl[WAVES];
     uint32_t gen = 0;
     if (threadIdx.x == 0)
         gen = __hip_atomic_load(&ms.ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -601,7 +606,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_de
         const uint32_t nq = s_state[0] < ms.qcap ? s_state[0] : ms.qcap, err = s_state[1];
         __syncthreads();
         if (nq == 0 || err) break;
-        ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+        ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], &s_nl[threadIdx.x / GGA_WAVE], cnt);
         grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
         if (ms_flag_step(ms, nq)) {
             const ListOut o = ms_list_out(g, ms, step);
@@ -613,7 +618,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_de
             for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
             grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
         }
-        ms_finish(g, ms, step, s_or);
+        ms_finish(g, ms, step);
         grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
     }
     publish_counters<COUNT>(ms, cnt);
@@ -856,11 +861,8 @@ extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int c
         k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
         k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     }
-    k_ms_finish<<<grid_for(g.C, 1024), BLOCK, 0, s>>>(g, ms, step);
-    if (ms.far) {
-        k_ms_refill<<<RBLOCKS, BLOCK, 0, s>>>(g, ms, step);
-        k_ms_refill_reduce<<<1, BLOCK, 0, s>>>(g, ms, step);
-    }
+    // lpart holds one row per workgroup of the tail pass
+    k_ms_tail<<<RBLOCKS, BLOCK, 0, s>>>(g, ms, step);
     return launch_status();
 }
 

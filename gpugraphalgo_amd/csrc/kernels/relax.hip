@@ -302,7 +302,7 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     // batch per 1/1024 of the list, but at least bmin chunks
     const uint32_t wpb = blockDim.x / GGA_WAVE;   // waves per workgroup (128, 256 or 512 threads)
     const uint32_t waves = gridDim.x * wpb;
-    const uint32_t wid = blockIdx.x * wpb + threadIdx.x / GGA_WAVE;
+    const uint32_t wid = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);
     uint32_t bsz = w.bmin > 1u ? nq >> 10 : nq / (16u * waves);
     bsz = bsz < w.bmin ? w.bmin : bsz;
     bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
@@ -411,7 +411,8 @@ void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl
     if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
     if (nq != 0) {
         Counters cnt;
-        worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+        worklist_step<BFS, COUNT>(g, w, step, nq, it,
+                                  s_pbuf[__builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE)], cnt);
         publish_counters<COUNT>(w, cnt);
     }
     if (TL) {
@@ -590,7 +591,8 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
         if (nq == 0 || err) break;
         const uint32_t it = w.ctl[GGA_CTL_ITER(sin)];
         if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
-        worklist_step<BFS, COUNT>(g, w, step, nq, it, s_pbuf[threadIdx.x / GGA_WAVE], cnt);
+        worklist_step<BFS, COUNT>(g, w, step, nq, it,
+                                  s_pbuf[__builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE)], cnt);
         if (flag_step(w, nq)) {
             const ListOut o = list_out(g, w, step);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
