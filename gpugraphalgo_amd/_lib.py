@@ -72,7 +72,7 @@ class Stats(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", u64), ("steps_per_round", u32), ("lanes", u32), ("use_graph", u32),
                 ("dense_min", u32), ("host_results", u32), ("dry_run", u32), ("dry_steps", u32),
-                ("layout", u32), ("engines_per_graph", u32)]
+                ("layout", u32), ("engines_per_graph", u32), ("mesh_reserve", ctypes.c_int32)]
 
 
 class Query(ctypes.Structure):

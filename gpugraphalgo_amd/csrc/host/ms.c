@@ -300,6 +300,9 @@ fail:
     return rc;
 }
 
+static uint32_t ms_grid(gga_ms *e, int persistent, int count);
+uint32_t gga_ms_step_grid(gga_ms *e) { return e ? ms_grid(e, 0, 0) : 0u; }
+
 static uint32_t ms_grid(gga_ms *e, int persistent, int count)
 {
     uint32_t *slot = &e->grid[persistent][count];

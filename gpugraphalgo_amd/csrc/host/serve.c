@@ -47,6 +47,7 @@ typedef struct srv_engine {
 typedef struct srv_graph {
     gga_graph *g;             /* NULL on a dry-run server */
     uint32_t n, C;
+    int mesh;                 /* mesh-like: steps are latency-bound, its engines keep the full grid */
     uint64_t engine_bytes;
     srv_engine eng[GGA_SERVER_MAX_ENGINES];
 } srv_graph;
@@ -111,6 +112,11 @@ int gga_server_create(const gga_server_opts *o, gga_server **out)
     if (!s->o.dry_steps) s->o.dry_steps = 3;
     if (!s->o.engines_per_graph) s->o.engines_per_graph = 1;
     if (s->o.engines_per_graph > GGA_SERVER_MAX_ENGINES) s->o.engines_per_graph = GGA_SERVER_MAX_ENGINES;
+    if (s->o.mesh_reserve == 0) s->o.mesh_reserve = 2;
+    if (s->o.mesh_reserve < -1 || s->o.mesh_reserve > 6) {
+        free(s);
+        return GGA_EINVAL;
+    }
     clock_gettime(CLOCK_MONOTONIC, &s->t0);
     pthread_mutex_init(&s->mu, NULL);
     *out = s;
@@ -143,6 +149,7 @@ static int add_graph(gga_server *s, gga_graph *g, uint32_t n, uint32_t C, uint64
     x->g = g;
     x->n = n;
     x->C = C;
+    x->mesh = g ? gga_graph_mesh_like(g) : 0;
     x->engine_bytes = bytes ? bytes : gga_ms_bytes(n, C);
     if (s->o.host_results) x->engine_bytes += 4ull * GGA_MS_LANES * n;   /* fetch staging */
     if (id) *id = s->ngr;
@@ -399,6 +406,10 @@ int gga_server_pump(gga_server *s, uint32_t *active)
             G->start_unit = 0;
         }
         if (G->busy) {
+            const srv_graph *gr = &s->gr[i / GGA_SERVER_MAX_ENGINES];
+            o.grid = 0;
+            if (!s->o.dry_run && !gr->mesh && s->o.mesh_reserve > 0)
+                o.grid = gga_ms_step_grid(G->e) / 8u * (8u - (uint32_t)s->o.mesh_reserve);
             if (!s->o.dry_run && (rc = gga_ms_launch(G->e, s->o.steps_per_round, &o))) return rc;
             G->launched = 1;
             G->last_used = now;

@@ -50,6 +50,14 @@ constexpr uint32_t FLAT_UNROLL = GGA_MS_FLAT_UNROLL;
 #endif
 // a batch averaging at most GGA_MS_FLAT lanes per edge slot is flattened
 constexpr uint32_t PBUF = 256;                 // pushed vertices staged per wave before a flush
+// Mesh engines: (vertex, lane) pairs a wave may follow past its batch within
+// one step (0: off), the wave's LDS queue of them, and the largest degree
+// followed (a hub is left to the next step's chunks).
+#ifndef GGA_MS_CONT
+#define GGA_MS_CONT 256
+#endif
+constexpr uint32_t CQ = 128;
+constexpr uint32_t CONT_DEG = 16;
 
 struct Counters {
     uint64_t edges = 0, chunks = 0, relax = 0;
@@ -125,7 +133,8 @@ __device__ __forceinline__ bool ms_flag_step(const gga_dev_ms &ms, uint32_t nq) 
 // unconditional full-line loads)
 template <bool COUNT, bool LM>
 __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step,
-                                         uint32_t nq, uint32_t *pbuf, uint32_t *nl, Counters &cnt) {
+                                         uint32_t nq, uint32_t *pbuf, uint32_t *nl, uint32_t *cq,
+                                         Counters &cnt) {
     const bool push = !ms_flag_step(ms, nq);
     uint32_t *qout = (step & 1u) ? ms.q0 : ms.q1;
     uint32_t *tail = &ms.ctl[GGA_CTL_TAIL((step + 1u) % 3u)];
@@ -148,6 +157,8 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     // distance but is deferred (no mask bit, a far bit instead) until a refill
     // reaches it
     const uint32_t thr_s = ms.far ? ms.ctl[GGA_CTL_LTHR(s)] : 0xFFFFFFFFu;
+    const bool cont = GGA_MS_CONT > 0 && ms.bmin > 1u;
+    uint32_t *const cqv = cq, *const cqs = cq + CQ, *const cqa = cq + 2u * CQ;   // vertex, lane, distance
     // lanes the wave sets in next masks, ORed into its LDS word (no register
     // stays live across the loop for it)
     if (lane_id() == 0) *nl = 0u;
@@ -168,11 +179,23 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
         if (COUNT) cnt.chunks += (uint32_t)__popcll(__ballot(ne > 0));
         // (edge, lane) pairs of the batch; with few lanes per edge slot on
         // average, one pair per wave lane beats one edge per half wave
-        const uint32_t pc = (uint32_t)__popc(m);
-        const uint32_t pairs = pc * ne;
-        const uint32_t pincl = wave_incl_scan(pairs);
-        const uint32_t ptotal = wave_bcast(pincl, GGA_WAVE - 1);
+        uint32_t pc = (uint32_t)__popc(m);
+        uint32_t pairs = pc * ne;
+        uint32_t pincl = wave_incl_scan(pairs);
+        uint32_t ptotal = wave_bcast(pincl, GGA_WAVE - 1);
         if (ptotal <= GGA_MS_FLAT * total) {
+            // Round 0 walks the batch's owners.  On meshes (cont) each later
+            // round walks up to 64 (vertex, lane) pairs that this wave
+            // improved below the lane's threshold in an earlier round, from
+            // the distance it wrote there: the wave follows its own wavefront
+            // a few hops further within the step.  Each such vertex is listed
+            // for the next step all the same, so the extra rounds only bring
+            // improvements forward; the fixed point, and so the result, is
+            // unchanged.
+            bool later = false;          // a continuation round (wave-uniform)
+            uint32_t from = 0;           // its pair's distance at the owner
+            uint32_t qh = 0, qt = 0, budget = cont ? (uint32_t)GGA_MS_CONT : 0u;
+            for (;;) {
             const uint32_t pexcl = pincl - pairs;
             for (uint32_t base = 0; base < ptotal; base += GGA_WAVE * FLAT_UNROLL) {
                 uint32_t v[FLAT_UNROLL], wt[FLAT_UNROLL], du[FLAT_UNROLL], dv[FLAT_UNROLL], sl[FLAT_UNROLL];
@@ -193,7 +216,7 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                     const uint32_t uu = wave_bcast(u, j);
                     v[i] = stream_load(&g.col[e]);
                     wt[i] = stream_load(&g.w[e]);
-                    du[i] = D[LM ? (uint64_t)sl[i] * g.n + uu : (uint64_t)uu * LANES + sl[i]];
+                    du[i] = later ? wave_bcast(from, j) : D[LM ? (uint64_t)sl[i] * g.n + uu : (uint64_t)uu * LANES + sl[i]];
                 }
 #pragma unroll
                 for (uint32_t i = 0; i < FLAT_UNROLL; i++)
@@ -218,6 +241,22 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                         atomicOr(&ms.far[v[i]], 1u << sl[i]);
                         ms.tfar[v[i] >> 6] = 1u;
                     }
+                    if (budget) {
+                        // queue the pair for a later round (dropped when the queue is full)
+                        const uint64_t qm = __ballot(near);
+                        if (qm) {
+                            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(qm >> 32),
+                                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)qm, 0u));
+                            const uint32_t room = CQ - (qt - qh), nnear = (uint32_t)__popcll(qm);
+                            if (near && rank < room) {
+                                const uint32_t slot = (qt + rank) % CQ;
+                                cqv[slot] = v[i];
+                                cqs[slot] = sl[i];
+                                cqa[slot] = a;
+                            }
+                            qt += nnear < room ? nnear : room;
+                        }
+                    }
                     if (COUNT) {
                         cnt.edges += (uint32_t)__popcll(__ballot(on));
                         cnt.relax += (uint32_t)__popcll(__ballot(imp));
@@ -241,6 +280,32 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                         }
                     }
                 }
+            }
+            if (qh == qt || budget == 0) break;
+            // the next round: up to 64 queued pairs, one per wave lane, each
+            // owning all edges of its vertex
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t avail = qt - qh;
+            uint32_t take = avail < GGA_WAVE ? avail : GGA_WAVE;
+            take = take < budget ? take : budget;
+            later = true;
+            u = 0; m = 0; eb = 0; from = 0;
+            uint32_t ne1 = 0;
+            if (l < take) {
+                const uint32_t slot = (qh + l) % CQ;
+                u = cqv[slot];
+                m = 1u << cqs[slot];
+                from = cqa[slot];
+                eb = g.cbeg[g.cstart[u]];
+                ne1 = g.cbeg[g.cstart[u + 1]] - eb;
+                if (ne1 > CONT_DEG) ne1 = 0;
+            }
+            qh += take;
+            budget -= take;
+            pc = l < take ? 1u : 0u;
+            pairs = pc * ne1;
+            pincl = wave_incl_scan(pairs);
+            ptotal = wave_bcast(pincl, GGA_WAVE - 1);
             }
             continue;
         }
@@ -418,12 +483,12 @@ template <bool COUNT, bool LM>
 // from 7.4 to 5.9 ms (grids unchanged).
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_pbuf[WAVES][PBUF], s_nl[WAVES];
+    __shared__ uint32_t s_pbuf[WAVES][PBUF], s_nl[WAVES], s_cq[WAVES][3 * CQ];
     const uint32_t nq = ms_len(ms, step);
     if (nq == 0) return;
     Counters cnt;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);   // wave-uniform
-    ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[wv], &s_nl[wv], cnt);
+    ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[wv], &s_nl[wv], s_cq[wv], cnt);
     publish_counters<COUNT>(ms, cnt);
 }
 
@@ -590,7 +655,7 @@ template <bool COUNT, bool LM>
 __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0,
                                                          uint32_t max_steps, uint64_t spin_ticks) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF];
-    __shared__ uint32_t s_wave[WAVES], s_state[2], s_nl[WAVES];
+    __shared__ uint32_t s_wave[WAVES], s_state[2], s_nl[WAVES], s_cq[WAVES][3 * CQ];
     uint32_t gen = 0;
     if (threadIdx.x == 0)
         gen = __hip_atomic_load(&ms.ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -606,7 +671,8 @@ __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_de
         const uint32_t nq = s_state[0] < ms.qcap ? s_state[0] : ms.qcap, err = s_state[1];
         __syncthreads();
         if (nq == 0 || err) break;
-        ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[threadIdx.x / GGA_WAVE], &s_nl[threadIdx.x / GGA_WAVE], cnt);
+        const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);
+        ms_relax<COUNT, LM>(g, ms, step, nq, s_pbuf[wv], &s_nl[wv], s_cq[wv], cnt);
         grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
         if (ms_flag_step(ms, nq)) {
             const ListOut o = ms_list_out(g, ms, step);

@@ -77,10 +77,10 @@ class Server:
     def __init__(self, hbm_budget: int = 0, steps_per_round: int = 0, lanes: int = 0,
                  use_graph: bool = False, dense_min: int = 0, host_results: bool = False,
                  dry_run: bool = False, dry_steps: int = 0, layout: str = "auto",
-                 engines_per_graph: int = 0):
+                 engines_per_graph: int = 0, mesh_reserve: int = 0):
         o = _lib.ServerOpts(hbm_budget, steps_per_round, lanes, 1 if use_graph else 0, dense_min,
                             1 if host_results else 0, 1 if dry_run else 0, dry_steps,
-                            {"auto": 0, "vertex": 1, "lane": 2}[layout], engines_per_graph)
+                            {"auto": 0, "vertex": 1, "lane": 2}[layout], engines_per_graph, mesh_reserve)
         self._h = ctypes.c_void_p()
         check(load().gga_server_create(ctypes.byref(o), ctypes.byref(self._h)), "gga_server_create")
         self.graphs: list = []

@@ -472,6 +472,11 @@ typedef struct gga_server_opts {
     uint32_t engines_per_graph; /* engines one graph may hold at once, 1..GGA_SERVER_MAX_ENGINES
                                    (0 -> 1): a further engine takes a graph's queries while the
                                    others' lanes are all busy */
+    int32_t  mesh_reserve;    /* eighths of each CU's workgroup slots that engines on graphs
+                                 that are not mesh-like leave free (their relax grid is cut by
+                                 as much), so that the short launches of engines on mesh-like
+                                 graphs -- thousands of steps per query -- do not queue behind
+                                 theirs: 0 -> 2, 1..6 as given, -1 -> none */
 } gga_server_opts;
 #define GGA_SERVER_MAX_ENGINES 4
 typedef struct gga_query {

@@ -233,3 +233,13 @@ def test_joined_counts_queries_admitted_beside_running_lanes():
         assert st["joined"] == 2 and st["peak_running"] == 4
         s.drain()
         assert s.stats()["completed"] == 4
+
+
+def test_mesh_reserve_range():
+    """The workgroup-slot reserve for mesh engines accepts -1 (none) and 0..6
+    eighths; anything else is an argument error."""
+    for r in (-1, 0, 1, 6):
+        Server(dry_run=True, mesh_reserve=r).close()
+    for r in (-2, 7, 8):
+        with pytest.raises(_lib.GgaError, match="GGA_EINVAL"):
+            Server(dry_run=True, mesh_reserve=r)
