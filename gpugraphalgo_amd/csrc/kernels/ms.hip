@@ -56,7 +56,10 @@ constexpr uint32_t PBUF = 256;                 // pushed vertices staged per wav
 #ifndef GGA_MS_CONT
 #define GGA_MS_CONT 256
 #endif
-constexpr uint32_t CQ = 128;
+#ifndef GGA_MS_CQ
+#define GGA_MS_CQ 128
+#endif
+constexpr uint32_t CQ = GGA_MS_CQ;
 constexpr uint32_t CONT_DEG = 16;
 
 struct Counters {

@@ -65,6 +65,19 @@ constexpr uint32_t PBUF = 256;          // queued vertices staged per wave befor
 #define GGA_UNROLL 4
 #endif
 constexpr uint32_t UNROLL = GGA_UNROLL; // edge slots per lane per round in expand_batch
+// Meshes: vertices a wave may follow past its batch within one step (0: off),
+// its LDS queue of them, and the largest degree followed.
+#ifndef GGA_CONT
+#define GGA_CONT 1024
+#endif
+constexpr uint32_t CQ = 128;
+constexpr uint32_t CONT_DEG = 16;
+// Only steps whose list is at most this many chunks follow: a thin wavefront
+// leaves most of the chip idle and each step is a chain of dependent loads;
+// a wide one (a mesh with shortcuts, past its first levels) already fills the
+// chip, and following would only expand the same vertices twice (4096^2 grid
+// + 0.5% shortcuts, BFS: 5.1 ms without, 8.4 ms following in every step).
+constexpr uint32_t CONT_LIST = 65536;
 
 enum { OUT_CHANGED = 0, OUT_QUEUE = 1, OUT_FLAG = 2 };
 
@@ -98,11 +111,21 @@ __device__ __forceinline__ ChunkDesc load_chunk(const gga_dev_graph &g, uint32_t
 // (when set) records the distance each owner was expanded at, so that a
 // deferred vertex is recognised by dist < last.  thr = 0xFFFFFFFF: plain
 // worklist (every improvement is listed).
-template <bool BFS, bool COUNT, int OUT>
+//
+// CONT (meshes), cq given (a short list), list steps: after the batch, the wave follows
+// its own wavefront -- each later round expands up to 64 vertices that this
+// wave lowered below the threshold in an earlier round, from the distance it
+// wrote there, all their edges (degree <= CONT_DEG), until its queue empties
+// or GGA_CONT vertices were followed.  Every such vertex is listed for the
+// next step all the same (stamp or flag), so the extra rounds only bring
+// improvements forward: distances converge to the same fixed point.  (BFS
+// then takes the stamp too: levels reached out of order can drop twice.)
+template <bool BFS, bool COUNT, int OUT, bool CONT = false>
 __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_dev_work &w,
                                              const ChunkDesc &d, uint32_t it, uint32_t *pbuf,
                                              uint32_t &pcount, uint32_t *qout, uint32_t *tail,
-                                             bool &improved, Counters &cnt, uint32_t thr = 0xFFFFFFFFu) {
+                                             bool &improved, Counters &cnt, uint32_t thr = 0xFFFFFFFFu,
+                                             uint32_t *cq = nullptr) {
     const uint32_t l = lane_id();
     uint32_t *__restrict__ dist = w.dist;
     const uint32_t inf = w.inf;
@@ -134,6 +157,10 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
         n = go ? ee - eb : 0u;
         if (w.last && go) w.last[own] = du;
     }
+    constexpr bool cont = CONT && OUT != OUT_CHANGED;
+    uint32_t *const cqv = cq, *const cqa = cq + CQ;   // vertex, the distance this wave wrote
+    uint32_t qh = 0, qt = 0, budget = (cont && cq) ? (uint32_t)GGA_CONT : 0u;   // wave-uniform
+    for (;;) {
     const uint32_t incl = wave_incl_scan(n);
     const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
     if (COUNT) {
@@ -178,9 +205,24 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
                 }
             }
             if (COUNT) cnt.relax += __builtin_amdgcn_readfirstlane((uint32_t)__popcll(__ballot(imp)));
+            if (budget) {
+                // queue the vertex for a later round (dropped when the queue is full)
+                const bool near = imp && alt < thr;
+                const uint64_t qm = __ballot(near);
+                if (qm) {
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(qm >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)qm, 0u));
+                    const uint32_t room = CQ - (qt - qh), nnear = (uint32_t)__popcll(qm);
+                    if (near && rank < room) {
+                        cqv[(qt + rank) % CQ] = v[u];
+                        cqa[(qt + rank) % CQ] = alt;
+                    }
+                    qt += nnear < room ? nnear : room;
+                }
+            }
             if (OUT == OUT_QUEUE) {
                 bool want = imp && alt < thr;
-                if (!BFS && want)
+                if ((!BFS || cont) && want)
                     want = (w.mark[v[u]] != it) && (atomic_xchg_u32(&w.mark[v[u]], it) != it);
                 const uint64_t mask = __ballot(want);
                 if (mask) {
@@ -202,6 +244,26 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
                 improved |= imp;
             }
         }
+    }
+    if (qh == qt || budget == 0) break;
+    // the next round: up to 64 queued vertices, one per wave lane
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t avail = qt - qh;
+    uint32_t take = avail < GGA_WAVE ? avail : GGA_WAVE;
+    take = take < budget ? take : budget;
+    eb = 0;
+    n = 0;
+    du = inf;
+    if (l < take) {
+        const uint32_t slot = (qh + l) % CQ;
+        const uint32_t x = cqv[slot];
+        du = cqa[slot];
+        eb = g.cbeg[g.cstart[x]];
+        n = g.cbeg[g.cstart[x + 1]] - eb;
+        if (n > CONT_DEG) n = 0;
+    }
+    qh += take;
+    budget -= take;
     }
 }
 
@@ -284,10 +346,10 @@ __device__ __forceinline__ bool flag_step(const gga_dev_work &w, uint32_t nq) {
 }
 
 // One step = expand the list of ring slot (step % 3), fill slot (step + 1) % 3.
-template <bool BFS, bool COUNT>
+template <bool BFS, bool COUNT, bool CONT>
 __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_dev_work &w,
                                               uint32_t step, uint32_t nq, uint32_t it,
-                                              uint32_t *pbuf, Counters &cnt) {
+                                              uint32_t *pbuf, uint32_t *cq, Counters &cnt) {
     const uint32_t sout = (step + 1u) % 3u;
     const uint32_t *qin = (step & 1u) ? w.q1 : w.q0;
     uint32_t *qout = (step & 1u) ? w.q0 : w.q1;
@@ -312,6 +374,7 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     const uint32_t thr = w.ctl[GGA_CTL_THR(step % 3u)];
     uint32_t pcount = 0;
     bool dummy = false;
+    uint32_t *const follow = (CONT && nq <= CONT_LIST) ? cq : nullptr;   // the continuation's queue
     // Software pipeline over the wave's batches b, b + waves, b + 2 waves...:
     // while a batch is expanded, the next batch's chunk descriptors and the
     // list entries of the one after it are already in flight, so a batch
@@ -336,9 +399,9 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
         const uint32_t c_after = entry(b + 2ull * waves, ok_after);
         const ChunkDesc nxt = load_chunk(g, c_next, ok_next);
         if (flags)
-            expand_batch<BFS, COUNT, OUT_FLAG>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, thr);
+            expand_batch<BFS, COUNT, OUT_FLAG, CONT>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, thr, follow);
         else
-            expand_batch<BFS, COUNT, OUT_QUEUE>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, thr);
+            expand_batch<BFS, COUNT, OUT_QUEUE, CONT>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, thr, follow);
         cur = nxt;
         c_next = c_after;
         ok_next = ok_after;
@@ -400,10 +463,10 @@ __device__ __forceinline__ uint32_t hw_place() {
 // per SIMD (at most 64 VGPRs; the counting variants would otherwise take 62-65
 // and drop to seven): the relax is bound by the latency of its gathers.
 // BS: workgroup size, 128, 256 or 512 threads (the tuner times all three).
-template <bool BFS, bool COUNT, bool TL, uint32_t BS>
+template <bool BFS, bool COUNT, bool TL, uint32_t BS, bool CONT>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl) {
-    __shared__ uint32_t s_pbuf[BS / GGA_WAVE][PBUF];
+    __shared__ uint32_t s_pbuf[BS / GGA_WAVE][PBUF], s_cq[CONT ? BS / GGA_WAVE : 1][CONT ? 2 * CQ : 1];
     uint64_t t0 = 0;
     if (TL && threadIdx.x == 0) t0 = wall_clock64();
     const uint32_t nq = step_len(w, step);
@@ -411,8 +474,8 @@ void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl
     if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
     if (nq != 0) {
         Counters cnt;
-        worklist_step<BFS, COUNT>(g, w, step, nq, it,
-                                  s_pbuf[__builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE)], cnt);
+        const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);
+        worklist_step<BFS, COUNT, CONT>(g, w, step, nq, it, s_pbuf[wv], s_cq[CONT ? wv : 0], cnt);
         publish_counters<COUNT>(w, cnt);
     }
     if (TL) {
@@ -569,10 +632,10 @@ __global__ __launch_bounds__(BLOCK) void k_refill_write(gga_dev_graph g, gga_dev
 }
 
 // ------------------------------------------------------ persistent kernel
-template <bool BFS, bool COUNT>
+template <bool BFS, bool COUNT, bool CONT>
 __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_work w,
                                                       uint32_t max_steps, uint64_t spin_ticks) {
-    __shared__ uint32_t s_pbuf[WAVES][PBUF];
+    __shared__ uint32_t s_pbuf[WAVES][PBUF], s_cq[CONT ? WAVES : 1][CONT ? 2 * CQ : 1];
     __shared__ uint32_t s_state[2];
     __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
     uint32_t gen = 0;
@@ -591,8 +654,8 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
         if (nq == 0 || err) break;
         const uint32_t it = w.ctl[GGA_CTL_ITER(sin)];
         if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
-        worklist_step<BFS, COUNT>(g, w, step, nq, it,
-                                  s_pbuf[__builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE)], cnt);
+        const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);
+        worklist_step<BFS, COUNT, CONT>(g, w, step, nq, it, s_pbuf[wv], s_cq[CONT ? wv : 0], cnt);
         if (flag_step(w, nq)) {
             const ListOut o = list_out(g, w, step);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
@@ -835,27 +898,35 @@ extern "C" int ggak_topo_step(gga_dev_graph g, gga_dev_work w, int bfs, int coun
     return launch_status();
 }
 
-// Kernel variants by (bfs, count, instrumented) and workgroup size.
-template <bool BFS, bool COUNT, bool TL>
-static const void *wl_fn(uint32_t block) {
+// Kernel variants by (bfs, count, instrumented, mesh continuation) and
+// workgroup size.  The continuation has kernels of its own, so that the
+// registers of the others stay as they were.
+template <bool BFS, bool COUNT, bool TL, bool CONT>
+static const void *wl_fn_c(uint32_t block) {
     switch (block) {
-    case 128: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 128>);
-    case 512: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 512>);
-    default: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 256>);
+    case 128: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 128, CONT>);
+    case 512: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 512, CONT>);
+    default: return reinterpret_cast<const void *>(&k_wl_step<BFS, COUNT, TL, 256, CONT>);
     }
+}
+template <bool BFS, bool COUNT, bool TL>
+static const void *wl_fn(uint32_t block, int cont) {
+    return cont ? wl_fn_c<BFS, COUNT, TL, true>(block) : wl_fn_c<BFS, COUNT, TL, false>(block);
 }
 template <bool BFS, bool COUNT>
-static const void *persistent_fn() {
-    return reinterpret_cast<const void *>(&k_persistent<BFS, COUNT>);
+static const void *persistent_fn(int cont) {
+    return cont ? reinterpret_cast<const void *>(&k_persistent<BFS, COUNT, true>)
+                : reinterpret_cast<const void *>(&k_persistent<BFS, COUNT, false>);
 }
-static const void *pick_kernel(int persistent, int bfs, int count, uint32_t block) {
+// cont: the graph is a mesh (bmin > 1) and the continuation is compiled in
+static int use_cont(const gga_dev_work &w) { return GGA_CONT > 0 && w.bmin > 1u; }
+static const void *pick_kernel(int persistent, int bfs, int count, uint32_t block, int cont) {
     if (persistent) {
-        static const void *const pk[4] = {persistent_fn<false, false>(), persistent_fn<false, true>(),
-                                          persistent_fn<true, false>(), persistent_fn<true, true>()};
-        return pk[(bfs ? 2 : 0) | (count ? 1 : 0)];
+        if (bfs) return count ? persistent_fn<true, true>(cont) : persistent_fn<true, false>(cont);
+        return count ? persistent_fn<false, true>(cont) : persistent_fn<false, false>(cont);
     }
-    if (bfs) return count ? wl_fn<true, true, false>(block) : wl_fn<true, false, false>(block);
-    return count ? wl_fn<false, true, false>(block) : wl_fn<false, false, false>(block);
+    if (bfs) return count ? wl_fn<true, true, false>(block, cont) : wl_fn<true, false, false>(block, cont);
+    return count ? wl_fn<false, true, false>(block, cont) : wl_fn<false, false, false>(block, cont);
 }
 
 extern "C" int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, int bfs, int count,
@@ -863,8 +934,9 @@ extern "C" int ggak_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, int 
     if (grid == 0 || (block != 128 && block != 256 && block != 512)) return (int)hipErrorInvalidValue;
     void *args[] = {&g, &w, &step, &tl};
     // the instrumented variant always counts (its timeline goes with the work counters)
-    const void *fn = tl ? (bfs ? wl_fn<true, true, true>(block) : wl_fn<false, true, true>(block))
-                        : pick_kernel(0, bfs, count, block);
+    const int cont = use_cont(w);
+    const void *fn = tl ? (bfs ? wl_fn<true, true, true>(block, cont) : wl_fn<false, true, true>(block, cont))
+                        : pick_kernel(0, bfs, count, block, cont);
     return (int)hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
 }
 
@@ -920,7 +992,9 @@ extern "C" uint32_t ggak_wl_grid(int device, int persistent, int bfs, int count,
     if (persistent) block = BLOCK;
     if (block != 128 && block != 256 && block != 512) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    const void *fn = pick_kernel(persistent, bfs, count, block);
+    // (the continuation variants fit as many workgroups: same register cap,
+    // LDS well within a CU's; persistent grids take one per CU either way)
+    const void *fn = pick_kernel(persistent, bfs, count, block, 0);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, (int)block, 0) != hipSuccess) return 0;
     if (per <= 0) return 0;
     // A grid barrier needs every workgroup resident at once.  On gfx950 the
@@ -942,8 +1016,8 @@ extern "C" int ggak_persistent(gga_dev_graph g, gga_dev_work w, int bfs, int cou
         khz = 100000;  // 100 MHz
     uint64_t spin_ticks = (uint64_t)spin_limit_ms * (uint64_t)khz;
     void *args[] = {&g, &w, &max_steps, &spin_ticks};
-    hipError_t e = hipLaunchCooperativeKernel(pick_kernel(1, bfs, count, BLOCK), dim3(grid), dim3(BLOCK),
-                                              args, 0, s);
+    hipError_t e = hipLaunchCooperativeKernel(pick_kernel(1, bfs, count, BLOCK, use_cont(w)), dim3(grid),
+                                              dim3(BLOCK), args, 0, s);
     if (e != hipSuccess) return (int)e;
     return launch_status();
 }

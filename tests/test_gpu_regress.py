@@ -63,15 +63,17 @@ def test_stamps_do_not_leak_across_queries(gpu):
 
 def test_failed_query_does_not_poison_the_next(gpu):
     """A query stopped by its step cap (GGA_ETIMEDOUT) leaves stamps of
-    unknown step numbers; the next query on the handle is still exact."""
-    g = gga.gen_grid(30, 30, shortcut_frac=0.0, seed=2, wlo=1, whi=50)
+    unknown step numbers; the next query on the handle is still exact.  (A
+    mesh large enough that two steps cannot finish it, although its waves
+    follow their wavefronts past their batches.)"""
+    g = gga.gen_grid(200, 200, shortcut_frac=0.0, seed=2, wlo=1, whi=50)
     with gga.DeviceGraph(g) as dg:
         dg.split(mda=4)
         for mode in ("worklist", "persistent"):
             announce(f"capped {mode}")
             with pytest.raises(_lib.GgaError, match="GGA_ETIMEDOUT"):
-                dg.sssp(0, mode=mode, max_iters=3, delta=0xFFFFFFFF, dense_min=0xFFFFFFFF)
-            for s in (0, 450, 899):
+                dg.sssp(0, mode=mode, max_iters=2, delta=0xFFFFFFFF, dense_min=0xFFFFFFFF)
+            for s in (0, g.n // 2 + 100, g.n - 1):
                 d, _ = dg.sssp(s, mode="worklist", delta=0xFFFFFFFF, dense_min=0xFFFFFFFF)
                 np.testing.assert_array_equal(d, oracle.dijkstra(g.row_ptr, g.dst, g.w, s))
 
@@ -257,3 +259,56 @@ def test_near_far_flag_steps_past_the_byte_wrap(gpu):
             d, st = dg.sssp(s, mode="persistent", dense_min=1, delta=7)
             np.testing.assert_array_equal(d, oracle.dijkstra(g.row_ptr, g.dst, g.w, s))
     assert wrapped >= 4, f"only {wrapped} queries ran past 256 steps"
+
+
+DENSE = {"auto": 0, "flags": 1, "queue": 0xFFFFFFFF}
+
+
+def test_mesh_wavefront_continuation_equals_oracle(gpu):
+    """On meshes a wave follows the vertices it lowered past its batch within
+    the step (single source and the 32-lane engine).  The extra rounds only
+    bring improvements forward, so every result must still equal the
+    oracle's, in every mode and list representation, BFS (which then takes
+    queue stamps, being no longer level-synchronous within a step) and SSSP
+    with and without near-far buckets -- and the step count must fall well
+    below the BFS eccentricity, showing that the rounds ran."""
+    g = gga.gen_grid(160, 190, 0.0, seed=23, wlo=1, whi=1000)
+    srcs = [0, g.n // 2 + 77, g.n - 1]
+    with gga.DeviceGraph(g) as dg:
+        dg.split()
+        for s in srcs:
+            want_b = oracle.bfs_queue(g.row_ptr, g.dst, s)
+            want_s = oracle.dijkstra(g.row_ptr, g.dst, g.w, s)
+            ecc = int(want_b.max())
+            for mode in ("worklist", "persistent", "auto"):
+                for rep in DENSE:
+                    announce(f"mesh continuation src={s} mode={mode} frontier={rep}")
+                    d, st = dg.bfs(s, mode=mode, dense_min=DENSE[rep])
+                    np.testing.assert_array_equal(d, want_b, err_msg=f"bfs src={s} {mode} {rep}")
+                    if mode == "worklist":
+                        assert st["iterations"] < ecc // 2, (st["iterations"], ecc)
+                    for delta in (0xFFFFFFFF, 0, 1, 2000):
+                        d, _ = dg.sssp(s, mode=mode, dense_min=DENSE[rep], delta=delta)
+                        np.testing.assert_array_equal(d, want_s, err_msg=f"sssp src={s} {mode} {rep} {delta}")
+        # the engine: 16 BFS + 16 SSSP lanes on the same mesh
+        rng = np.random.default_rng(9)
+        lanes = [int(x) for x in rng.choice(g.n, 32, replace=False)]
+        eng = gga.MultiSourceEngine(dg)
+        try:
+            announce("mesh continuation engine 32 lanes")
+            eng.start(dict(enumerate(lanes)), bfs=set(range(16)))
+            live, steps = 0xFFFFFFFF, 0
+            while live and steps < 100000:
+                live = eng.run(max_steps=12, mode="worklist", sync_every=12)
+                steps += 12
+            assert live == 0
+            out = eng.fetch(range(32))
+            eccs = []
+            for i, s in enumerate(lanes):
+                wb = oracle.bfs_queue(g.row_ptr, g.dst, s)
+                want = wb if i < 16 else oracle.dijkstra(g.row_ptr, g.dst, g.w, s)
+                np.testing.assert_array_equal(out[i], want, err_msg=f"engine lane {i} src {s}")
+                eccs.append(int(wb.max()))
+            assert steps < max(eccs), (steps, max(eccs))
+        finally:
+            eng.close()
