@@ -14,6 +14,10 @@
 
 #include "gga_host.h"
 
+#ifndef GGA_MS_MESH_BMIN
+#define GGA_MS_MESH_BMIN 16u   /* owners per wave batch in engines on mesh-like graphs */
+#endif
+
 struct gga_ms {
     gga_graph *g;
     gga_graph_view_t v;
@@ -138,7 +142,7 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
     e->mesh = gga_graph_mesh_like(g);
     /* meshes: lanes rarely share a vertex, batches flatten (edge, lane) pairs,
      * so each wave takes at least 16 low-degree chunks per batch */
-    e->d.bmin = gga_graph_mesh_like(g) ? 16u : 1u;
+    e->d.bmin = gga_graph_mesh_like(g) ? GGA_MS_MESH_BMIN : 1u;
     HIP_TRY(hipSetDevice(e->v.device));
     /* a stream of its own, so that engines of one graph run side by side */
     HIP_TRY(hipStreamSynchronize(e->v.stream));

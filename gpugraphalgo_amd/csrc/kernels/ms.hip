@@ -165,6 +165,30 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     // lanes the wave sets in next masks, ORed into its LDS word (no register
     // stays live across the loop for it)
     if (lane_id() == 0) *nl = 0u;
+    // Lists (x, lane sx) for the next step where `want`: its next-mask bit,
+    // the lane word, and a push when the mask was empty.  Whole-wave call.
+    auto list_pair = [&](bool want, uint32_t x, uint32_t sx) {
+        bool first = false;
+        if (want) {
+            first = __hip_atomic_fetch_or(&mnxt[x], 1u << sx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+            __hip_atomic_fetch_or(nl, 1u << sx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+        if (push) {
+            const uint64_t pm = __ballot(first);
+            if (pm) {
+                const uint32_t np = (uint32_t)__popcll(pm);
+                if (pcount + np > PBUF) {
+                    flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
+                    pcount = 0;
+                }
+                if (first)
+                    pbuf[pcount + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] = x;
+                pcount += np;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    };
     for (uint32_t b = wid; b < nbatch; b += waves) {
         const uint32_t idx = b * bsz + l;
         const bool valid = l < bsz && idx < nq;
@@ -191,10 +215,12 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
             // round walks up to 64 (vertex, lane) pairs that this wave
             // improved below the lane's threshold in an earlier round, from
             // the distance it wrote there: the wave follows its own wavefront
-            // a few hops further within the step.  Each such vertex is listed
-            // for the next step all the same, so the extra rounds only bring
-            // improvements forward; the fixed point, and so the result, is
-            // unchanged.
+            // a few hops further within the step.  A queued pair is listed
+            // for the next step only if it is not followed (queue full,
+            // budget spent, degree above CONT_DEG); a followed pair had every
+            // edge relaxed from the distance written, so each improvement is
+            // either expanded at its value or listed, as in a plain step, and
+            // the fixed point -- the result -- is unchanged.
             bool later = false;          // a continuation round (wave-uniform)
             uint32_t from = 0;           // its pair's distance at the owner
             uint32_t qh = 0, qt = 0, budget = cont ? (uint32_t)GGA_MS_CONT : 0u;
@@ -236,22 +262,20 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                         imp = a < old;
                     }
                     const bool near = imp && a < thr;
-                    if (near)
-                        first = __hip_atomic_fetch_or(&mnxt[v[i]], 1u << sl[i], __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT) == 0u;
-                    if (near) __hip_atomic_fetch_or(nl, 1u << sl[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-                    else if (imp && ms.far) {
+                    if (imp && !near && ms.far) {
                         atomicOr(&ms.far[v[i]], 1u << sl[i]);
                         ms.tfar[v[i] >> 6] = 1u;
                     }
+                    bool queued = false;
                     if (budget) {
-                        // queue the pair for a later round (dropped when the queue is full)
+                        // queue the pair for a later round; listed now only when the queue is full
                         const uint64_t qm = __ballot(near);
                         if (qm) {
                             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(qm >> 32),
                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)qm, 0u));
                             const uint32_t room = CQ - (qt - qh), nnear = (uint32_t)__popcll(qm);
-                            if (near && rank < room) {
+                            queued = near && rank < room;
+                            if (queued) {
                                 const uint32_t slot = (qt + rank) % CQ;
                                 cqv[slot] = v[i];
                                 cqs[slot] = sl[i];
@@ -264,45 +288,44 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                         cnt.edges += (uint32_t)__popcll(__ballot(on));
                         cnt.relax += (uint32_t)__popcll(__ballot(imp));
                     }
-                    if (push) {
-                        const uint64_t pm = __ballot(first);
-                        if (pm) {
-                            const uint32_t np = (uint32_t)__popcll(pm);
-                            if (pcount + np > PBUF) {
-                                flush_pushes(g, pbuf, pcount, qout, tail, ms.qcap, ms.ctl);
-                                pcount = 0;
-                            }
-                            if (first) {
-                                const uint32_t pos = pcount + __builtin_amdgcn_mbcnt_hi(
-                                                                  (uint32_t)(pm >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
-                                pbuf[pos] = v[i];
-                            }
-                            pcount += np;
-                            __builtin_amdgcn_wave_barrier();
-                        }
-                    }
+                    list_pair(near && !queued, v[i], sl[i]);
                 }
             }
-            if (qh == qt || budget == 0) break;
+            if (qh == qt) break;
+            __builtin_amdgcn_wave_barrier();
+            if (budget == 0) {
+                // budget spent: what is still queued is listed for the next step
+                for (; qh != qt; qh += (qt - qh) < GGA_WAVE ? (qt - qh) : GGA_WAVE) {
+                    const bool has = l < qt - qh;
+                    const uint32_t slot = (qh + l) % CQ;
+                    list_pair(has, has ? cqv[slot] : 0u, has ? cqs[slot] : 0u);
+                }
+                break;
+            }
             // the next round: up to 64 queued pairs, one per wave lane, each
             // owning all edges of its vertex
-            __builtin_amdgcn_wave_barrier();
             const uint32_t avail = qt - qh;
             uint32_t take = avail < GGA_WAVE ? avail : GGA_WAVE;
             take = take < budget ? take : budget;
             later = true;
             u = 0; m = 0; eb = 0; from = 0;
-            uint32_t ne1 = 0;
+            uint32_t ne1 = 0, sq = 0;
+            bool hub = false;
             if (l < take) {
                 const uint32_t slot = (qh + l) % CQ;
                 u = cqv[slot];
-                m = 1u << cqs[slot];
+                sq = cqs[slot];
+                m = 1u << sq;
                 from = cqa[slot];
                 eb = g.cbeg[g.cstart[u]];
                 ne1 = g.cbeg[g.cstart[u + 1]] - eb;
-                if (ne1 > CONT_DEG) ne1 = 0;
+                hub = ne1 > CONT_DEG;
+                if (hub) ne1 = 0;
+                // followed from below the threshold: a far bit of the lane
+                // here is stale (no deferral below the threshold can follow it)
+                else if (ms.far && (ms.far[u] & m)) atomicAnd(&ms.far[u], ~m);
             }
+            list_pair(hub, u, sq);   // too many edges to follow: the next step expands it
             qh += take;
             budget -= take;
             pc = l < take ? 1u : 0u;

@@ -160,6 +160,28 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
     constexpr bool cont = CONT && OUT != OUT_CHANGED;
     uint32_t *const cqv = cq, *const cqa = cq + CQ;   // vertex, the distance this wave wrote
     uint32_t qh = 0, qt = 0, budget = (cont && cq) ? (uint32_t)GGA_CONT : 0u;   // wave-uniform
+    // Lists x for the next step where `want`: the step's flag byte, or (queue
+    // steps) the step stamp taken by exchange and a push.  Whole-wave call.
+    auto list_vertex = [&](bool want, uint32_t x) {
+        if (OUT == OUT_FLAG) {
+            if (want) w.flag[x] = (uint8_t)it;
+        } else if (OUT == OUT_QUEUE) {
+            if ((!BFS || cont) && want) want = (w.mark[x] != it) && (atomic_xchg_u32(&w.mark[x], it) != it);
+            const uint64_t mask = __ballot(want);
+            if (mask) {
+                const uint32_t nw = (uint32_t)__popcll(mask);
+                if (pcount + nw > PBUF) {
+                    flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
+                    pcount = 0;
+                }
+                if (want)
+                    pbuf[pcount + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u))] = x;
+                pcount += nw;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    };
     for (;;) {
     const uint32_t incl = wave_incl_scan(n);
     const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
@@ -197,7 +219,6 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
             if (act[u] && alt >= ds[u] && alt < inf && alt < cur[u]) {
                 if (OUT == OUT_FLAG) {
                     (void)atomic_min_u32(&dist[v[u]], alt);  // result unused: no-return atomic
-                    if (alt < thr) w.flag[v[u]] = (uint8_t)it;
                     imp = true;
                 } else {
                     const uint32_t old = atomic_min_u32(&dist[v[u]], alt);
@@ -205,63 +226,63 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
                 }
             }
             if (COUNT) cnt.relax += __builtin_amdgcn_readfirstlane((uint32_t)__popcll(__ballot(imp)));
+            if (OUT == OUT_CHANGED) {
+                improved |= imp;
+                continue;
+            }
+            const bool near = imp && alt < thr;
+            bool queued = false;
             if (budget) {
-                // queue the vertex for a later round (dropped when the queue is full)
-                const bool near = imp && alt < thr;
+                // queue the vertex for a later round; listed now only when the queue is full
                 const uint64_t qm = __ballot(near);
                 if (qm) {
                     const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(qm >> 32),
                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)qm, 0u));
                     const uint32_t room = CQ - (qt - qh), nnear = (uint32_t)__popcll(qm);
-                    if (near && rank < room) {
+                    queued = near && rank < room;
+                    if (queued) {
                         cqv[(qt + rank) % CQ] = v[u];
                         cqa[(qt + rank) % CQ] = alt;
                     }
                     qt += nnear < room ? nnear : room;
                 }
             }
-            if (OUT == OUT_QUEUE) {
-                bool want = imp && alt < thr;
-                if ((!BFS || cont) && want)
-                    want = (w.mark[v[u]] != it) && (atomic_xchg_u32(&w.mark[v[u]], it) != it);
-                const uint64_t mask = __ballot(want);
-                if (mask) {
-                    const uint32_t nw = (uint32_t)__popcll(mask);
-                    if (pcount + nw > PBUF) {
-                        flush_pushes(g, pbuf, pcount, qout, tail, w.qcap, w.ctl);
-                        pcount = 0;
-                    }
-                    if (want) {
-                        const uint32_t pos = pcount + __builtin_amdgcn_mbcnt_hi(
-                                                          (uint32_t)(mask >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                        pbuf[pos] = v[u];
-                    }
-                    pcount += nw;
-                    __builtin_amdgcn_wave_barrier();
-                }
-            } else {
-                improved |= imp;
-            }
+            list_vertex(near && !queued, v[u]);
         }
     }
-    if (qh == qt || budget == 0) break;
-    // the next round: up to 64 queued vertices, one per wave lane
+    if (qh == qt) break;
     __builtin_amdgcn_wave_barrier();
+    if (budget == 0) {
+        // budget spent: what is still queued is listed for the next step
+        for (; qh != qt; qh += (qt - qh) < GGA_WAVE ? (qt - qh) : GGA_WAVE) {
+            const bool has = l < qt - qh;
+            list_vertex(has, has ? cqv[(qh + l) % CQ] : 0u);
+        }
+        break;
+    }
+    // the next round: up to 64 queued vertices, one per wave lane; a vertex
+    // with more than CONT_DEG edges is listed instead.  A followed vertex
+    // records the distance it was expanded at (near-far), as a listed one
+    // would when the next step expands it.
     const uint32_t avail = qt - qh;
     uint32_t take = avail < GGA_WAVE ? avail : GGA_WAVE;
     take = take < budget ? take : budget;
     eb = 0;
     n = 0;
     du = inf;
+    bool hub = false;
+    uint32_t x = 0;
     if (l < take) {
         const uint32_t slot = (qh + l) % CQ;
-        const uint32_t x = cqv[slot];
+        x = cqv[slot];
         du = cqa[slot];
         eb = g.cbeg[g.cstart[x]];
         n = g.cbeg[g.cstart[x + 1]] - eb;
-        if (n > CONT_DEG) n = 0;
+        hub = n > CONT_DEG;
+        if (hub) n = 0;
+        else if (w.last) w.last[x] = du;
     }
+    list_vertex(hub, x);
     qh += take;
     budget -= take;
     }
