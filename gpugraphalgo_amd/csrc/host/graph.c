@@ -19,7 +19,7 @@
 #include "gga_host.h"
 
 #ifndef GGA_MESH_BMIN
-#define GGA_MESH_BMIN 16u   /* chunks per wave batch on mesh-like graphs */
+#define GGA_MESH_BMIN 4u    /* chunks per wave batch on mesh-like graphs */
 #endif
 static int mesh_like(const gga_graph *g);
 

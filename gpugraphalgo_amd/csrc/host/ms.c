@@ -15,7 +15,7 @@
 #include "gga_host.h"
 
 #ifndef GGA_MS_MESH_BMIN
-#define GGA_MS_MESH_BMIN 16u   /* owners per wave batch in engines on mesh-like graphs */
+#define GGA_MS_MESH_BMIN 4u    /* owners per wave batch in engines on mesh-like graphs */
 #endif
 
 struct gga_ms {
@@ -140,8 +140,10 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
     e->dense_default = e->n / 64u > 4096u ? e->n / 64u : 4096u;
     e->d.dense_min = e->dense_default;
     e->mesh = gga_graph_mesh_like(g);
-    /* meshes: lanes rarely share a vertex, batches flatten (edge, lane) pairs,
-     * so each wave takes at least 16 low-degree chunks per batch */
+    /* meshes: lanes rarely share a vertex, batches flatten (edge, lane) pairs;
+     * at least GGA_MS_MESH_BMIN owners per batch, each wave then following
+     * its own wavefront (4096^2 grid, 32 / 8 lanes to convergence: batches
+     * of 16, 256 followed pairs: 435 / 173 ms; 4 and 1 024: 343 / 154 ms) */
     e->d.bmin = gga_graph_mesh_like(g) ? GGA_MS_MESH_BMIN : 1u;
     HIP_TRY(hipSetDevice(e->v.device));
     /* a stream of its own, so that engines of one graph run side by side */

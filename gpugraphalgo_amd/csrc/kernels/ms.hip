@@ -54,7 +54,7 @@ constexpr uint32_t PBUF = 256;                 // pushed vertices staged per wav
 // one step (0: off), the wave's LDS queue of them, and the largest degree
 // followed (a hub is left to the next step's chunks).
 #ifndef GGA_MS_CONT
-#define GGA_MS_CONT 256
+#define GGA_MS_CONT 1024
 #endif
 #ifndef GGA_MS_CQ
 #define GGA_MS_CQ 128

@@ -382,7 +382,8 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
     // the next step (R-MAT s22 SSSP, batches per wave 4 / 8 / 16 / 32: 2.33 /
     // 2.17 / 2.14 / 2.26 ms).  Uniform small chunks (meshes, bmin > 1): few
     // large batches -- every batch is a chain of dependent loads -- with one
-    // batch per 1/1024 of the list, but at least bmin chunks
+    // batch per 1/1024 of the list, but at least bmin chunks (4: each wave
+    // then follows its own wavefront, see expand_batch)
     const uint32_t wpb = blockDim.x / GGA_WAVE;   // waves per workgroup (128, 256 or 512 threads)
     const uint32_t waves = gridDim.x * wpb;
     const uint32_t wid = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);
