@@ -32,6 +32,7 @@ uint32_t gga_graph_wmean(const gga_graph *g);
 uint32_t gga_graph_auto_delta(const gga_graph *g, int bfs);
 int gga_graph_mesh_like(const gga_graph *g);   /* max degree within 4x the mean (+8) */
 uint32_t gga_ms_step_grid(gga_ms *e);          /* an engine's default relax grid (0 on error) */
+uint32_t gga_graph_engine_delta(const gga_graph *g);   /* engines' default near-far width (0: off) */
 uint32_t gga_graph_maxdeg(const gga_graph *g);
 /* Occupancy grid of the worklist kernel at `block` threads (0: the graph's size). */
 uint32_t gga_graph_default_grid(gga_graph *g, int bfs, uint32_t block);

@@ -167,7 +167,7 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out)
     K_TRY(ggak_fill_u32(e->d.M1, 0u, e->n, e->v.stream));
     K_TRY(ggak_ms_clear(e->d, e->v.stream));
     HIP_TRY(hipStreamSynchronize(e->v.stream));
-    if ((rc = gga_ms_set_delta(e, gga_graph_auto_delta(g, 0)))) goto fail;
+    if ((rc = gga_ms_set_delta(e, gga_graph_engine_delta(g)))) goto fail;
     *out = e;
     return GGA_OK;
 fail:
@@ -657,7 +657,7 @@ int gga_multi_sp(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t unit_l
     {
         /* near-far width: opts, else the graph's; the cooperative launch runs
          * plain Bellman-Ford lanes */
-        uint32_t want = oo.delta == 0xFFFFFFFFu ? 0u : (oo.delta ? oo.delta : gga_graph_auto_delta(g, 0));
+        uint32_t want = oo.delta == 0xFFFFFFFFu ? 0u : (oo.delta ? oo.delta : gga_graph_engine_delta(g));
         if (oo.mode == GGA_MODE_PERSISTENT) want = 0;
         if (want != e->d.delta && (rc = gga_ms_set_delta(e, want))) return rc;
     }

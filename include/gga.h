@@ -343,8 +343,9 @@ int gga_ms_create(gga_graph *g, uint32_t inf, uint32_t layout, gga_ms **out);
 int gga_ms_free(gga_ms *e);
 /* Near-far buckets per SSSP lane (as opts.delta for one source): a lane's
  * improvement at or beyond its threshold waits until a refill reaches it.
- * Set at creation from the graph (gga_tune's width, or 32 mean weights on
- * mesh-like graphs, else off); delta 0 or 0xFFFFFFFF turns it off.  Only while
+ * Set at creation from the graph (gga_tune's width, or 16 mean weights on
+ * mesh-like graphs -- half the single-source default -- else off); delta 0 or
+ * 0xFFFFFFFF turns it off.  gga_multi_sp uses the same default.  Only while
  * no lane runs; near-far engines run in step mode (GGA_MODE_WORKLIST).
  * +4 B per vertex (the far masks). */
 int gga_ms_set_delta(gga_ms *e, uint32_t delta);
