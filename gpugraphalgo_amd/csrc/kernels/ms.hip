@@ -148,7 +148,7 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
     uint32_t *__restrict__ D = ms.D;
     const uint32_t inf = ms.inf;
     const uint32_t waves = gridDim.x * WAVES;
-    const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
+    const uint32_t wid = blockIdx.x * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);
     uint32_t bsz = ms.bmin > 1u ? nq >> 10 : nq / (16u * waves);   // ~16 batches per wave on ragged graphs
     bsz = bsz < ms.bmin ? ms.bmin : bsz;
     bsz = bsz < 1u ? 1u : (bsz > GGA_WAVE ? GGA_WAVE : bsz);
