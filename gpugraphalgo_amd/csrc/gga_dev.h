@@ -102,9 +102,9 @@ typedef struct gga_dev_ms {
     /* near-far per lane (NULL / 0: plain Bellman-Ford lanes) */
     uint32_t *far;       /* per vertex: lanes whose distance there dropped by a deferred
                             improvement and was not listed or expanded since (n words) */
-    uint32_t *tfar;      /* per 64-vertex tile: non-zero when a vertex of the tile may hold
-                            far bits (set with every far bit, cleared by the refill when it
-                            finds the tile empty; GGA_MS_FAR_TILES(n) words) */
+    uint32_t *tfar;      /* per 64-vertex tile: the lanes that may hold far bits in the tile
+                            (ORed with every far bit, rewritten by the refill from the bits
+                            it leaves, cleared for restarted lanes; GGA_MS_FAR_TILES(n) words) */
     uint32_t *lpart;     /* refill: per-workgroup per-lane minimum deferred distance */
     uint32_t delta;      /* bucket width */
     uint32_t bmin;       /* fewest chunks per wave batch (more on meshes, whose lanes rarely meet) */

@@ -264,7 +264,9 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                     const bool near = imp && a < thr;
                     if (imp && !near && ms.far) {
                         atomicOr(&ms.far[v[i]], 1u << sl[i]);
-                        ms.tfar[v[i] >> 6] = 1u;
+                        // the tile's lane summary (bits are only added during
+                        // a step, so a stale read shows a subset)
+                        if (!(ms.tfar[v[i] >> 6] & (1u << sl[i]))) atomicOr(&ms.tfar[v[i] >> 6], 1u << sl[i]);
                     }
                     bool queued = false;
                     if (budget) {
@@ -382,7 +384,7 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
                     const uint32_t fm = (uint32_t)(__ballot(imp && a >= thr_s) >> (32u * half));
                     if (s == 0 && fm) {
                         atomicOr(&ms.far[v[i]], fm);
-                        ms.tfar[v[i] >> 6] = 1u;
+                        if ((ms.tfar[v[i] >> 6] & fm) != fm) atomicOr(&ms.tfar[v[i] >> 6], fm);
                     }
                 }
                 if (COUNT) {
@@ -460,7 +462,10 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reset(gga_dev_graph g, gga_dev_ms 
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t s = (uint32_t)(i & (LANES - 1u)), v = (uint32_t)(i / LANES);
         if ((lanes >> s) & 1u) ms.D[i] = v == src.v[s] ? 0u : ms.inf;
-        if (s == 0 && ms.far) ms.far[v] &= ~lanes;
+        if (s == 0 && ms.far) {
+            ms.far[v] &= ~lanes;
+            if ((v & 63u) == 0) ms.tfar[v >> 6] &= ~lanes;
+        }
     }
 }
 
@@ -474,7 +479,10 @@ __global__ __launch_bounds__(BLOCK) void k_ms_reset_lm(gga_dev_graph g, gga_dev_
     const bool clear_far = ms.far && s == (uint32_t)__builtin_ctz(lanes);
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (uint64_t)gridDim.x * blockDim.x) {
         row[v] = v == src.v[s] ? 0u : ms.inf;
-        if (clear_far) ms.far[v] &= ~lanes;
+        if (clear_far) {
+            ms.far[v] &= ~lanes;
+            if ((v & 63u) == 0) ms.tfar[v >> 6] &= ~lanes;
+        }
     }
 }
 
@@ -555,8 +563,9 @@ __global__ __launch_bounds__(CBLOCK) void k_ms_write(gga_dev_graph g, gga_dev_ms
 // delta; with nothing under the new threshold but deferred pairs left, the
 // threshold jumps to the smallest deferred distance (listed by the next
 // step's refill); with nothing deferred the lane has converged.  The scan
-// reads one far-tile word per 64 vertices, the far word of each vertex in a
-// marked tile, and D only where a candidate's bit is set.
+// reads one far-tile word per 64 vertices (the lanes deferred in the tile),
+// the far word of each vertex in a tile with a candidate's lane, and D only
+// where a candidate's bit is set.
 constexpr uint32_t RBLOCKS = GGA_MS_RBLOCKS;
 __global__ __launch_bounds__(BLOCK) void k_ms_tail(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_t1[LANES], s_min[LANES], s_listed, s_last, s_done;
@@ -581,14 +590,14 @@ __global__ __launch_bounds__(BLOCK) void k_ms_tail(gga_dev_graph g, gga_dev_ms m
     __syncthreads();
     uint32_t listed = 0, pcount = 0;
     // Each wave reads 64 consecutive far-tile words in one load, then walks
-    // only the marked tiles, one vertex per lane (wave-uniform: the pushes
-    // below use whole-wave operations).
+    // only the tiles holding a candidate lane's bits, one vertex per lane
+    // (wave-uniform: the pushes below use whole-wave operations).
     const uint64_t ntiles = GGA_MS_FAR_TILES(g.n);
     const uint64_t wstride = (uint64_t)gridDim.x * WAVES * GGA_WAVE;
     for (uint64_t tb = ((uint64_t)blockIdx.x * WAVES + threadIdx.x / GGA_WAVE) * GGA_WAVE; tb < ntiles;
          tb += wstride) {
         const uint64_t tl = tb + lane_id();
-        uint64_t tm = __ballot(tl < ntiles && ms.tfar[tl] != 0u);
+        uint64_t tm = __ballot(tl < ntiles && (ms.tfar[tl] & cand) != 0u);   // tiles with a candidate's bit
         while (tm) {
             const uint64_t tile = tb + (uint64_t)__builtin_ctzll(tm);
             tm &= tm - 1u;
@@ -612,9 +621,11 @@ __global__ __launch_bounds__(BLOCK) void k_ms_tail(gga_dev_graph g, gga_dev_ms m
                     first = __hip_atomic_fetch_or(&mnxt[v], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
                 }
             }
-            // nothing deferred left in the tile (far bits are only cleared
-            // while this kernel runs; the step that sets them ran before)
-            if (!__ballot(left != 0u) && lane_id() == 0) ms.tfar[tile] = 0u;
+            // the lanes still deferred in the tile (far bits are only cleared
+            // while this kernel runs; the step that sets them ran before, and
+            // each tile has one wave here)
+            const uint32_t keep = wave_or(left);
+            if (lane_id() == 0) ms.tfar[tile] = keep;
             const uint64_t pm = __ballot(first);
             if (pm) {
                 const uint32_t np = (uint32_t)__popcll(pm);
