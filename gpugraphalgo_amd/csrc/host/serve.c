@@ -235,6 +235,21 @@ static int make_room(gga_server *s, uint64_t need)
     return 1;
 }
 
+/* Layout of a new engine: the options', else the graph's tuned one, else
+ * lane-major.  A served engine mostly runs a few lanes, and lane-major rows
+ * keep each lane's gathers near one another, where a 128-byte D[v] line
+ * carries 32 lanes for the few in use: on the config-5 set, lane-major
+ * engines took the p99 at 240 / 330 / 395 q/s offered from 313 / 688 /
+ * 736 ms (vertex-major) to 202 / 357 / 492 ms, while 400 queries at once
+ * (full 32-lane batches, where vertex-major lines are shared) dropped from
+ * 775 to 615 q/s.  GGA_MS_VERTEX_MAJOR in the options restores the latter. */
+static uint32_t engine_layout(const gga_server *s, const srv_graph *G)
+{
+    if (s->o.layout) return s->o.layout;
+    const uint32_t t = G->g ? gga_graph_tuned_layout(G->g) : 0u;
+    return t ? t : GGA_MS_LANE_MAJOR;
+}
+
 /* The engine a query of G goes to: the fullest resident engine with a free
  * lane, else a new one (budget and engines_per_graph permitting), else NULL. */
 static srv_engine *place(gga_server *s, srv_graph *G, uint32_t lanes_all, int *rc)
@@ -255,7 +270,7 @@ static srv_engine *place(gga_server *s, srv_graph *G, uint32_t lanes_all, int *r
         srv_engine *E = &G->eng[k];
         if (E->resident) continue;
         if (!make_room(s, G->engine_bytes)) return NULL;
-        if (!s->o.dry_run && (*rc = gga_ms_create(G->g, 0, s->o.layout, &E->e))) return NULL;
+        if (!s->o.dry_run && (*rc = gga_ms_create(G->g, 0, engine_layout(s, G), &E->e))) return NULL;
         E->resident = 1;
         s->used += G->engine_bytes;
         if (s->used > s->peak) s->peak = s->used;

@@ -469,7 +469,9 @@ typedef struct gga_server_opts {
                                  completes dry_steps rounds after admission (tests of
                                  admission and batching on hosts without a GPU) */
     uint32_t dry_steps;       /* 0 -> 3 */
-    uint32_t layout;          /* GGA_MS_* layout of every engine (0 -> per graph, GGA_MS_AUTO) */
+    uint32_t layout;          /* GGA_MS_* layout of every engine (0 -> the graph's tuned layout,
+                                 else lane-major: fewer bytes per step when an engine runs a few
+                                 lanes; GGA_MS_VERTEX_MAJOR serves full 32-lane bursts faster) */
     uint32_t engines_per_graph; /* engines one graph may hold at once, 1..GGA_SERVER_MAX_ENGINES
                                    (0 -> 1): a further engine takes a graph's queries while the
                                    others' lanes are all busy */
