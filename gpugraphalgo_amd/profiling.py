@@ -89,8 +89,9 @@ def summarize_traffic(kernel: str, fetch: str, write: str, copy_fetch: str, copy
     return "\n".join(lines) + "\n"
 
 
-def _per_dispatch(d: str, kernel: str, exact: bool = True) -> dict:
-    """Mean over the dispatches of `kernel` of each counter in d's collection."""
+def _per_dispatch(d: str, kernel: str, exact: bool = True, last: int = 0) -> dict:
+    """Mean over the dispatches of `kernel` (the last `last` of them, when
+    given) of each counter in d's collection."""
     path = _one(os.path.join(d, "**", "*counter_collection.csv"))
     per: dict = {}
     for r in csv.DictReader(open(path)):
@@ -100,6 +101,8 @@ def _per_dispatch(d: str, kernel: str, exact: bool = True) -> dict:
             per[r["Dispatch_Id"]][r["Counter_Name"]] += float(r["Counter_Value"])
     if not per:
         raise ValueError(f"no dispatch of {kernel} in {path}")
+    if last:
+        per = {k: per[k] for k in sorted(per, key=int)[-last:]}
     keys = {k for v in per.values() for k in v}
     return {"_dispatches": len(per), **{k: sum(v.get(k, 0.0) for v in per.values()) / len(per) for k in keys}}
 
@@ -109,20 +112,24 @@ def _read_bytes(c: dict) -> float:
             + 128.0 * c["TCC_EA0_RDREQ_128B_sum"])
 
 
-def _write_bytes(c: dict) -> float:
+def _write32(c: dict) -> float:
     # WRREQ counts 32- and 64-byte transactions; atomics travel on the same
-    # interface as 32-byte ones
-    n64 = c["TCC_EA0_WRREQ_64B_sum"]
-    n32 = c["TCC_EA0_WRREQ_sum"] - n64 - c.get("TCC_EA0_ATOMIC_sum", 0.0)
-    return 64.0 * n64 + 32.0 * n32
+    # interface as 32-byte ones.  When WRREQ falls short of the atomics plus
+    # the 64-byte writes (the counters do not add up for some atomic mixes),
+    # no 32-byte plain write is inferred.
+    return max(0.0, c["TCC_EA0_WRREQ_sum"] - c["TCC_EA0_WRREQ_64B_sum"] - c.get("TCC_EA0_ATOMIC_sum", 0.0))
+
+
+def _write_bytes(c: dict) -> float:
+    return 64.0 * c["TCC_EA0_WRREQ_64B_sum"] + 32.0 * _write32(c)
 
 
 def summarize_counters(kernel: str, hitmiss: str, rdsize: str, wratom: str, dram: str, copy_rd: str,
-                       copy_wr: str, copy_bytes: int, cmd: str) -> str:
-    hm = _per_dispatch(hitmiss, kernel)
-    rd = _per_dispatch(rdsize, kernel)
-    wa = _per_dispatch(wratom, kernel)
-    dr = _per_dispatch(dram, kernel)
+                       copy_wr: str, copy_bytes: int, cmd: str, last: int = 0) -> str:
+    hm = _per_dispatch(hitmiss, kernel, last=last)
+    rd = _per_dispatch(rdsize, kernel, last=last)
+    wa = _per_dispatch(wratom, kernel, last=last)
+    dr = _per_dispatch(dram, kernel, last=last)
     cr = _per_dispatch(copy_rd, "copyBuffer", exact=False)
     cw = _per_dispatch(copy_wr, "copyBuffer", exact=False)
     # the copy: its largest dispatches are the timed 1 GiB copies
@@ -132,7 +139,7 @@ def summarize_counters(kernel: str, hitmiss: str, rdsize: str, wratom: str, dram
         "# L2 (TCC) counters per dispatch: hit rate, bytes past L2 by request size, atomics",
         f"command = {cmd}",
         f"kernel = {kernel}",
-        f"dispatches = {rd['_dispatches']}",
+        f"dispatches = {rd['_dispatches']}" + (f" (the last {last})" if last else ""),
         f"l2_requests = {hm['TCC_REQ_sum']:.0f}",
         f"l2_hits = {hm['TCC_HIT_sum']:.0f}",
         f"l2_misses = {hm['TCC_MISS_sum']:.0f}",
@@ -143,6 +150,7 @@ def summarize_counters(kernel: str, hitmiss: str, rdsize: str, wratom: str, dram
         f"read_requests_dram = {dr['TCC_EA0_RDREQ_DRAM_sum']:.0f}",
         f"write_requests = {wa['TCC_EA0_WRREQ_sum']:.0f}",
         f"write_requests_64B = {wa['TCC_EA0_WRREQ_64B_sum']:.0f}",
+        f"write_requests_32B_inferred = {_write32(wa):.0f}",
         f"write_requests_dram = {dr['TCC_EA0_WRREQ_DRAM_sum']:.0f}",
         f"atomics = {wa['TCC_ATOMIC_sum']:.0f}",
         f"atomics_past_l2 = {wa['TCC_EA0_ATOMIC_sum']:.0f}",
@@ -181,12 +189,13 @@ def main(argv=None) -> int:
     for k in ("kernel", "hitmiss", "rdsize", "wratom", "dram", "copy-rd", "copy-wr", "cmd"):
         c.add_argument("--" + k, required=True)
     c.add_argument("--copy-bytes", type=int, required=True)
+    c.add_argument("--last", type=int, default=0)
     a = ap.parse_args(argv)
     if a.what == "stats":
         sys.stdout.write(summarize_stats(a.dir, a.cmd))
     elif a.what == "counters":
         sys.stdout.write(summarize_counters(a.kernel, a.hitmiss, a.rdsize, a.wratom, a.dram, a.copy_rd,
-                                            a.copy_wr, a.copy_bytes, a.cmd))
+                                            a.copy_wr, a.copy_bytes, a.cmd, a.last))
     else:
         sys.stdout.write(summarize_traffic(a.kernel, a.fetch, a.write, a.copy_fetch, a.copy_write,
                                            a.copy_bytes, a.cmd, a.last))
