@@ -335,7 +335,8 @@ typedef struct gga_ms gga_ms;
  * vertices share every gather -- small-world graphs), GGA_MS_LANE_MAJOR (each
  * lane's distances contiguous: lanes whose frontiers rarely overlap, e.g.
  * meshes, gather only their own 4 bytes), or GGA_MS_AUTO (the tuner's choice
- * for the graph, gga_tune_layout).  Re-splitting the graph invalidates it. */
+ * for the graph, gga_tune_layout; untuned: lane-major on mesh-like graphs,
+ * else vertex-major).  Re-splitting the graph invalidates it. */
 #define GGA_MS_AUTO          0
 #define GGA_MS_VERTEX_MAJOR  1
 #define GGA_MS_LANE_MAJOR    2
