@@ -124,8 +124,8 @@ template <bool BFS, bool COUNT, int OUT, bool CONT = false>
 __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_dev_work &w,
                                              const ChunkDesc &d, uint32_t it, uint32_t *pbuf,
                                              uint32_t &pcount, uint32_t *qout, uint32_t *tail,
-                                             bool &improved, Counters &cnt, uint32_t thr = 0xFFFFFFFFu,
-                                             uint32_t *cq = nullptr) {
+                                             bool &improved, Counters &cnt, uint2 *tab, uint32_t nown,
+                                             uint32_t thr = 0xFFFFFFFFu, uint32_t *cq = nullptr) {
     const uint32_t l = lane_id();
     uint32_t *__restrict__ dist = w.dist;
     const uint32_t inf = w.inf;
@@ -190,30 +190,46 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
         cnt.chunks += __builtin_amdgcn_readfirstlane((uint32_t)__popcll(__ballot(n > 0)));
     }
     const uint32_t excl = incl - n;
-    // UNROLL slots per lane per round.  Slots past the end are clamped to the
-    // last slot, so every load below has an in-bounds address and all UNROLL
-    // gathers of a lane are issued before the first result is used.
+    // the wave's owner table (LDS): slot k of owner j reads edge
+    // (eb_j - excl_j) + k from distance du_j -- one LDS load per slot
+    // instead of two more shuffles
+    tab[l] = make_uint2(eb - excl, du);
+    __builtin_amdgcn_wave_barrier();
+    // owners are lanes 0 .. nown - 1: search the next power of two of them
+    const uint32_t pw = nown <= 1u ? 1u : 1u << (32 - __builtin_clz(nown - 1u));
+    // UNROLL groups of 64 slots per round.  Within a group, slots past the end
+    // are clamped to the last slot, so every load below has an in-bounds
+    // address and all gathers of a lane are issued before the first result is
+    // used; a group wholly past the end is skipped (wave-uniform) -- a small
+    // batch, e.g. one chunk of a dozen edges, runs one group, not UNROLL.
     for (uint32_t base = 0; base < total; base += GGA_WAVE * UNROLL) {
         uint32_t e[UNROLL], ds[UNROLL], v[UNROLL], wt[UNROLL], cur[UNROLL];
         bool act[UNROLL];
+        const uint32_t left = total - base;   // > 0
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; u++) {
+            if (u * GGA_WAVE >= left) break;
             const uint32_t k = base + u * GGA_WAVE + l;
             act[u] = k < total;
             const uint32_t kk = act[u] ? k : total - 1;
-            const uint32_t j = wave_owner(incl, kk);
-            e[u] = wave_bcast(eb, j) + (kk - wave_bcast(excl, j));
-            ds[u] = wave_bcast(du, j);
+            const uint2 t = tab[wave_owner_p(incl, kk, pw)];
+            e[u] = t.x + kk;
+            ds[u] = t.y;
         }
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; u++) {
+            if (u * GGA_WAVE >= left) break;
             v[u] = stream_load(&g.col[e[u]]);
             wt[u] = BFS ? 1u : stream_load(&g.w[e[u]]);
         }
 #pragma unroll
-        for (uint32_t u = 0; u < UNROLL; u++) cur[u] = dist[v[u]];
+        for (uint32_t u = 0; u < UNROLL; u++) {
+            if (u * GGA_WAVE >= left) break;
+            cur[u] = dist[v[u]];
+        }
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; u++) {
+            if (u * GGA_WAVE >= left) break;
             bool imp = false;
             const uint32_t alt = ds[u] + wt[u];
             if (act[u] && alt >= ds[u] && alt < inf && alt < cur[u]) {
@@ -251,7 +267,7 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
         }
     }
     if (qh == qt) break;
-    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_wave_barrier();   // (also: this round's table reads precede the next round's writes)
     if (budget == 0) {
         // budget spent: what is still queued is listed for the next step
         for (; qh != qt; qh += (qt - qh) < GGA_WAVE ? (qt - qh) : GGA_WAVE) {
@@ -285,6 +301,7 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
     list_vertex(hub, x);
     qh += take;
     budget -= take;
+    nown = take;
     }
 }
 
@@ -355,8 +372,10 @@ __global__ __launch_bounds__(BLOCK) void k_topo_step(gga_dev_graph g, gga_dev_wo
     bool improved = false;
     uint32_t pcount = 0;
     Counters cnt;
+    __shared__ uint2 s_tab[WAVES][GGA_WAVE];
     expand_batch<BFS, COUNT, OUT_CHANGED>(g, w, load_chunk(g, valid ? (uint32_t)c : 0u, valid), 0u, nullptr,
-                                          pcount, nullptr, nullptr, improved, cnt);
+                                          pcount, nullptr, nullptr, improved, cnt,
+                                          s_tab[__builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE)], GGA_WAVE);
     if (__ballot(improved) && lane_id() == 0) w.ctl[GGA_CTL_CHANGED] = 1u;  // *changed = true
     publish_counters<COUNT>(w, cnt);
 }
@@ -370,7 +389,7 @@ __device__ __forceinline__ bool flag_step(const gga_dev_work &w, uint32_t nq) {
 template <bool BFS, bool COUNT, bool CONT>
 __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_dev_work &w,
                                               uint32_t step, uint32_t nq, uint32_t it,
-                                              uint32_t *pbuf, uint32_t *cq, Counters &cnt) {
+                                              uint32_t *pbuf, uint32_t *cq, uint2 *tab, Counters &cnt) {
     const uint32_t sout = (step + 1u) % 3u;
     const uint32_t *qin = (step & 1u) ? w.q1 : w.q0;
     uint32_t *qout = (step & 1u) ? w.q0 : w.q1;
@@ -421,9 +440,11 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
         const uint32_t c_after = entry(b + 2ull * waves, ok_after);
         const ChunkDesc nxt = load_chunk(g, c_next, ok_next);
         if (flags)
-            expand_batch<BFS, COUNT, OUT_FLAG, CONT>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, thr, follow);
+            expand_batch<BFS, COUNT, OUT_FLAG, CONT>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, tab, bsz,
+                                                     thr, follow);
         else
-            expand_batch<BFS, COUNT, OUT_QUEUE, CONT>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, thr, follow);
+            expand_batch<BFS, COUNT, OUT_QUEUE, CONT>(g, w, cur, it, pbuf, pcount, qout, tail, dummy, cnt, tab, bsz,
+                                                      thr, follow);
         cur = nxt;
         c_next = c_after;
         ok_next = ok_after;
@@ -489,6 +510,7 @@ template <bool BFS, bool COUNT, bool TL, uint32_t BS, bool CONT>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl) {
     __shared__ uint32_t s_pbuf[BS / GGA_WAVE][PBUF], s_cq[CONT ? BS / GGA_WAVE : 1][CONT ? 2 * CQ : 1];
+    __shared__ uint2 s_tab[BS / GGA_WAVE][GGA_WAVE];
     uint64_t t0 = 0;
     if (TL && threadIdx.x == 0) t0 = wall_clock64();
     const uint32_t nq = step_len(w, step);
@@ -497,7 +519,7 @@ void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl
     if (nq != 0) {
         Counters cnt;
         const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);
-        worklist_step<BFS, COUNT, CONT>(g, w, step, nq, it, s_pbuf[wv], s_cq[CONT ? wv : 0], cnt);
+        worklist_step<BFS, COUNT, CONT>(g, w, step, nq, it, s_pbuf[wv], s_cq[CONT ? wv : 0], s_tab[wv], cnt);
         publish_counters<COUNT>(w, cnt);
     }
     if (TL) {
@@ -658,6 +680,7 @@ template <bool BFS, bool COUNT, bool CONT>
 __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_work w,
                                                       uint32_t max_steps, uint64_t spin_ticks) {
     __shared__ uint32_t s_pbuf[WAVES][PBUF], s_cq[CONT ? WAVES : 1][CONT ? 2 * CQ : 1];
+    __shared__ uint2 s_tab[WAVES][GGA_WAVE];
     __shared__ uint32_t s_state[2];
     __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
     uint32_t gen = 0;
@@ -677,7 +700,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
         const uint32_t it = w.ctl[GGA_CTL_ITER(sin)];
         if (blockIdx.x == 0 && threadIdx.x == 0) step_bookkeeping(w.ctl, step, it, nq);
         const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / GGA_WAVE);
-        worklist_step<BFS, COUNT, CONT>(g, w, step, nq, it, s_pbuf[wv], s_cq[CONT ? wv : 0], cnt);
+        worklist_step<BFS, COUNT, CONT>(g, w, step, nq, it, s_pbuf[wv], s_cq[CONT ? wv : 0], s_tab[wv], cnt);
         if (flag_step(w, nq)) {
             const ListOut o = list_out(g, w, step);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
