@@ -41,7 +41,7 @@ __device__ __forceinline__ void flush_pushes(const gga_dev_graph &g, const uint3
             nch = g.cstart[v + 1] - cs;
         }
         const uint32_t incl = wave_incl_scan(nch);
-        const uint32_t tot = wave_bcast(incl, GGA_WAVE - 1);
+        const uint32_t tot = wave_last(incl);
         uint32_t base = 0;
         if (l == 0) base = atomic_add_u32(tail, tot);
         base = wave_bcast(base, 0);

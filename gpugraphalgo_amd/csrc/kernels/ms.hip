@@ -201,7 +201,7 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
             ne = m ? stream_load(&g.cbeg[c + 1]) - eb : 0u;
         }
         const uint32_t incl = wave_incl_scan(ne);
-        const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+        const uint32_t total = wave_last(incl);
         const uint32_t excl = incl - ne;
         if (COUNT) cnt.chunks += (uint32_t)__popcll(__ballot(ne > 0));
         // (edge, lane) pairs of the batch; with few lanes per edge slot on
@@ -209,7 +209,7 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
         uint32_t pc = (uint32_t)__popc(m);
         uint32_t pairs = pc * ne;
         uint32_t pincl = wave_incl_scan(pairs);
-        uint32_t ptotal = wave_bcast(pincl, GGA_WAVE - 1);
+        uint32_t ptotal = wave_last(pincl);
         if (ptotal <= GGA_MS_FLAT * total) {
             // Round 0 walks the batch's owners.  On meshes (cont) each later
             // round walks up to 64 (vertex, lane) pairs that this wave
@@ -333,7 +333,7 @@ __device__ __forceinline__ void ms_relax(const gga_dev_graph &g, const gga_dev_m
             pc = l < take ? 1u : 0u;
             pairs = pc * ne1;
             pincl = wave_incl_scan(pairs);
-            ptotal = wave_bcast(pincl, GGA_WAVE - 1);
+            ptotal = wave_last(pincl);
             }
             continue;
         }

@@ -87,7 +87,7 @@ __device__ __forceinline__ void push_level(const gga_dev_graph &g, const gga_dev
             ne = stream_load(&g.cbeg[c + 1]) - eb;
         }
         const uint32_t incl = wave_incl_scan(ne);
-        const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+        const uint32_t total = wave_last(incl);
         const uint32_t excl = incl - ne;
         for (uint32_t base = 0; base < total; base += GROUPS * UNROLL) {
             uint32_t v[UNROLL];

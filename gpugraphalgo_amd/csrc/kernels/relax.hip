@@ -184,7 +184,7 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
     };
     for (;;) {
     const uint32_t incl = wave_incl_scan(n);
-    const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+    const uint32_t total = wave_last(incl);
     if (COUNT) {
         cnt.edges += __builtin_amdgcn_readfirstlane(total);
         cnt.chunks += __builtin_amdgcn_readfirstlane((uint32_t)__popcll(__ballot(n > 0)));
@@ -195,25 +195,41 @@ __device__ __forceinline__ void expand_batch(const gga_dev_graph &g, const gga_d
     // instead of two more shuffles
     tab[l] = make_uint2(eb - excl, du);
     __builtin_amdgcn_wave_barrier();
-    // owners are lanes 0 .. nown - 1: search the next power of two of them
-    const uint32_t pw = nown <= 1u ? 1u : 1u << (32 - __builtin_clz(nown - 1u));
+    // owners are lanes 0 .. nown - 1: the search starts at half the next
+    // power of two of them (wave-uniform)
+    const uint32_t half = nown <= 1u ? 0u : 1u << (31 - __builtin_clz(nown - 1u));
     // UNROLL groups of 64 slots per round.  Within a group, slots past the end
     // are clamped to the last slot, so every load below has an in-bounds
     // address and all gathers of a lane are issued before the first result is
-    // used; a group wholly past the end is skipped (wave-uniform) -- a small
-    // batch, e.g. one chunk of a dozen edges, runs one group, not UNROLL.
+    // used; a group wholly past the end is skipped (total and left are
+    // scalars, so the skips are scalar branches) -- a small batch, e.g. one
+    // chunk of a dozen edges, runs one group, not UNROLL.
     for (uint32_t base = 0; base < total; base += GGA_WAVE * UNROLL) {
-        uint32_t e[UNROLL], ds[UNROLL], v[UNROLL], wt[UNROLL], cur[UNROLL];
+        uint32_t e[UNROLL], ds[UNROLL], v[UNROLL], wt[UNROLL], cur[UNROLL], kk[UNROLL], own[UNROLL];
         bool act[UNROLL];
         const uint32_t left = total - base;   // > 0
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; u++) {
-            if (u * GGA_WAVE >= left) break;
             const uint32_t k = base + u * GGA_WAVE + l;
             act[u] = k < total;
-            const uint32_t kk = act[u] ? k : total - 1;
-            const uint2 t = tab[wave_owner_p(incl, kk, pw)];
-            e[u] = t.x + kk;
+            kk[u] = act[u] ? k : total - 1;
+            own[u] = 0;
+        }
+        // slot k's owner: the first lane whose inclusive prefix exceeds k
+        // (binary search, one shuffle per halving; the groups' shuffles are
+        // independent and issued together, skipped groups included)
+        for (uint32_t step = half; step != 0; step >>= 1) {
+#pragma unroll
+            for (uint32_t u = 0; u < UNROLL; u++) {
+                const uint32_t y = __shfl(incl, (int)(own[u] + step - 1), GGA_WAVE);
+                own[u] += y <= kk[u] ? step : 0u;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; u++) {
+            if (u * GGA_WAVE >= left) break;
+            const uint2 t = tab[own[u]];
+            e[u] = t.x + kk[u];
             ds[u] = t.y;
         }
 #pragma unroll
@@ -843,7 +859,7 @@ __global__ __launch_bounds__(BLOCK) void k_verify_edges(gga_dev_graph g, gga_dev
         n = du < w.inf ? g.cbeg[c + 1] - eb : 0u;
     }
     const uint32_t incl = wave_incl_scan(n);
-    const uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+    const uint32_t total = wave_last(incl);
     const uint32_t excl = incl - n;
     uint64_t nerr = 0;
     for (uint32_t base = 0; base < total; base += GGA_WAVE) {

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_split_fill(const uint32_t *__restrict__
         local = deg ? (deg - 1u) / nch + 1u : 0u;  // ceil(deg / nch)
     }
     uint32_t incl = wave_incl_scan(nch);
-    uint32_t total = wave_bcast(incl, GGA_WAVE - 1);
+    uint32_t total = wave_last(incl);
     uint32_t first = wave_bcast(cs, 0);
     for (uint32_t base = 0; base < total; base += GGA_WAVE) {
         uint32_t k = base + l;

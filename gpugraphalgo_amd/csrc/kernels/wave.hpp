@@ -15,15 +15,32 @@ namespace gga {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Inclusive prefix sum over the 64 lanes (all lanes must be active).
+// x + (x of the lane DPP control CTRL selects), in the rows of 16 lanes
+// ROW_MASK enables; lanes whose source lies outside their row, and rows not
+// enabled, add 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_add(uint32_t x) {
+    return x + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xf, false);
+}
+
+// Inclusive prefix sum over the 64 lanes (all lanes must be active), on the
+// DPP network: shifts right by 1, 2, 4 and 8 lanes within each row of 16
+// give each row's own prefix sums; a row broadcast of lane 15 adds row 0's
+// total to row 1 and row 2's to row 3, one of lane 31 adds rows 0-1's total
+// to rows 2 and 3.  Six VALU operations, no LDS traffic.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-    const uint32_t l = lane_id();
-#pragma unroll
-    for (uint32_t o = 1; o < GGA_WAVE; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, GGA_WAVE);
-        if (l >= o) x += y;
-    }
+    x = dpp_add<0x111, 0xf>(x);   // row_shr:1
+    x = dpp_add<0x112, 0xf>(x);   // row_shr:2
+    x = dpp_add<0x114, 0xf>(x);   // row_shr:4
+    x = dpp_add<0x118, 0xf>(x);   // row_shr:8
+    x = dpp_add<0x142, 0xa>(x);   // row_bcast:15 into rows 1 and 3
+    x = dpp_add<0x143, 0xc>(x);   // row_bcast:31 into rows 2 and 3
     return x;
+}
+
+// Lane 63's x, as a wave-uniform (scalar) value.
+__device__ __forceinline__ uint32_t wave_last(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)(GGA_WAVE - 1));
 }
 
 __device__ __forceinline__ uint32_t wave_bcast(uint32_t x, uint32_t src_lane) {
