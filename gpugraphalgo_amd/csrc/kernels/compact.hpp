@@ -194,4 +194,28 @@ __device__ __forceinline__ void compact_write_tile(const ListOut &o, uint32_t n,
     write_items(o, cs, nch, o.tiles[t] + excl);
 }
 
+// Passes 2 and 3 in one launch (every workgroup of the grid calls this):
+// tiles[] still holds the counts, and each workgroup sums the counts of the
+// tiles before its own -- at most GGA_TILES(n) L2-resident words per
+// workgroup, cheaper than the single-workgroup scan pass's launch, which
+// costs a kernel boundary (~4.5 us on gfx950) per list.  The workgroup of
+// the last tile stores the list length.
+template <class Marked>
+__device__ __forceinline__ void compact_write_tiles(const ListOut &o, uint32_t n, const Marked &marked,
+                                                    uint32_t *s_wave) {
+    const uint32_t ntiles = (uint32_t)GGA_TILES((uint64_t)n);
+    uint32_t off = 0, upto = 0;   // off = tiles[0] + ... + tiles[upto - 1]
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint32_t part = 0, ex;
+        for (uint32_t i = upto + threadIdx.x; i < t; i += blockDim.x) part += o.tiles[i];
+        off += block_excl_scan(part, ex, s_wave);
+        upto = t;
+        uint32_t cs[CITEMS], nch[CITEMS], excl;
+        const uint32_t mine = tile_items(o, n, t, marked, cs, nch);
+        const uint32_t tot = block_excl_scan(mine, excl, s_wave);
+        write_items(o, cs, nch, off + excl);
+        if (t == ntiles - 1u && threadIdx.x == 0) *o.tail = off + tot;
+    }
+}
+
 }  // namespace gga

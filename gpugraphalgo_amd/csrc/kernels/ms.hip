@@ -526,7 +526,8 @@ void k_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     publish_counters<COUNT>(ms, cnt);
 }
 
-// The three compaction passes after a flag step (no-ops after a push step).
+// The compaction passes after a flag step, count then write (which sums the
+// counts before each tile itself); no-ops after a push step.
 __global__ __launch_bounds__(CBLOCK) void k_ms_count(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
     const uint32_t nq = ms_len(ms, step);
@@ -536,20 +537,11 @@ __global__ __launch_bounds__(CBLOCK) void k_ms_count(gga_dev_graph g, gga_dev_ms
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, mk, s_wave);
 }
 
-__global__ __launch_bounds__(CBLOCK) void k_ms_scan(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
-    __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
-    const uint32_t nq = ms_len(ms, step);
-    if (nq == 0 || !ms_flag_step(ms, nq)) return;
-    compact_scan(ms_list_out(g, ms, step), g.n, s_wave);
-}
-
 __global__ __launch_bounds__(CBLOCK) void k_ms_write(gga_dev_graph g, gga_dev_ms ms, uint32_t step) {
     __shared__ uint32_t s_wave[CBLOCK / GGA_WAVE];
     const uint32_t nq = ms_len(ms, step);
     if (nq == 0 || !ms_flag_step(ms, nq)) return;
-    const ListOut o = ms_list_out(g, ms, step);
-    const MaskMarked mk{ms_mnxt(ms, step)};
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
+    compact_write_tiles(ms_list_out(g, ms, step), g.n, MaskMarked{ms_mnxt(ms, step)}, s_wave);
 }
 
 // ---- the tail of a step: the finish pass, then (near-far) the refill of
@@ -961,7 +953,6 @@ extern "C" int ggak_ms_step(gga_dev_graph g, gga_dev_ms ms, uint32_t step, int c
     const uint32_t cg = compact_grid(g.n);
     if (ms.dense_min != 0xFFFFFFFFu) {   // push-only engines never rebuild from the masks
         k_ms_count<<<cg, CBLOCK, 0, s>>>(g, ms, step);
-        k_ms_scan<<<1, CBLOCK, 0, s>>>(g, ms, step);
         k_ms_write<<<cg, CBLOCK, 0, s>>>(g, ms, step);
     }
     // lpart holds one row per workgroup of the tail pass

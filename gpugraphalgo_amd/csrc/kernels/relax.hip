@@ -550,8 +550,9 @@ void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl
     }
 }
 
-// The three compaction passes run after k_wl_step(step); each is a no-op
-// unless that step was a flag step.
+// The compaction passes run after k_wl_step(step): count, then write (which
+// also sums the counts before each tile); each is a no-op unless that step
+// was a flag step.
 __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_dev_work w, uint32_t step) {
     __shared__ uint32_t s_wave[WAVES];
     const uint32_t nq = step_len(w, step);
@@ -561,20 +562,12 @@ __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_de
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_compact_scan(gga_dev_graph g, gga_dev_work w, uint32_t step) {
-    __shared__ uint32_t s_wave[WAVES];
-    const uint32_t nq = step_len(w, step);
-    if (nq == 0 || !flag_step(w, nq)) return;
-    compact_scan(list_out(g, w, step), g.n, s_wave);
-}
-
 __global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_dev_work w, uint32_t step) {
     __shared__ uint32_t s_wave[WAVES];
     const uint32_t nq = step_len(w, step);
     if (nq == 0 || !flag_step(w, nq)) return;
-    const ListOut o = list_out(g, w, step);
     const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
+    compact_write_tiles(list_out(g, w, step), g.n, StampMarked{w.flag, (uint8_t)it}, s_wave);
 }
 
 // ---- near-far refill: after a step whose next list came out empty, list
@@ -674,21 +667,62 @@ __device__ __forceinline__ void refill_write_body(const gga_dev_graph &g, const 
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, pred, s_wave);
 }
 
+// The standalone refill is two launches: the count, whose first thread also
+// records the threshold the refill lists under (RTHR) and that this step
+// refills (RSTEP = its step number), then the write, which sums the counts
+// before each tile itself (compact_write_tiles); its first workgroup folds
+// the total and the tiles' minima into the next threshold as the scan body
+// above does.  The write reads only RTHR / RSTEP, which no workgroup of it
+// changes, so the list length it stores midway does not change its course.
 __global__ __launch_bounds__(BLOCK) void k_refill_count(gga_dev_graph g, gga_dev_work w, uint32_t step) {
     __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
-    if (refill_needed(w, step)) refill_count_body(g, w, step, s_wave, s_min);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_refill_scan(gga_dev_graph g, gga_dev_work w, uint32_t step) {
-    __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
-    if (refill_needed(w, step)) refill_scan_body(g, w, step, s_wave, s_min);
+    if (!refill_needed(w, step)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.ctl[GGA_CTL_RTHR] = sat_add(w.ctl[GGA_CTL_THR((step + 1u) % 3u)], w.delta);
+        w.ctl[GGA_CTL_RSTEP] = w.ctl[GGA_CTL_ITER(step % 3u)];
+    }
+    refill_count_body(g, w, step, s_wave, s_min);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_refill_write(gga_dev_graph g, gga_dev_work w, uint32_t step) {
-    __shared__ uint32_t s_wave[WAVES];
+    __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
     if (!w.last) return;
     if (w.ctl[GGA_CTL_RSTEP] != w.ctl[GGA_CTL_ITER(step % 3u)]) return;  // no refill after this step
-    refill_write_body(g, w, step, s_wave);
+    const uint32_t sout = (step + 1u) % 3u, t1 = w.ctl[GGA_CTL_RTHR];
+    const ListOut o = list_out(g, w, step);
+    if (blockIdx.x == 0) {
+        // the new threshold: t1 when the refill lists anything, else a jump
+        // to the smallest deferred distance (FAR: refill again), else done
+        const uint32_t ntiles = GGA_TILES(g.n);
+        uint32_t sum = 0, m = 0xFFFFFFFFu, ex;
+        for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
+            sum += o.tiles[i];
+            m = w.tmin[i] < m ? w.tmin[i] : m;
+        }
+        const uint32_t total = block_excl_scan(sum, ex, s_wave);
+#pragma unroll
+        for (uint32_t off = 32; off >= 1; off >>= 1) {
+            const uint32_t y = __shfl_xor(m, (int)off, GGA_WAVE);
+            m = y < m ? y : m;
+        }
+        if (lane_id() == 0) s_min[threadIdx.x / GGA_WAVE] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t gmin = s_min[0];
+            for (uint32_t i = 1; i < WAVES; i++) gmin = s_min[i] < gmin ? s_min[i] : gmin;
+            if (total) {
+                w.ctl[GGA_CTL_THR(sout)] = t1;
+                w.ctl[GGA_CTL_FAR] = 0u;
+            } else if (gmin != 0xFFFFFFFFu) {
+                w.ctl[GGA_CTL_THR(sout)] = gmin;   // the next refill lists [gmin, gmin + delta)
+                w.ctl[GGA_CTL_FAR] = 1u;
+            } else {
+                w.ctl[GGA_CTL_FAR] = 0u;           // nothing deferred: converged
+            }
+        }
+        __syncthreads();
+    }
+    compact_write_tiles(o, g.n, DeferredUnder{w.dist, w.last, t1}, s_wave);
 }
 
 // ------------------------------------------------------ persistent kernel
@@ -1034,7 +1068,6 @@ extern "C" int ggak_refill(gga_dev_graph g, gga_dev_work w, uint32_t step, hipSt
     const uint32_t ntiles = GGA_TILES(g.n);
     const uint32_t grid = ntiles < 2048u ? (ntiles ? ntiles : 1u) : 2048u;
     k_refill_count<<<grid, BLOCK, 0, s>>>(g, w, step);
-    k_refill_scan<<<1, BLOCK, 0, s>>>(g, w, step);
     k_refill_write<<<grid, BLOCK, 0, s>>>(g, w, step);
     return launch_status();
 }
@@ -1043,7 +1076,6 @@ extern "C" int ggak_compact(gga_dev_graph g, gga_dev_work w, uint32_t step, hipS
     const uint32_t ntiles = GGA_TILES(g.n);
     const uint32_t grid = ntiles < 2048u ? (ntiles ? ntiles : 1u) : 2048u;
     k_compact_count<<<grid, BLOCK, 0, s>>>(g, w, step);
-    k_compact_scan<<<1, BLOCK, 0, s>>>(g, w, step);
     k_compact_write<<<grid, BLOCK, 0, s>>>(g, w, step);
     return launch_status();
 }
