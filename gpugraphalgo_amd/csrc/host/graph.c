@@ -675,6 +675,7 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
     const int tev = own && count;
     const uint32_t max_iters = p->max_iters;
     float ktime = 0.f;
+    int ended = 0;   /* reach pass and control words already fetched */
     *adv = 0;
 
     HIP_TRY(hipEventRecord(g->ev[0], g->stream));              /* bfs_sssp.cu:350 */
@@ -759,10 +760,16 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
                 if (abs_step < conv_n) K_TRY(ggak_conv(dw.dist, d_fin, g->n, d_conv + 2ull * abs_step, g->stream));
             }
             step += nb;
-            HIP_TRY(hipMemcpyAsync(g->h_pin, dw.ctl + GGA_CTL_TAIL(step % 3u), 4, hipMemcpyDeviceToHost,
-                                   g->stream));
-            HIP_TRY(hipMemcpyAsync(g->h_pin + 1, dw.ctl + GGA_CTL_FAR, 4, hipMemcpyDeviceToHost, g->stream));
+            /* the query's ending (m_reach, counters, all control words) is
+             * queued behind every batch, before the host looks: when the
+             * batch converged -- repeated queries do at the first look --
+             * nothing is left to launch after the host's round trip (~30 us
+             * of idle device before the reach pass otherwise) */
+            HIP_TRY(hipEventRecord(g->ev[1], g->stream));      /* bfs_sssp.cu:469 */
+            K_TRY(ggak_reach(dg, dw, g->stream));
+            HIP_TRY(hipMemcpyAsync(g->h_pin, dw.ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, g->stream));
             HIP_TRY(hipStreamSynchronize(g->stream));
+            ended = 1;
             if (tev)
                 for (uint32_t k = 0; k < nb; k++) {
                     float t = 0.f, tc = 0.f;
@@ -773,7 +780,7 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
                 }
             s->launches += nb;
             nb = per;
-            if (g->h_pin[0] == 0 && (!dw.last || g->h_pin[1] == 0)) break;
+            if (g->h_pin[GGA_CTL_TAIL(step % 3u)] == 0 && (!dw.last || g->h_pin[GGA_CTL_FAR] == 0)) break;
             if (step >= max_iters) {
                 rc = GGA_ETIMEDOUT;
                 gga_set_error("worklist mode: %u steps without convergence", step);
@@ -831,10 +838,12 @@ static int run_query(gga_graph *g, gga_dev_graph dg, gga_dev_work dw, const quer
         HIP_TRY(hipEventElapsedTime(&ktime, g->ev[2], g->ev[3]));
         s->launches = 1;
     }
-    HIP_TRY(hipEventRecord(g->ev[1], g->stream));              /* bfs_sssp.cu:469 */
-    K_TRY(ggak_reach(dg, dw, g->stream));
-    HIP_TRY(hipMemcpyAsync(g->h_pin, dw.ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (!ended) {
+        HIP_TRY(hipEventRecord(g->ev[1], g->stream));          /* bfs_sssp.cu:469 */
+        K_TRY(ggak_reach(dg, dw, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->h_pin, dw.ctl, 4ull * GGA_CTL_WORDS, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
     HIP_TRY(hipEventElapsedTime(&s->time_ms, g->ev[0], g->ev[1]));
     {
         const uint32_t *c = g->h_pin;

@@ -817,8 +817,26 @@ constexpr uint32_t REACH_BLOCKS = GGA_CNT_SLOTS / 2;
 __global__ __launch_bounds__(BLOCK) void k_reach(gga_dev_graph g, gga_dev_work w) {
     __shared__ uint64_t s_red[WAVES];
     uint64_t reached = 0, mr = 0;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-         v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t S = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // four vertices S apart per round, every load issued before the first
+    // is used (a latency-bound loop otherwise: 20 us on 4.2 M vertices)
+    for (; v + 3u * S < g.n; v += 4u * S) {
+        uint32_t d[4], r0[4], r1[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            d[k] = w.dist[v + k * S];
+            r0[k] = g.row[v + k * S];
+            r1[k] = g.row[v + k * S + 1u];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if (d[k] < w.inf) {
+                reached += 1;
+                mr += r1[k] - r0[k];
+            }
+    }
+    for (; v < g.n; v += S) {
         if (w.dist[v] < w.inf) {
             reached += 1;
             mr += g.row[v + 1] - g.row[v];
