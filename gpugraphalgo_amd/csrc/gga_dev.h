@@ -176,6 +176,9 @@ typedef struct gga_dev_msbfs {
 #define GGA_CTL_FAR       (6u * GGA_CTL_LINE + 8u)
 #define GGA_CTL_RTHR      (6u * GGA_CTL_LINE + 9u)
 #define GGA_CTL_RSTEP     (6u * GGA_CTL_LINE + 10u)   /* step number (ITER) of the last refill */
+/* non-zero when the count pass after a flag step found a stamped vertex (so
+ * the list it rebuilds for ring slot s is not empty) */
+#define GGA_CTL_LISTED(s) (6u * GGA_CTL_LINE + 12u + (s))
 
 /* multi-source engine (kernels/ms.hip): lane mask of a ring slot's list */
 #define GGA_CTL_LANES(s)  (7u * GGA_CTL_LINE + 8u + (s))

@@ -91,13 +91,15 @@ __device__ __forceinline__ uint32_t tile_items(const ListOut &o, uint32_t n, uin
     return mine;
 }
 
-// pass 1, one tile: tiles[t] = number of chunk ids tile t contributes
+// pass 1, one tile: tiles[t] = number of chunk ids tile t contributes (also
+// returned, to every thread)
 template <class Marked>
-__device__ __forceinline__ void compact_count_tile(const ListOut &o, uint32_t n, uint32_t t, const Marked &marked,
-                                                   uint32_t *s_wave) {
+__device__ __forceinline__ uint32_t compact_count_tile(const ListOut &o, uint32_t n, uint32_t t,
+                                                       const Marked &marked, uint32_t *s_wave) {
     uint32_t cs[CITEMS], nch[CITEMS], excl;
     const uint32_t tot = block_excl_scan(tile_items(o, n, t, marked, cs, nch), excl, s_wave);
     if (threadIdx.x == 0) o.tiles[t] = tot;
+    return tot;
 }
 
 // pass 2 (one workgroup): exclusive scan of tiles[]; the total is the new list

@@ -500,6 +500,7 @@ __device__ __forceinline__ void step_bookkeeping(uint32_t *ctl, uint32_t step, u
     ctl[GGA_CTL_ITER(sout)] = it + 1u;
     ctl[GGA_CTL_THR(sout)] = ctl[GGA_CTL_THR(step % 3u)];
     ctl[GGA_CTL_TAIL(sclr)] = 0u;
+    ctl[GGA_CTL_LISTED(sout)] = 0u;
     if (nq) ctl[GGA_CTL_STEPS] += 1u;
 }
 
@@ -551,23 +552,21 @@ void k_wl_step(gga_dev_graph g, gga_dev_work w, uint32_t step, gga_wg_record *tl
 }
 
 // The compaction passes run after k_wl_step(step): count, then write (which
-// also sums the counts before each tile); each is a no-op unless that step
-// was a flag step.
+// also sums the counts before each tile, and with near-far buckets starts
+// the refill when the list came out empty, below); both are no-ops after a
+// queue step.  A count workgroup that finds stamped vertices marks the slot
+// LISTED (plain stores of the same value), so that the write pass knows
+// whether the list is empty without summing every count.
 __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_dev_work w, uint32_t step) {
     __shared__ uint32_t s_wave[WAVES];
     const uint32_t nq = step_len(w, step);
     if (nq == 0 || !flag_step(w, nq)) return;
     const ListOut o = list_out(g, w, step);
     const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
-    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_dev_work w, uint32_t step) {
-    __shared__ uint32_t s_wave[WAVES];
-    const uint32_t nq = step_len(w, step);
-    if (nq == 0 || !flag_step(w, nq)) return;
-    const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
-    compact_write_tiles(list_out(g, w, step), g.n, StampMarked{w.flag, (uint8_t)it}, s_wave);
+    uint32_t any = 0;
+    for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
+        any |= compact_count_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
+    if (any && threadIdx.x == 0) w.ctl[GGA_CTL_LISTED((step + 1u) % 3u)] = 1u;
 }
 
 // ---- near-far refill: after a step whose next list came out empty, list
@@ -667,18 +666,38 @@ __device__ __forceinline__ void refill_write_body(const gga_dev_graph &g, const 
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, pred, s_wave);
 }
 
-// The standalone refill is two launches: the count, whose first thread also
-// records the threshold the refill lists under (RTHR) and that this step
-// refills (RSTEP = its step number), then the write, which sums the counts
-// before each tile itself (compact_write_tiles); its first workgroup folds
-// the total and the tiles' minima into the next threshold as the scan body
-// above does.  The write reads only RTHR / RSTEP, which no workgroup of it
-// changes, so the list length it stores midway does not change its course.
-__global__ __launch_bounds__(BLOCK) void k_refill_count(gga_dev_graph g, gga_dev_work w, uint32_t step) {
+// The standalone refill is two launches: its count runs in the compaction's
+// write launch (k_compact_write, whose first thread also records the
+// threshold the refill lists under, RTHR, and that this step refills, RSTEP
+// = its step number), then k_refill_write, which sums the counts before each
+// tile itself (compact_write_tiles); its first workgroup folds the total and
+// the tiles' minima into the next threshold as the scan body above does.
+// That write reads only RTHR / RSTEP, which no workgroup of it changes, so
+// the list length it stores midway does not change its course.
+//
+// In the compaction's write launch: after a flag
+// step the list is empty exactly when no count workgroup marked it LISTED,
+// after a queue step when no push reached it (its length, final since the
+// relax ended); either way every workgroup reads the same words, which no
+// workgroup of this launch changes.  (The refill's counts reuse tiles[],
+// which the compaction reads only when the list is not empty.)  Its first
+// thread records the threshold the refill lists under (RTHR) and that this
+// step refills (RSTEP = its step number).
+__global__ __launch_bounds__(BLOCK) void k_compact_write(gga_dev_graph g, gga_dev_work w, uint32_t step) {
     __shared__ uint32_t s_wave[WAVES], s_min[WAVES];
-    if (!refill_needed(w, step)) return;
+    const uint32_t nq = step_len(w, step), sout = (step + 1u) % 3u;
+    const bool flags = nq != 0 && flag_step(w, nq);
+    if (flags) {
+        if (!w.last || w.ctl[GGA_CTL_LISTED(sout)] != 0u) {
+            const uint32_t it = w.ctl[GGA_CTL_ITER(step % 3u)];
+            compact_write_tiles(list_out(g, w, step), g.n, StampMarked{w.flag, (uint8_t)it}, s_wave);
+            return;
+        }
+    } else if (!w.last || w.ctl[GGA_CTL_TAIL(sout)] != 0u) {
+        return;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        w.ctl[GGA_CTL_RTHR] = sat_add(w.ctl[GGA_CTL_THR((step + 1u) % 3u)], w.delta);
+        w.ctl[GGA_CTL_RTHR] = sat_add(w.ctl[GGA_CTL_THR(sout)], w.delta);
         w.ctl[GGA_CTL_RSTEP] = w.ctl[GGA_CTL_ITER(step % 3u)];
     }
     refill_count_body(g, w, step, s_wave, s_min);
@@ -1085,8 +1104,7 @@ extern "C" int ggak_refill(gga_dev_graph g, gga_dev_work w, uint32_t step, hipSt
     if (!w.last) return 0;
     const uint32_t ntiles = GGA_TILES(g.n);
     const uint32_t grid = ntiles < 2048u ? (ntiles ? ntiles : 1u) : 2048u;
-    k_refill_count<<<grid, BLOCK, 0, s>>>(g, w, step);
-    k_refill_write<<<grid, BLOCK, 0, s>>>(g, w, step);
+    k_refill_write<<<grid, BLOCK, 0, s>>>(g, w, step);   // (its count ran in k_compact_write)
     return launch_status();
 }
 
