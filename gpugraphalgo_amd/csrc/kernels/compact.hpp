@@ -1,10 +1,13 @@
 // compact.hpp -- list building after a flag step.
 //
 // A flag step records the next frontier as a per-vertex predicate (a step
-// stamp, a lane mask); three passes then rebuild the chunk list from it with
-// no shared counter: count the chunk ids each tile of GGA_TILE vertices
-// contributes, scan the tile counts (one workgroup), write.  Lists are in
-// vertex order, so chunks of one vertex stay adjacent.
+// stamp, a lane mask); the chunk list is then rebuilt from it with no shared
+// counter: count the chunk ids each tile of GGA_TILE vertices contributes,
+// scan the tile counts, write.  The cooperative kernels run the three passes
+// between grid barriers (the scan by one workgroup); the step launches run
+// two, the write pass summing the counts before its own tile
+// (compact_write_tiles).  Lists are in vertex order, so chunks of one vertex
+// stay adjacent.
 #pragma once
 #include "../gga_dev.h"
 #include "sync.hpp"
