@@ -92,20 +92,6 @@ __device__ __forceinline__ uint32_t wave_owner(uint32_t incl, uint32_t k) {
     return pos;
 }
 
-// wave_owner when only lanes 0 .. p-1 own slots (p a power of two <= 64,
-// wave-uniform, incl constant from lane p - 1 on): log2(p) shuffles, not 6.
-__device__ __forceinline__ uint32_t wave_owner_p(uint32_t incl, uint32_t k, uint32_t p) {
-    uint32_t pos = 0;
-#pragma unroll
-    for (uint32_t step = 32; step >= 1; step >>= 1) {
-        if (step < p) {   // wave-uniform
-            uint32_t v = __shfl(incl, (int)(pos + step - 1), GGA_WAVE);
-            if (v <= k) pos += step;
-        }
-    }
-    return pos;
-}
-
 // Workgroups are dispatched round-robin over the 8 XCDs; this maps the
 // hardware block id to a logical id so that each XCD (and its L2) works on
 // one contiguous range.  Bijective on [0, nblk).
