@@ -75,14 +75,47 @@ struct ListOut {
 // thread's ids go to consecutive positions.  Against vertices strided by
 // CBLOCK: R-MAT s22 SSSP 2.24 -> 2.18 ms, the 4096² grid's persistent BFS /
 // SSSP 9.5 / 15.4 -> 8.7 / 14.1 ms.
+//
+// A predicate with vec8 reads the marks of a thread's eight vertices with one
+// or two vector loads (mark8: bit i for vertex v + i), and the marked
+// vertices' chunk starts then come from two 16-byte loads of cstart and one
+// word: a thread's vertices start at a multiple of 8, and the arrays are
+// 256-byte aligned device allocations.  Threads whose vertices run past n
+// take them one at a time.
+static_assert(CITEMS == 8, "the vector path reads eight vertices per thread");
 template <class Marked>
 __device__ __forceinline__ uint32_t tile_items(const ListOut &o, uint32_t n, uint32_t t, const Marked &marked,
                                                uint32_t *cs, uint32_t *nch) {
     const uint64_t v0 = (uint64_t)t * GGA_TILE;
+    const uint64_t vb = v0 + (uint64_t)threadIdx.x * CITEMS;
     uint32_t mine = 0;
+    if constexpr (Marked::vec8) {
+        if (vb + CITEMS <= n) {
+            const uint32_t bits = marked.mark8(vb);
+#pragma unroll
+            for (uint32_t i = 0; i < CITEMS; i++) {
+                cs[i] = 0;
+                nch[i] = 0;
+            }
+            if (bits) {
+                const uint4 a = *reinterpret_cast<const uint4 *>(o.cstart + vb);
+                const uint4 b = *reinterpret_cast<const uint4 *>(o.cstart + vb + 4);
+                const uint32_t c[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, o.cstart[vb + 8]};
+#pragma unroll
+                for (uint32_t i = 0; i < CITEMS; i++)
+                    if ((bits >> i) & 1u) {
+                        cs[i] = c[i];
+                        nch[i] = c[i + 1] - c[i];
+                    }
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < CITEMS; i++) mine += nch[i];
+            return mine;
+        }
+    }
 #pragma unroll
     for (uint32_t i = 0; i < CITEMS; i++) {
-        const uint64_t v = v0 + (uint64_t)threadIdx.x * CITEMS + i;
+        const uint64_t v = vb + i;
         cs[i] = 0;
         nch[i] = 0;
         if (v < n && marked((uint32_t)v)) {

@@ -103,7 +103,15 @@ __device__ __forceinline__ uint32_t ms_len(const gga_dev_ms &ms, uint32_t step) 
 
 struct MaskMarked {
     const uint32_t *mask;
+    static constexpr bool vec8 = true;
     __device__ __forceinline__ bool operator()(uint32_t v) const { return mask[v] != 0u; }
+    // bit i: vertex v + i has a non-empty mask (v a multiple of 8)
+    __device__ __forceinline__ uint32_t mark8(uint64_t v) const {
+        const uint4 a = *reinterpret_cast<const uint4 *>(mask + v);
+        const uint4 b = *reinterpret_cast<const uint4 *>(mask + v + 4);
+        return (a.x ? 1u : 0u) | (a.y ? 2u : 0u) | (a.z ? 4u : 0u) | (a.w ? 8u : 0u) |
+               (b.x ? 16u : 0u) | (b.y ? 32u : 0u) | (b.z ? 64u : 0u) | (b.w ? 128u : 0u);
+    }
 };
 
 __device__ __forceinline__ ListOut ms_list_out(const gga_dev_graph &g, const gga_dev_ms &ms, uint32_t step) {

@@ -51,6 +51,7 @@ __device__ __forceinline__ uint64_t lane_mask(uint32_t k, uint32_t j) {
 
 struct FrontMarked {
     const uint64_t *front;
+    static constexpr bool vec8 = false;
     __device__ __forceinline__ bool operator()(uint32_t v) const {
         const uint64_t *f = front + (uint64_t)v * WORDS;
         return (f[0] | f[1] | f[2] | f[3]) != 0ull;

@@ -477,7 +477,19 @@ __device__ __forceinline__ void worklist_step(const gga_dev_graph &g, const gga_
 struct StampMarked {
     const uint8_t *flag;
     uint8_t it8;
+    static constexpr bool vec8 = true;
     __device__ __forceinline__ bool operator()(uint32_t v) const { return flag[v] == it8; }
+    // bit i: vertex v + i holds the stamp (v a multiple of 8)
+    __device__ __forceinline__ uint32_t mark8(uint64_t v) const {
+        const uint2 x = *reinterpret_cast<const uint2 *>(flag + v);
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            m |= ((x.x >> (8u * i)) & 0xFFu) == it8 ? 1u << i : 0u;
+            m |= ((x.y >> (8u * i)) & 0xFFu) == it8 ? 1u << (4u + i) : 0u;
+        }
+        return m;
+    }
 };
 
 // The single list of the plain layout for step `step`.
@@ -574,6 +586,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_de
 struct DeferredUnder {
     const uint32_t *dist, *last;
     uint32_t t;
+    static constexpr bool vec8 = false;   // (dist may be a caller's buffer of any alignment)
     __device__ __forceinline__ bool operator()(uint32_t v) const {
         const uint32_t d = dist[v];
         return d < last[v] && d < t;
