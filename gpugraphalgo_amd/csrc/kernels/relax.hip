@@ -586,10 +586,30 @@ __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_de
 struct DeferredUnder {
     const uint32_t *dist, *last;
     uint32_t t;
-    static constexpr bool vec8 = false;   // (dist may be a caller's buffer of any alignment)
+    static constexpr bool vec8 = true;
     __device__ __forceinline__ bool operator()(uint32_t v) const {
         const uint32_t d = dist[v];
         return d < last[v] && d < t;
+    }
+    // bit i: vertex v + i is deferred under t (v a multiple of 8); dist may
+    // be a caller's buffer (the allocation-free query), so 16-byte loads
+    // only when it is 16-byte aligned (last always is)
+    __device__ __forceinline__ uint32_t mark8(uint64_t v) const {
+        uint32_t m = 0;
+        if ((reinterpret_cast<uintptr_t>(dist) & 15u) == 0) {
+            const uint4 d0 = *reinterpret_cast<const uint4 *>(dist + v);
+            const uint4 d1 = *reinterpret_cast<const uint4 *>(dist + v + 4);
+            const uint4 l0 = *reinterpret_cast<const uint4 *>(last + v);
+            const uint4 l1 = *reinterpret_cast<const uint4 *>(last + v + 4);
+            const uint32_t d[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+            const uint32_t s[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++) m |= (d[i] < s[i] && d[i] < t) ? 1u << i : 0u;
+        } else {
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++) m |= (*this)((uint32_t)(v + i)) ? 1u << i : 0u;
+        }
+        return m;
     }
 };
 
