@@ -264,6 +264,42 @@ def test_near_far_flag_steps_past_the_byte_wrap(gpu):
 DENSE = {"auto": 0, "flags": 1, "queue": 0xFFFFFFFF}
 
 
+def test_list_rebuild_vector_path_edges(gpu):
+    """The compaction and refill passes read each thread's eight vertices
+    with vector loads when all eight exist, one at a time otherwise.  Graphs
+    whose vertex count is not a multiple of 8 (15, 221, 1 073, 2 047) take
+    both paths in the last tile; every step is a flag step (dense_min = 1)
+    so every list goes through them, with and without near-far buckets
+    (whose refill reads dist and last), single-source and 32-lane engine,
+    and must equal the oracle."""
+    graphs = [gga.gen_grid(3, 5, 0.0, seed=41, wlo=1, whi=9),
+              gga.gen_grid(13, 17, 0.02, seed=42, wlo=1, whi=1000),
+              gga.gen_grid(37, 29, 0.01, seed=43, wlo=1, whi=100),
+              gga.gen_grid(23, 89, 0.05, seed=44, wlo=1, whi=1000)]
+    for g in graphs:
+        assert g.n % 8 != 0
+        rng = np.random.default_rng(g.n)
+        srcs = [0, g.n - 1, int(rng.integers(0, g.n))]
+        with gga.DeviceGraph(g) as dg:
+            dg.split()
+            for s in srcs:
+                want_b = oracle.bfs_queue(g.row_ptr, g.dst, s)
+                want_s = oracle.dijkstra(g.row_ptr, g.dst, g.w, s)
+                for mode in ("worklist", "persistent"):
+                    announce(f"vector path n={g.n} src={s} {mode}")
+                    d, _ = dg.bfs(s, mode=mode, dense_min=1)
+                    np.testing.assert_array_equal(d, want_b, err_msg=f"bfs n={g.n} {s} {mode}")
+                    for delta in (0xFFFFFFFF, 1, 50):
+                        d, _ = dg.sssp(s, mode=mode, dense_min=1, delta=delta)
+                        np.testing.assert_array_equal(d, want_s, err_msg=f"sssp n={g.n} {s} {mode} {delta}")
+            lanes = [int(x) for x in rng.choice(g.n, 32, replace=g.n < 32)]
+            announce(f"vector path engine n={g.n}")
+            out, _ = dg.multi_sp(lanes, bfs_lanes=0xFFFF, mode="worklist", dense_min=1)
+            for i, s in enumerate(lanes):
+                want = oracle.bfs_queue(g.row_ptr, g.dst, s) if i < 16 else oracle.dijkstra(g.row_ptr, g.dst, g.w, s)
+                np.testing.assert_array_equal(out[i], want, err_msg=f"engine lane {i} n={g.n} src={s}")
+
+
 def test_mesh_wavefront_continuation_equals_oracle(gpu):
     """On meshes a wave follows the vertices it lowered past its batch within
     the step (single source and the 32-lane engine).  The extra rounds only
