@@ -201,7 +201,7 @@ def batched_leg(gga, np, quick=False, coop=True):
     cand = np.flatnonzero(g.degrees() > 0)
     bsrc = rng.choice(cand, 256, replace=False).astype(np.uint32)
     ssrc = rng.choice(cand, 32, replace=False).astype(np.uint32)
-    out = {"workload": "R-MAT s20 ef16 seed 13, 256 BFS + 32 SSSP sources (config 4)", "n": g.n, "m": g.m}
+    out = {"workload": "R-MAT s20 ef16 seed 13, 256 BFS + 32 SSSP sources", "n": g.n, "m": g.m}
     with gga.DeviceGraph(g) as dg:
         tr = dg.tune(int(ssrc[0]), bfs=False, layout=not quick, multi=True)
         out["tuned"] = {"split_threshold": tr["mda"], "seed_threshold": tr["seed_mda"], "grid": tr["grid"],
@@ -257,7 +257,7 @@ def grid_leg(gga, np, quick=False, coop=True):
     leaves out the persistent (cooperative) launch mode."""
     side = 1024 if quick else 4096
     g = gga.gen_grid(side, side, 0.005, seed=11, wlo=1, whi=1000)
-    out = {"workload": f"{side}x{side} grid + 0.5% shortcuts, weights [1,1000], source 0 (config 3)",
+    out = {"workload": f"{side}x{side} grid + 0.5% shortcuts, weights [1,1000], source 0",
            "n": g.n, "m": g.m}
     with gga.DeviceGraph(g) as dg:
         dg.split()
@@ -344,8 +344,8 @@ def serving_leg(gga, np, full=True, quick=False):
     degs = [np.flatnonzero(g.degrees() > 0).astype(np.uint32) for _, g in gs]
     nq = 120 if quick else 400
     nplain = 40 if quick else 80
-    out = {"workload": ("config 5 resident set: R-MAT s16-s23 + grids 512^2, 1024^2, 2048^2, 4096^2" if full else
-                        "reduced config 5 set: R-MAT s16-s20 + grids 512^2, 1024^2") +
+    out = {"workload": ("resident set: R-MAT s16-s23 + grids 512^2, 1024^2, 2048^2, 4096^2" if full else
+                        "reduced resident set: R-MAT s16-s20 + grids 512^2, 1024^2") +
                        ", BFS/SSSP 50/50, uniform graph and source",
            "graphs": [n for n, _ in gs], "resident_gb": round(sum(d.device_bytes for d in dgs) / 1e9, 3)}
     with Server(hbm_budget=16 << 30, steps_per_round=12, use_graph=True) as srv:
@@ -408,14 +408,14 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="profiling run: one T, no CPU leg")
-    ap.add_argument("--no-grid", action="store_true", help="skip the grid leg (config 3)")
-    ap.add_argument("--no-batched", action="store_true", help="skip the multi-source leg (config 4)")
-    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg (config 5)")
+    ap.add_argument("--no-grid", action="store_true", help="skip the grid leg (4096^2 grid + shortcuts)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the multi-source leg (R-MAT s20, 256 BFS + 32 SSSP)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg (12 resident graphs)")
     ap.add_argument("--serve-reduced", action="store_true",
-                    help="serving leg on a reduced config 5 set (R-MAT s16-s20, grids 512^2 and 1024^2) "
+                    help="serving leg on a reduced resident set (R-MAT s16-s20, grids 512^2 and 1024^2) "
                          "instead of the full one (R-MAT s16-s23, grids 512^2 to 4096^2)")
     ap.add_argument("--leg", choices=("grid", "batched", "serve", "engine"),
-                    help="profiling run: only this leg (configs 3, 4, 5), its JSON object on stdout")
+                    help="profiling run: only this leg (grid, batched, serving or engine), its JSON object on stdout")
     ap.add_argument("--no-coop", action="store_true",
                     help="with --leg: leave out cooperative launches (rocprofv3 can lose its output at "
                          "process exit after them)")
@@ -434,7 +434,7 @@ def main() -> int:
         print(json.dumps({"calibrate_copy_bytes": 4 * n, "copies": 3}))
         return 0
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+        print(json.dumps({"error": "this library runs on one GPU; multi-GPU sharding is a design "
                                    "note only (queries shard with no exchange)", "n_gpus": args.gpus}))
         return 2
 
@@ -480,15 +480,15 @@ def main() -> int:
 
     batched = serving = grid = None
     if not (args.no_grid or args.quick):
-        log("grid leg (config 3)")
+        log("grid leg (4096^2 grid + shortcuts)")
         grid = grid_leg(gga, np)
         log(f"grid: {json.dumps(grid)}")
     if not (args.no_batched or args.quick):
-        log("batched multi-source leg (config 4)")
+        log("batched multi-source leg (R-MAT s20)")
         batched = batched_leg(gga, np)
         log(f"batched: {json.dumps(batched)}")
     if not (args.no_serve or args.quick):
-        log("serving leg (config 5%s)" % (", reduced set" if args.serve_reduced else ""))
+        log("serving leg (%s)" % ("reduced resident set" if args.serve_reduced else "12 resident graphs"))
         serving = serving_leg(gga, np, full=not args.serve_reduced)
         if not args.serve_reduced:
             serving["mesh_engine"] = engine_leg(gga, np)
@@ -529,7 +529,7 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic R-MAT (built-in seeded generator), weights uniform u32 [1,100]",
         "config": {
-            "workload": "node-split SSSP, R-MAT scale 22 edge factor 16 (config 2)",
+            "workload": "node-split SSSP, R-MAT scale 22 edge factor 16",
             "scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed,
             "n": g.n, "m": g.m, "source": src, "maxdeg": maxdeg, "m_reach": r["m_reach"],
             "split_threshold": best, "chunks": r["split"]["ns_nnode"], "mode": mode_name,

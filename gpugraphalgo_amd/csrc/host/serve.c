@@ -238,7 +238,7 @@ static int make_room(gga_server *s, uint64_t need)
 /* Layout of a new engine: the options', else the graph's tuned one, else
  * lane-major (also gga_tune_layout's default on mesh-like graphs).  A served engine mostly runs a few lanes, and lane-major rows
  * keep each lane's gathers near one another, where a 128-byte D[v] line
- * carries 32 lanes for the few in use: on the config-5 set, lane-major
+ * carries 32 lanes for the few in use: on the 12-graph serving set, lane-major
  * engines took the p99 at 240 / 330 / 395 q/s offered from 313 / 688 /
  * 736 ms (vertex-major) to 202 / 357 / 492 ms, while 400 queries at once
  * (full 32-lane batches, where vertex-major lines are shared) dropped from

@@ -173,7 +173,7 @@ __device__ __forceinline__ void compact_scan(const ListOut &o, uint32_t n, uint3
 // per round: in R-MAT the hubs sit at the lowest ids, so one tile -- and
 // within it the first wave's vertices -- holds most of them, and written by
 // their own wave they made that wave the whole pass's long pole (62 of the
-// 154 us of a config-2 query's list writes, in one pass).  All threads of
+// 154 us of an R-MAT s22 SSSP query's list writes, in one pass).  All threads of
 // the workgroup must call this (it synchronises).
 constexpr uint32_t CBIG = 256;   // long runs staged per round
 __device__ __forceinline__ void write_items(const ListOut &o, const uint32_t *cs, const uint32_t *nch,
