@@ -586,10 +586,13 @@ __global__ __launch_bounds__(BLOCK) void k_compact_count(gga_dev_graph g, gga_de
 struct DeferredUnder {
     const uint32_t *dist, *last;
     uint32_t t;
+    uint32_t *mn = nullptr;   // when set: the smallest deferred distance seen (under t or not)
     static constexpr bool vec8 = true;
     __device__ __forceinline__ bool operator()(uint32_t v) const {
         const uint32_t d = dist[v];
-        return d < last[v] && d < t;
+        const bool deferred = d < last[v];
+        if (mn && deferred && d < *mn) *mn = d;
+        return deferred && d < t;
     }
     // bit i: vertex v + i is deferred under t (v a multiple of 8); dist may
     // be a caller's buffer (the allocation-free query), so 16-byte loads
@@ -603,8 +606,13 @@ struct DeferredUnder {
             const uint4 l1 = *reinterpret_cast<const uint4 *>(last + v + 4);
             const uint32_t d[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
             const uint32_t s[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+            uint32_t lo = 0xFFFFFFFFu;
 #pragma unroll
-            for (uint32_t i = 0; i < 8; i++) m |= (d[i] < s[i] && d[i] < t) ? 1u << i : 0u;
+            for (uint32_t i = 0; i < 8; i++) {
+                m |= (d[i] < s[i] && d[i] < t) ? 1u << i : 0u;
+                lo = d[i] < s[i] && d[i] < lo ? d[i] : lo;
+            }
+            if (mn && lo < *mn) *mn = lo;
         } else {
 #pragma unroll
             for (uint32_t i = 0; i < 8; i++) m |= (*this)((uint32_t)(v + i)) ? 1u << i : 0u;
@@ -626,18 +634,11 @@ __device__ __forceinline__ bool refill_needed(const gga_dev_work &w, uint32_t st
 __device__ __forceinline__ void refill_count_body(const gga_dev_graph &g, const gga_dev_work &w, uint32_t step,
                                                   uint32_t *s_wave, uint32_t *s_min) {
     const ListOut o = list_out(g, w, step);
-    const DeferredUnder pred{w.dist, w.last, sat_add(w.ctl[GGA_CTL_THR((step + 1u) % 3u)], w.delta)};
+    uint32_t m = 0xFFFFFFFFu;   // the thread's smallest deferred distance, gathered by the count
+    const DeferredUnder pred{w.dist, w.last, sat_add(w.ctl[GGA_CTL_THR((step + 1u) % 3u)], w.delta), &m};
     for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) {
+        m = 0xFFFFFFFFu;
         compact_count_tile(o, g.n, t, pred, s_wave);
-        uint32_t m = 0xFFFFFFFFu;
-        const uint64_t v0 = (uint64_t)t * GGA_TILE;
-        for (uint32_t i = 0; i < CITEMS; i++) {
-            const uint64_t v = v0 + (uint64_t)i * BLOCK + threadIdx.x;
-            if (v < g.n) {
-                const uint32_t d = w.dist[v];
-                if (d < w.last[v] && d < m) m = d;
-            }
-        }
 #pragma unroll
         for (uint32_t off = 32; off >= 1; off >>= 1) {
             const uint32_t y = __shfl_xor(m, (int)off, GGA_WAVE);
