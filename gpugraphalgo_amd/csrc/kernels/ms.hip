@@ -645,10 +645,11 @@ __global__ __launch_bounds__(BLOCK) void k_ms_tail(gga_dev_graph g, gga_dev_ms m
     listed = wave_or(listed);
     if (lane_id() == 0 && listed) atomicOr(&s_listed, listed);
     __syncthreads();
-    if (threadIdx.x < LANES) ms.lpart[(uint64_t)blockIdx.x * LANES + threadIdx.x] = s_min[threadIdx.x];
-    __threadfence();   // this workgroup's lpart row before its arrival
-    __syncthreads();
     if (threadIdx.x == 0) {
+        // the arriving thread writes the workgroup's lpart row itself, so
+        // the release of its arrival covers the row (rather than an
+        // agent-scope fence in every thread of every workgroup)
+        for (uint32_t s = 0; s < LANES; s++) ms.lpart[(uint64_t)blockIdx.x * LANES + s] = s_min[s];
         if (s_listed)
             __hip_atomic_fetch_or(&ms.ctl[GGA_CTL_RLISTED], s_listed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = __hip_atomic_fetch_add(&ms.ctl[GGA_CTL_TDONE], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
