@@ -144,13 +144,25 @@ __device__ __forceinline__ void pull_level(const gga_dev_graph &t, const gga_dev
 }
 
 // Level values of the set bits of `bits` (word j of vertex v), uniform loop.
+// The wave walks only the bit positions set in some lane's word (their OR,
+// a butterfly over the wave, kept scalar), not all 64: one coalesced store
+// per position for the lanes that hold it.
 __device__ __forceinline__ void write_levels(const gga_dev_msbfs &b, uint32_t n, uint32_t v, bool valid,
                                              uint32_t j, uint64_t bits, uint32_t value) {
-    if (!__ballot(valid && bits)) return;
-    for (uint32_t t = 0; t < 64u; t++) {
+    const uint64_t mine = valid ? bits : 0ull;
+    uint32_t lo = (uint32_t)mine, hi = (uint32_t)(mine >> 32);
+#pragma unroll
+    for (uint32_t o = 32; o >= 1; o >>= 1) {
+        lo |= __shfl_xor(lo, (int)o, GGA_WAVE);
+        hi |= __shfl_xor(hi, (int)o, GGA_WAVE);
+    }
+    uint64_t any = (uint64_t)__builtin_amdgcn_readfirstlane(hi) << 32 | __builtin_amdgcn_readfirstlane(lo);
+    while (any) {
+        const uint32_t t = (uint32_t)__builtin_ctzll(any);
+        any &= any - 1u;
         const uint32_t lane = j * 64u + t;
         if (lane >= b.k) break;
-        if (valid && ((bits >> t) & 1ull)) b.lev[(uint64_t)lane * n + v] = value;
+        if ((mine >> t) & 1ull) b.lev[(uint64_t)lane * n + v] = value;
     }
 }
 
