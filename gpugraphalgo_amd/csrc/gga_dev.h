@@ -114,14 +114,17 @@ typedef struct gga_ms_outputs { uint32_t *p[GGA_MS_LANES]; } gga_ms_outputs;
 
 /* Multi-source BFS with bitmask lanes (kernels/msbfs.hip): up to 256 BFS share
  * every level; a vertex holds 4 x 64-bit words per bitmap.
- *   seen / front / next [n * 4]  visited lanes, lanes discovered by the last
- *                                level, lanes reached in the current level
+ *   seen / next [n * 4]  visited lanes, lanes reached in the current level
+ *   front[(nsnap + 1) * n * 4]  lanes discovered by a level: slot s < nsnap
+ *                holds level s (kept to the end), slot nsnap every later level
  *   lev[k * n]   levels, lane-major (row i = source i), written once per
- *                (vertex, lane): at discovery or, unreached, inf at the end
+ *                (vertex, lane): at discovery from level nsnap on, else at the
+ *                end from the front slots (inf where never reached)
  *   q            push list (forward chunk ids), tiles: its compaction
  *   part[grid * 4]  per-workgroup level partials (frontier vertices,
  *                frontier out-edges, newly reached pairs, their out-edges) */
 #define GGA_MSBFS_MAX 256u
+#define GGA_MSBFS_SNAP_MAX 16u   /* front slots kept to the end, at most */
 typedef struct gga_dev_msbfs {
     uint64_t *seen;
     uint64_t *front;
@@ -135,6 +138,7 @@ typedef struct gga_dev_msbfs {
     uint32_t inf;
     uint32_t qcap;
     uint32_t alpha;      /* pull (bottom-up) level when frontier out-edges * alpha > m; 0: never */
+    uint32_t nsnap;      /* levels whose front slots are kept (0 .. GGA_MSBFS_SNAP_MAX) */
 } gga_dev_msbfs;
 /* multi-source BFS control words (within the ctl line of CHANGED / STEPS) */
 #define GGA_CTL_DIR       (7u * GGA_CTL_LINE + 16u)   /* 0: next level pushes, 1: pulls */

@@ -5,13 +5,18 @@
  *
  * The graph keeps, once built: the transposed CSR (in-edges) with its own
  * chunk table (threshold min(forward threshold, 32)), the per-vertex bitmaps
- * and the level buffer of the largest k seen.
+ * and the level buffer of the largest k seen.  The fronts of the first
+ * MSBFS_SNAP levels are kept (32 B per vertex each) so that the level rows
+ * are written once at the end; meshes, whose BFS run thousands of levels,
+ * keep none.
  */
 #define _GNU_SOURCE
 #include <stdlib.h>
 #include <string.h>
 
 #include "gga_host.h"
+
+#define MSBFS_SNAP 8u
 
 typedef struct msbfs_state {
     int device;
@@ -24,7 +29,7 @@ typedef struct msbfs_state {
     uint64_t *seen, *front, *next, *part;
     uint32_t *lev, *q, *tiles, *ctl, *d_src;
     uint64_t lev_words;
-    uint32_t qcap, grid;
+    uint32_t qcap, grid, nsnap;
     uint32_t *h_pin;
     hipEvent_t ev[2];
 } msbfs_state;
@@ -158,7 +163,8 @@ static int get_state(gga_graph *gr, const gga_graph_view_t *v, uint32_t k, msbfs
         if ((rc = build_transpose(s, v))) goto fail;
         const uint64_t words = 4ull * s->n;
         if ((rc = xalloc((void **)&s->seen, 8ull * words))) goto fail;
-        if ((rc = xalloc((void **)&s->front, 8ull * words))) goto fail;
+        s->nsnap = gga_graph_mesh_like(gr) ? 0u : MSBFS_SNAP;
+        if ((rc = xalloc((void **)&s->front, 8ull * words * (s->nsnap + 1u)))) goto fail;
         if ((rc = xalloc((void **)&s->next, 8ull * words))) goto fail;
         s->qcap = v->dg.C;
         if ((rc = xalloc((void **)&s->q, 4ull * s->qcap))) goto fail;
@@ -228,6 +234,7 @@ int gga_multi_bfs(gga_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *out,
     b.inf = oo.inf ? oo.inf : GGA_INF;
     b.qcap = s->qcap;
     b.alpha = oo.pull_alpha == 0 ? 14u : (oo.pull_alpha == 0xFFFFFFFFu ? 0u : oo.pull_alpha);
+    b.nsnap = s->nsnap;
     gga_dev_graph t;
     memset(&t, 0, sizeof t);
     t.row = s->trow;

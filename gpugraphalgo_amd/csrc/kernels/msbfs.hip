@@ -15,9 +15,14 @@
 //     owner still misses lanes ORs the front bits of its in-neighbours until
 //     the missing lanes are all found (direction-optimising BFS, Beamer et al.
 //     2012, on bitmaps as in multi-source BFS, Then et al. 2014).
-//   finalize: new = next & ~seen; seen |= new; front = new; levels written for
-//     the new bits (the level loop is uniform over bit positions so that each
-//     store instruction writes consecutive vertices of one lane row).
+//   finalize: new = next & ~seen; seen |= new; front = new.
+// The fronts of the first nsnap levels stay in slots of their own (front
+// slot s holds the lanes discovered at level s) and one pass at the end
+// writes every level row from them and from seen, each output line once:
+// level rows are lane-major, and writing each level's new bits as they
+// appear dirtied every line of them again at every wide level (256 BFS on
+// R-MAT s20: about 1.1 ms of 3.3 in the writes).  Levels from nsnap on
+// share one slot and write their values at discovery.
 // One cooperative launch runs all levels; grid barriers separate the phases.
 #include "../gga_kernels.h"
 #include "compact.hpp"
@@ -70,7 +75,13 @@ __device__ __forceinline__ ListOut bfs_list(const gga_dev_graph &g, const gga_de
 }
 
 // ------------------------------------------------------------- push level
-__device__ __forceinline__ void push_level(const gga_dev_graph &g, const gga_dev_msbfs &b, uint32_t nq) {
+// front slot of level L: its own below nsnap, the shared one from there on
+__device__ __forceinline__ uint64_t *front_at(const gga_dev_msbfs &b, uint32_t n, uint32_t level) {
+    return b.front + (uint64_t)(level < b.nsnap ? level : b.nsnap) * n * WORDS;
+}
+
+__device__ __forceinline__ void push_level(const gga_dev_graph &g, const gga_dev_msbfs &b, const uint64_t *front,
+                                           uint32_t nq) {
     const uint32_t waves = gridDim.x * WAVES;
     const uint32_t wid = blockIdx.x * WAVES + threadIdx.x / GGA_WAVE;
     uint32_t bsz = nq / (8u * waves);   // ~8 batches per wave (16 measured no faster here)
@@ -103,7 +114,7 @@ __device__ __forceinline__ void push_level(const gga_dev_graph &g, const gga_dev
                 const uint32_t e = wave_bcast(eb, o) + (kk - wave_bcast(excl, o));
                 const uint32_t uu = wave_bcast(u, o);
                 v[i] = stream_load(&g.col[e]);
-                f[i] = b.front[(uint64_t)uu * WORDS + j];
+                f[i] = front[(uint64_t)uu * WORDS + j];
             }
 #pragma unroll
             for (uint32_t i = 0; i < UNROLL; i++) sn[i] = b.seen[(uint64_t)v[i] * WORDS + j];
@@ -118,7 +129,7 @@ __device__ __forceinline__ void push_level(const gga_dev_graph &g, const gga_dev
 
 // ------------------------------------------------------------- pull level
 // t: transposed graph with its own chunk table (in-edges of each vertex).
-__device__ __forceinline__ void pull_level(const gga_dev_graph &t, const gga_dev_msbfs &b) {
+__device__ __forceinline__ void pull_level(const gga_dev_graph &t, const gga_dev_msbfs &b, const uint64_t *front) {
     const uint32_t l = lane_id(), grp = l / WORDS, j = l % WORDS;
     const uint64_t lm = lane_mask(b.k, j);
     const uint64_t gstride = (uint64_t)gridDim.x * WAVES * GROUPS;
@@ -135,9 +146,9 @@ __device__ __forceinline__ void pull_level(const gga_dev_graph &t, const gga_dev
 #pragma unroll
             for (uint32_t i = 0; i < UNROLL; i++) u[i] = t.col[e + i];
 #pragma unroll
-            for (uint32_t i = 0; i < UNROLL; i++) acc |= b.front[(uint64_t)u[i] * WORDS + j];
+            for (uint32_t i = 0; i < UNROLL; i++) acc |= front[(uint64_t)u[i] * WORDS + j];
         }
-        for (; e < ee && (acc & miss) != miss; e++) acc |= b.front[(uint64_t)t.col[e] * WORDS + j];
+        for (; e < ee && (acc & miss) != miss; e++) acc |= front[(uint64_t)t.col[e] * WORDS + j];
         acc &= miss;
         if (acc) atomic_or_u64(&b.next[(uint64_t)v * WORDS + j], acc);
     }
@@ -170,9 +181,11 @@ struct LevelSums {
     uint64_t nf = 0, mf = 0, reached = 0, mreach = 0;
 };
 
-// finalize one tile; returns the tile's push-list chunk count in tiles[t]
-__device__ __forceinline__ void finalize_tile(const gga_dev_graph &g, const gga_dev_msbfs &b, uint32_t t,
-                                              uint32_t level, LevelSums &ls, uint32_t *s_wave) {
+// finalize one tile into the front slot of level + 1; returns the tile's
+// push-list chunk count in tiles[t]
+__device__ __forceinline__ void finalize_tile(const gga_dev_graph &g, const gga_dev_msbfs &b, uint64_t *nfront,
+                                              uint32_t t, uint32_t level, LevelSums &ls, uint32_t *s_wave) {
+    const bool direct = level + 1u >= b.nsnap;
     const uint64_t v0 = (uint64_t)t * GGA_TILE;
     uint32_t chunks = 0;
     for (uint32_t i = 0; i < CITEMS; i++) {
@@ -190,11 +203,11 @@ __device__ __forceinline__ void finalize_tile(const gga_dev_graph &g, const gga_
                 nw = nx & ~sn;
                 if (nx) b.next[(uint64_t)v * WORDS + j] = 0ull;
                 if (nw) b.seen[(uint64_t)v * WORDS + j] = sn | nw;
-                b.front[(uint64_t)v * WORDS + j] = nw;
+                nfront[(uint64_t)v * WORDS + j] = nw;
             }
             any |= nw;
             pc += (uint32_t)__popcll(nw);
-            write_levels(b, g.n, v, valid, j, nw, level + 1u);
+            if (direct) write_levels(b, g.n, v, valid, j, nw, level + 1u);
         }
         if (valid && any) {
             const uint32_t deg = g.row[v + 1] - g.row[v];
@@ -210,17 +223,38 @@ __device__ __forceinline__ void finalize_tile(const gga_dev_graph &g, const gga_
     if (threadIdx.x == 0) b.tiles[t] = tot;
 }
 
-// inf for every (vertex, lane) never reached
-__device__ __forceinline__ void fill_unreached(const gga_dev_graph &g, const gga_dev_msbfs &b) {
+// The level rows, each (vertex, lane) written once: the level of its front
+// slot (slots 0 .. nslots-1; a lane's bit is in at most one, and the slot
+// index is gathered bit-sliced: s0 holds the bits of odd slots, s1 of slots
+// with bit 1 set, ...), inf where the lane never reached the vertex, nothing
+// where it did at a level from nsnap on (written at discovery).
+__device__ __forceinline__ void write_rows(const gga_dev_graph &g, const gga_dev_msbfs &b, uint32_t nslots) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t nn = ((uint64_t)g.n + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves loop
+    const uint64_t slot_words = (uint64_t)g.n * WORDS;
     for (uint64_t vv = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; vv < nn; vv += stride) {
         const bool valid = vv < g.n;
         const uint32_t v = valid ? (uint32_t)vv : 0u;
-#pragma unroll
-        for (uint32_t j = 0; j < WORDS; j++) {
-            const uint64_t miss = valid ? (~b.seen[(uint64_t)v * WORDS + j] & lane_mask(b.k, j)) : 0ull;
-            write_levels(b, g.n, v, valid, j, miss, b.inf);
+        for (uint32_t j = 0; j < WORDS && j * 64u < b.k; j++) {
+            uint64_t found = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, miss = 0;
+            if (valid) {
+                for (uint32_t s = 0; s < nslots; s++) {
+                    const uint64_t w = b.front[s * slot_words + (uint64_t)v * WORDS + j];
+                    found |= w;
+                    s0 |= (s & 1u) ? w : 0ull;
+                    s1 |= (s & 2u) ? w : 0ull;
+                    s2 |= (s & 4u) ? w : 0ull;
+                    s3 |= (s & 8u) ? w : 0ull;
+                }
+                miss = ~b.seen[(uint64_t)v * WORDS + j] & lane_mask(b.k, j);
+            }
+            const uint64_t put = found | miss;
+            for (uint32_t t = 0; t < 64u && j * 64u + t < b.k; t++) {
+                if (!((put >> t) & 1ull)) continue;
+                const uint32_t lv = (uint32_t)((s0 >> t) & 1ull) | (uint32_t)((s1 >> t) & 1ull) << 1 |
+                                    (uint32_t)((s2 >> t) & 1ull) << 2 | (uint32_t)((s3 >> t) & 1ull) << 3;
+                b.lev[(uint64_t)(j * 64u + t) * g.n + v] = ((found >> t) & 1ull) ? lv : b.inf;
+            }
         }
     }
 }
@@ -260,7 +294,7 @@ __global__ __launch_bounds__(BLOCK) void k_msbfs_seed(gga_dev_graph g, gga_dev_m
     const uint64_t bit = 1ull << (i % 64u);
     atomic_or_u64(&b.seen[(uint64_t)x * WORDS + i / 64u], bit);
     atomic_or_u64(&b.front[(uint64_t)x * WORDS + i / 64u], bit);
-    b.lev[(uint64_t)i * g.n + x] = 0u;
+    if (b.nsnap == 0) b.lev[(uint64_t)i * g.n + x] = 0u;  // else from front slot 0 at the end
     uint64_t *tot = reinterpret_cast<uint64_t *>(b.ctl + GGA_CTL_CNT);
     atomic_add_u64(&tot[0], 1ull);
     atomic_add_u64(&tot[1], (uint64_t)(g.row[x + 1] - g.row[x]));
@@ -293,7 +327,7 @@ __global__ __launch_bounds__(BLOCK) void k_msbfs(gga_dev_graph g, gga_dev_graph 
         gen = __hip_atomic_load(&b.ctl[GGA_CTL_BARGEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const ListOut o = bfs_list(g, b);
     const uint32_t ntiles = GGA_TILES(g.n);
-    uint32_t level = 0;
+    uint32_t level = 0, filled = 1;   // front slots written: 0 (the sources), then one per level
     for (; level < max_levels; level++) {
         if (threadIdx.x == 0) {
             s_state[0] = b.ctl[GGA_CTL_DIR];
@@ -304,11 +338,15 @@ __global__ __launch_bounds__(BLOCK) void k_msbfs(gga_dev_graph g, gga_dev_graph 
         const uint32_t dir = s_state[0], nq = s_state[1] < b.qcap ? s_state[1] : b.qcap, err = s_state[2];
         __syncthreads();
         if (err) break;
-        if (dir) pull_level(t, b);
-        else push_level(g, b, nq);
+        const uint64_t *front = front_at(b, g.n, level);
+        uint64_t *nfront = front_at(b, g.n, level + 1u);
+        if (dir) pull_level(t, b, front);
+        else push_level(g, b, front, nq);
         grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
         LevelSums ls;
-        for (uint32_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) finalize_tile(g, b, tt, level, ls, s_wave);
+        for (uint32_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x)
+            finalize_tile(g, b, nfront, tt, level, ls, s_wave);
+        filled = level + 2u;
         const uint64_t nf = block_sum(ls.nf, s_red), mf = block_sum(ls.mf, s_red);
         const uint64_t rs = block_sum(ls.reached, s_red), mr = block_sum(ls.mreach, s_red);
         if (threadIdx.x == 0) {
@@ -346,11 +384,11 @@ __global__ __launch_bounds__(BLOCK) void k_msbfs(gga_dev_graph g, gga_dev_graph 
         if (done) break;
         if (!pull) {
             for (uint32_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x)
-                compact_write_tile(o, g.n, tt, FrontMarked{b.front}, s_wave);
+                compact_write_tile(o, g.n, tt, FrontMarked{nfront}, s_wave);
             grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
         }
     }
-    fill_unreached(g, b);
+    write_rows(g, b, filled < b.nsnap ? filled : b.nsnap);
 }
 
 // ------------------------------------------------------ transposed graph
@@ -409,7 +447,7 @@ extern "C" uint32_t ggak_msbfs_grid(int device, uint32_t per_cu) {
 }
 
 extern "C" int ggak_msbfs_init(gga_dev_graph g, gga_dev_msbfs b, const uint32_t *d_src, hipStream_t s) {
-    if (b.k == 0 || b.k > GGA_MSBFS_MAX) return (int)hipErrorInvalidValue;
+    if (b.k == 0 || b.k > GGA_MSBFS_MAX || b.nsnap > GGA_MSBFS_SNAP_MAX) return (int)hipErrorInvalidValue;
     k_msbfs_clear<<<grid_for((uint64_t)g.n * WORDS, 8192), BLOCK, 0, s>>>(g, b);
     k_msbfs_seed<<<(b.k + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(g, b, d_src);
     const uint32_t cg = compact_grid(g.n);

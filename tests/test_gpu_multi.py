@@ -243,6 +243,48 @@ def test_multi_bfs_directed_grid_and_isolated(gpu):
             dg.multi_bfs(np.zeros(257, dtype=np.uint32))
 
 
+def lollipop(path, leaves):
+    """A path 0 .. path-1 (both directions) whose vertex 0 is also a hub with
+    `leaves` leaf neighbours: deep levels, and a degree spread that does not
+    read as a mesh."""
+    src, dst = [], []
+    for v in range(path - 1):
+        src += [v, v + 1]
+        dst += [v + 1, v]
+    for i in range(leaves):
+        src += [0, path + i]
+        dst += [path + i, 0]
+    n = path + leaves
+    order = np.argsort(np.array(src), kind="stable")
+    d = np.array(dst, dtype=np.uint32)[order]
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    np.add.at(rp, np.array(src) + 1, 1)
+    return gga.csr_from_arrays(np.cumsum(rp).astype(np.uint32), d, None)
+
+
+def test_multi_bfs_kept_fronts_and_deep_levels(gpu):
+    """Levels below the kept-front count are written at the end from the front
+    slots, later ones at discovery: a 300-level lollipop and a sparse R-MAT
+    (levels past the kept slots, unreached vertices) mix the two in every
+    lane, push-only and pull-eager, against the oracle."""
+    cases = [("lollipop300+60", lollipop(300, 60), 2),
+             ("rmat12 ef2", gga.gen_rmat(12, 2, seed=21, wlo=1, whi=1), 1)]
+    for name, g, mda in cases:
+        with gga.DeviceGraph(g) as dg:
+            dg.split(mda=mda)
+            for k in (200, 64):
+                srcs = nonisolated(g, k, 3 + k)
+                for pa in (0xFFFFFFFF, 1):
+                    announce(f"multi_bfs {name} k={k} pull_alpha={pa}")
+                    out, st = dg.multi_bfs(srcs, pull_alpha=pa)
+                    depth = 0
+                    for i, s in enumerate(srcs):
+                        want = want_for(g, s, True)
+                        np.testing.assert_array_equal(out[i], want, err_msg=f"{name} lane {i} src {s}")
+                        depth = max(depth, int(want[want != INF].max()))
+                    assert st["iterations"] >= depth >= 8   # level 8 is the first written at discovery
+
+
 def test_autotuner_keeps_results_and_seeds_with_histogram(gpu):
     """gga_tune: the histogram heuristic (10 buckets) is the seed, the chosen
     threshold is one of the timed candidates, and every query after tuning
