@@ -172,10 +172,12 @@ static int get_state(gga_graph *gr, const gga_graph_view_t *v, uint32_t k, msbfs
         if ((rc = xalloc((void **)&s->ctl, 4ull * GGA_CTL_WORDS))) goto fail;
         if ((rc = xalloc((void **)&s->d_src, 4ull * GGA_MSBFS_MAX))) goto fail;
         K_TRY(ggak_fill_u32(s->ctl, 0u, GGA_CTL_WORDS, v->stream));
-        /* two workgroups per CU hide more gather latency (256 BFS on R-MAT s20:
-         * 4.8 -> 3.9 ms); meshes run thousands of levels, where the grid
-         * barriers of the larger grid cost more (512^2: 148 -> 156 ms) */
-        s->grid = ggak_msbfs_grid(v->device, gga_graph_mesh_like(gr) ? 1u : 2u);
+        /* more workgroups per CU hide more gather latency (256 BFS on R-MAT
+         * s20: 4.8 ms with one, 3.9 with two, before the kept fronts; then
+         * 2.43 with two, 2.36 with three, 2.44 with four); meshes run
+         * thousands of levels, where the grid barriers of a larger grid cost
+         * more (512^2: 148 ms with one, 156 with two) */
+        s->grid = ggak_msbfs_grid(v->device, gga_graph_mesh_like(gr) ? 1u : 3u);
         if (s->grid == 0) {
             gga_set_error("occupancy query failed");
             rc = GGA_EHIP;
