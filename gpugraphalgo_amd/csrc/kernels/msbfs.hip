@@ -398,11 +398,16 @@ __global__ __launch_bounds__(BLOCK) void k_indeg(const uint32_t *col, uint32_t m
         atomic_add_u32(&indeg[col[e]], 1u);
 }
 
-// one thread per forward chunk; the in-edge order within a vertex is arbitrary
-__global__ __launch_bounds__(BLOCK) void k_transpose_fill(gga_dev_graph g, uint32_t *pos, uint32_t *tcol) {
+// one thread per forward chunk whose owner's out-degree is in [lo, hi) (hi 0:
+// no bound); the
+// in-edge order within one such pass is arbitrary
+__global__ __launch_bounds__(BLOCK) void k_transpose_fill(gga_dev_graph g, uint32_t *pos, uint32_t *tcol, uint32_t lo,
+                                                          uint32_t hi) {
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g.C;
          c += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t u = g.cown[c];
+        const uint32_t d = g.row[u + 1] - g.row[u];
+        if (d < lo || (hi != 0u && d >= hi)) continue;
         for (uint32_t e = g.cbeg[c]; e < g.cbeg[c + 1]; e++) tcol[atomic_add_u32(&pos[g.col[e]], 1u)] = u;
     }
 }
@@ -427,8 +432,16 @@ extern "C" int ggak_indeg(const uint32_t *col, uint32_t m, uint32_t *indeg, hipS
     return launch_status();
 }
 
+// In-edges from sources of higher out-degree first: one pass per power-of-two
+// degree class, from the top.  Hubs are reached by most lanes within the
+// first levels, so a pull level finds a vertex's missing lanes after fewer
+// gathers.
 extern "C" int ggak_transpose_fill(gga_dev_graph g, uint32_t *pos, uint32_t *tcol, hipStream_t s) {
-    k_transpose_fill<<<grid_for(g.C, 8192), BLOCK, 0, s>>>(g, pos, tcol);
+    for (int cls = 31; cls >= 0; cls--) {
+        const uint32_t lo = 1u << cls, hi = cls == 31 ? 0u : 2u << cls;
+        if (lo > g.m && cls > 0) continue;
+        k_transpose_fill<<<grid_for(g.C, 8192), BLOCK, 0, s>>>(g, pos, tcol, lo, hi);
+    }
     return launch_status();
 }
 
