@@ -27,6 +27,9 @@ uint32_t ggak_split_tmp_words(uint32_t n);
 /* cstart[v] = exclusive prefix of max(1, ceil(deg / mda)); cstart[n] = C. */
 int ggak_split_scan(const uint32_t *row, uint32_t n, uint32_t mda, uint32_t *cstart,
                     uint32_t *d_tmp, hipStream_t s);
+/* The same with `empty` (0 or 1) chunks for a degree-0 vertex. */
+int ggak_split_scan_ex(const uint32_t *row, uint32_t n, uint32_t mda, uint32_t empty, uint32_t *cstart,
+                       uint32_t *d_tmp, hipStream_t s);
 /* Chunk descriptors cbeg[C + 1], cown[C]; d_maxchunk[GGA_MAXCHUNK_SLOTS] (zeroed)
  * receives per-slot maxima of the chunk sizes. */
 #define GGA_MAXCHUNK_SLOTS 64u

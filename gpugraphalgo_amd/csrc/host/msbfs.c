@@ -122,7 +122,9 @@ static int build_transpose(msbfs_state *s, const gga_graph_view_t *v)
     K_TRY(ggak_scan_u32(indeg, n, s->trow, tmp, v->stream));
     HIP_TRY(hipMemcpyAsync(pos, s->trow, 4ull * (n + 1ull), hipMemcpyDeviceToDevice, v->stream));
     K_TRY(ggak_transpose_fill(*g, pos, s->tcol, v->stream));
-    K_TRY(ggak_split_scan(s->trow, n, s->tT, s->tcstart, tmp, v->stream));
+    /* vertices without in-edges get no in-chunk: a pull level has nothing to
+     * look at there (R-MAT s20: half the vertices) */
+    K_TRY(ggak_split_scan_ex(s->trow, n, s->tT, 0u, s->tcstart, tmp, v->stream));
     HIP_TRY(hipMemcpyAsync(s->h_pin, s->tcstart + n, 4, hipMemcpyDeviceToHost, v->stream));
     HIP_TRY(hipStreamSynchronize(v->stream));
     s->tC = s->h_pin[0];

@@ -25,9 +25,10 @@ constexpr uint32_t SCAN_THREADS = 256;
 constexpr uint32_t SCAN_ITEMS = 16;
 constexpr uint32_t SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 4096 vertices per workgroup
 
-__device__ __forceinline__ uint32_t nchunks(uint32_t deg, uint32_t mda) {
-    // (deg + mda - 1) / mda without the uint32 overflow; a degree-0 vertex is one node
-    return deg == 0 ? 1u : (deg - 1u) / mda + 1u;
+__device__ __forceinline__ uint32_t nchunks(uint32_t deg, uint32_t mda, uint32_t empty) {
+    // (deg + mda - 1) / mda without the uint32 overflow; a degree-0 vertex is
+    // one node of the split (empty = 1), or has no chunk (empty = 0)
+    return deg == 0 ? empty : (deg - 1u) / mda + 1u;
 }
 
 
@@ -67,14 +68,14 @@ __global__ __launch_bounds__(256) void k_degree_hist(const uint32_t *__restrict_
 
 // Pass 1: per-tile sums of nch(v).
 __global__ __launch_bounds__(SCAN_THREADS) void k_split_reduce(const uint32_t *__restrict__ row,
-                                                               uint32_t n, uint32_t mda,
+                                                               uint32_t n, uint32_t mda, uint32_t empty,
                                                                uint32_t *tile_sum) {
     __shared__ uint32_t s_wave[SCAN_THREADS / GGA_WAVE];
     const uint64_t tile0 = (uint64_t)blockIdx.x * SCAN_TILE;
     uint32_t s = 0;
     for (uint32_t i = 0; i < SCAN_ITEMS; i++) {
         uint64_t v = tile0 + (uint64_t)i * SCAN_THREADS + threadIdx.x;  // coalesced
-        if (v < n) s += nchunks(row[v + 1] - row[v], mda);
+        if (v < n) s += nchunks(row[v + 1] - row[v], mda, empty);
     }
     uint32_t excl;
     uint32_t tot = block_excl_scan(s, excl, s_wave);
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(1024) void k_split_scan_top(uint32_t *tile_sum, uin
 
 // Pass 3: per-thread contiguous 16-vertex runs, block scan, write cstart.
 __global__ __launch_bounds__(SCAN_THREADS) void k_split_scan_down(const uint32_t *__restrict__ row,
-                                                                  uint32_t n, uint32_t mda,
+                                                                  uint32_t n, uint32_t mda, uint32_t empty,
                                                                   const uint32_t *__restrict__ tile_off,
                                                                   uint32_t ntiles,
                                                                   uint32_t *__restrict__ cstart) {
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_split_scan_down(const uint32_t
 #pragma unroll
     for (uint32_t i = 0; i < SCAN_ITEMS; i++) {
         uint64_t v = v0 + i;
-        c[i] = v < n ? nchunks(row[v + 1] - row[v], mda) : 0u;
+        c[i] = v < n ? nchunks(row[v + 1] - row[v], mda, empty) : 0u;
         s += c[i];
     }
     uint32_t excl;
@@ -248,19 +249,24 @@ extern "C" int ggak_degree_hist(const uint32_t *row, uint32_t n, uint32_t maxdeg
 
 extern "C" uint32_t ggak_split_tmp_words(uint32_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
-extern "C" int ggak_split_scan(const uint32_t *row, uint32_t n, uint32_t mda, uint32_t *cstart,
-                               uint32_t *d_tmp, hipStream_t s) {
-    if (mda == 0) return (int)hipErrorInvalidValue;
+extern "C" int ggak_split_scan_ex(const uint32_t *row, uint32_t n, uint32_t mda, uint32_t empty,
+                                  uint32_t *cstart, uint32_t *d_tmp, hipStream_t s) {
+    if (mda == 0 || empty > 1u) return (int)hipErrorInvalidValue;
     const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (ntiles) k_split_reduce<<<ntiles, SCAN_THREADS, 0, s>>>(row, n, mda, d_tmp);
+    if (ntiles) k_split_reduce<<<ntiles, SCAN_THREADS, 0, s>>>(row, n, mda, empty, d_tmp);
     k_split_scan_top<<<1, 1024, 0, s>>>(d_tmp, ntiles);
-    if (ntiles) k_split_scan_down<<<ntiles, SCAN_THREADS, 0, s>>>(row, n, mda, d_tmp, ntiles, cstart);
+    if (ntiles) k_split_scan_down<<<ntiles, SCAN_THREADS, 0, s>>>(row, n, mda, empty, d_tmp, ntiles, cstart);
     else {
         // n == 0: cstart[0] = 0 written by the top kernel's total via a copy
         hipError_t e = hipMemcpyAsync(cstart, d_tmp, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return (int)e;
     }
     return launch_status();
+}
+
+extern "C" int ggak_split_scan(const uint32_t *row, uint32_t n, uint32_t mda, uint32_t *cstart,
+                               uint32_t *d_tmp, hipStream_t s) {
+    return ggak_split_scan_ex(row, n, mda, 1u, cstart, d_tmp, s);
 }
 
 extern "C" int ggak_split_fill(const uint32_t *row, const uint32_t *cstart, uint32_t n, uint32_t m,
