@@ -328,14 +328,14 @@ __global__ __launch_bounds__(BLOCK) void k_msbfs(gga_dev_graph g, gga_dev_graph 
     const ListOut o = bfs_list(g, b);
     const uint32_t ntiles = GGA_TILES(g.n);
     uint32_t level = 0, filled = 1;   // front slots written: 0 (the sources), then one per level
+    uint32_t dir = 0;                 // the first level pushes from the sources' list
     for (; level < max_levels; level++) {
         if (threadIdx.x == 0) {
-            s_state[0] = b.ctl[GGA_CTL_DIR];
             s_state[1] = b.ctl[GGA_CTL_TAIL(0)];
             s_state[2] = __hip_atomic_load(&b.ctl[GGA_CTL_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        const uint32_t dir = s_state[0], nq = s_state[1] < b.qcap ? s_state[1] : b.qcap, err = s_state[2];
+        const uint32_t nq = s_state[1] < b.qcap ? s_state[1] : b.qcap, err = s_state[2];
         __syncthreads();
         if (err) break;
         const uint64_t *front = front_at(b, g.n, level);
@@ -356,37 +356,33 @@ __global__ __launch_bounds__(BLOCK) void k_msbfs(gga_dev_graph g, gga_dev_graph 
             b.part[4u * blockIdx.x + 3] = mr;
         }
         grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
-        if (blockIdx.x == 0) {
-            uint64_t x[4] = {0, 0, 0, 0};
-            for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-                for (uint32_t k = 0; k < 4; k++) x[k] += b.part[4u * i + k];
-            for (uint32_t k = 0; k < 4; k++) x[k] = block_sum(x[k], s_red);
-            const bool pull = b.alpha && x[1] * (uint64_t)b.alpha > (uint64_t)g.m;
-            if (threadIdx.x == 0) {
-                uint64_t *tot = reinterpret_cast<uint64_t *>(b.ctl + GGA_CTL_CNT);
-                tot[0] += x[2];
-                tot[1] += x[3];
-                b.ctl[GGA_CTL_DONE] = x[0] == 0 ? 1u : 0u;
-                b.ctl[GGA_CTL_DIR] = pull ? 1u : 0u;
-                b.ctl[GGA_CTL_STEPS] = level + 1u;
-                if (pull) b.ctl[GGA_CTL_NPULL] += 1u;
-            }
-            if (!pull && x[0]) compact_scan(o, g.n, s_wave);
+        // every workgroup sums the partials (exact integer sums: the same
+        // decision everywhere) rather than waiting on one workgroup's
+        // decision behind another grid barrier; the partials are rewritten
+        // only after the next level's first barrier, which every workgroup
+        // reaches after reading them
+        uint64_t x[4] = {0, 0, 0, 0};
+        for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+            for (uint32_t k = 0; k < 4; k++) x[k] += b.part[4u * i + k];
+        for (uint32_t k = 0; k < 4; k++) x[k] = block_sum(x[k], s_red);
+        const bool pull = b.alpha && x[1] * (uint64_t)b.alpha > (uint64_t)g.m;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            uint64_t *tot = reinterpret_cast<uint64_t *>(b.ctl + GGA_CTL_CNT);
+            tot[0] += x[2];
+            tot[1] += x[3];
+            b.ctl[GGA_CTL_DONE] = x[0] == 0 ? 1u : 0u;
+            b.ctl[GGA_CTL_DIR] = pull ? 1u : 0u;
+            b.ctl[GGA_CTL_STEPS] = level + 1u;
+            if (pull) b.ctl[GGA_CTL_NPULL] += 1u;
         }
-        grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
-        if (threadIdx.x == 0) {
-            s_state[0] = b.ctl[GGA_CTL_DONE];
-            s_state[1] = b.ctl[GGA_CTL_DIR];
-        }
-        __syncthreads();
-        const uint32_t done = s_state[0], pull = s_state[1];
-        __syncthreads();
-        if (done) break;
+        if (x[0] == 0) break;
         if (!pull) {
-            for (uint32_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x)
-                compact_write_tile(o, g.n, tt, FrontMarked{nfront}, s_wave);
+            // the list of the next level: each workgroup sums the counts of
+            // the tiles before its own, the last tile's stores the length
+            compact_write_tiles(o, g.n, FrontMarked{nfront}, s_wave);
             grid_barrier(b.ctl, gridDim.x, gen, spin_ticks);
         }
+        dir = pull ? 1u : 0u;
     }
     write_rows(g, b, filled < b.nsnap ? filled : b.nsnap);
 }
