@@ -810,10 +810,9 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(gga_dev_graph g, gga_dev_w
             for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
                 compact_count_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
             grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
-            if (blockIdx.x == 0) compact_scan(o, g.n, s_wave);
-            grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
-            for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x)
-                compact_write_tile(o, g.n, t, StampMarked{w.flag, (uint8_t)it}, s_wave);
+            // each workgroup sums the tile counts before its own tiles: no
+            // serial scan between two more grid barriers
+            compact_write_tiles(o, g.n, StampMarked{w.flag, (uint8_t)it}, s_wave);
         }
         grid_barrier(w.ctl, gridDim.x, gen, spin_ticks);
         // near-far: an empty next list is refilled from the deferred vertices,
