@@ -687,8 +687,8 @@ __global__ __launch_bounds__(BLOCK) void k_ms_tail(gga_dev_graph g, gga_dev_ms m
     }
 }
 
-// All steps in one cooperative launch: relax, then the three compaction
-// passes, separated by grid barriers.  Step numbers continue from `step0`.
+// All steps in one cooperative launch: relax, then the compaction's count
+// and write passes, separated by grid barriers.  Step numbers continue from `step0`.
 template <bool COUNT, bool LM>
 __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_dev_ms ms, uint32_t step0,
                                                          uint32_t max_steps, uint64_t spin_ticks) {
@@ -717,9 +717,8 @@ __global__ __launch_bounds__(BLOCK) void k_ms_persistent(gga_dev_graph g, gga_de
             const MaskMarked mk{ms_mnxt(ms, step)};
             for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_count_tile(o, g.n, t, mk, s_wave);
             grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
-            if (blockIdx.x == 0) compact_scan(o, g.n, s_wave);
-            grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
-            for (uint32_t t = blockIdx.x; t < GGA_TILES(g.n); t += gridDim.x) compact_write_tile(o, g.n, t, mk, s_wave);
+            // each workgroup sums the tile counts before its own tiles
+            compact_write_tiles(o, g.n, mk, s_wave);
             grid_barrier(ms.ctl, gridDim.x, gen, spin_ticks);
         }
         ms_finish(g, ms, step);
