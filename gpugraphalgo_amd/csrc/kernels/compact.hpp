@@ -3,11 +3,11 @@
 // A flag step records the next frontier as a per-vertex predicate (a step
 // stamp, a lane mask); the chunk list is then rebuilt from it with no shared
 // counter: count the chunk ids each tile of GGA_TILE vertices contributes,
-// scan the tile counts, write.  The cooperative kernels run the three passes
-// between grid barriers (the scan by one workgroup); the step launches run
-// two, the write pass summing the counts before its own tile
-// (compact_write_tiles).  Lists are in vertex order, so chunks of one vertex
-// stay adjacent.
+// scan the tile counts, write.  Step launches and the cooperative kernels'
+// steps run two passes, the write pass summing the counts before its own tile
+// (compact_write_tiles); the scan as a pass of its own, by one workgroup
+// (compact_scan), is left to the list of a query's sources and the near-far
+// refill.  Lists are in vertex order, so chunks of one vertex stay adjacent.
 #pragma once
 #include "../gga_dev.h"
 #include "sync.hpp"
