@@ -6,7 +6,7 @@
 // scan the tile counts, write.  Step launches and the cooperative kernels'
 // steps run two passes, the write pass summing the counts before its own tile
 // (compact_write_tiles); the scan as a pass of its own, by one workgroup
-// (compact_scan), is left to the list of a query's sources and the near-far
+// (compact_scan), is left to the bitmask BFS's first list and the near-far
 // refill.  Lists are in vertex order, so chunks of one vertex stay adjacent.
 #pragma once
 #include "../gga_dev.h"
