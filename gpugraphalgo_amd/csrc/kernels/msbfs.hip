@@ -177,6 +177,22 @@ __device__ __forceinline__ void write_levels(const gga_dev_msbfs &b, uint32_t n,
     }
 }
 
+// the four words of vertex v's bitmap, as two 16-byte accesses (bitmaps are
+// 32-byte aligned per vertex)
+__device__ __forceinline__ void load_bits(const uint64_t *p, uint32_t v, uint64_t w[WORDS]) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p + (uint64_t)v * WORDS);
+    const uint4 a = q[0], c = q[1];
+    w[0] = (uint64_t)a.x | (uint64_t)a.y << 32;
+    w[1] = (uint64_t)a.z | (uint64_t)a.w << 32;
+    w[2] = (uint64_t)c.x | (uint64_t)c.y << 32;
+    w[3] = (uint64_t)c.z | (uint64_t)c.w << 32;
+}
+__device__ __forceinline__ void store_bits(uint64_t *p, uint32_t v, const uint64_t w[WORDS]) {
+    uint4 *q = reinterpret_cast<uint4 *>(p + (uint64_t)v * WORDS);
+    q[0] = make_uint4((uint32_t)w[0], (uint32_t)(w[0] >> 32), (uint32_t)w[1], (uint32_t)(w[1] >> 32));
+    q[1] = make_uint4((uint32_t)w[2], (uint32_t)(w[2] >> 32), (uint32_t)w[3], (uint32_t)(w[3] >> 32));
+}
+
 struct LevelSums {
     uint64_t nf = 0, mf = 0, reached = 0, mreach = 0;
 };
@@ -192,23 +208,37 @@ __device__ __forceinline__ void finalize_tile(const gga_dev_graph &g, const gga_
         const uint64_t vv = v0 + (uint64_t)i * CBLOCK + threadIdx.x;
         const bool valid = vv < g.n;
         const uint32_t v = valid ? (uint32_t)vv : 0u;
-        uint64_t any = 0;
+        // a vertex's four words move as two 16-byte accesses
+        uint64_t nx[WORDS] = {0, 0, 0, 0}, sn[WORDS] = {0, 0, 0, 0}, nw[WORDS];
+        if (valid) {
+            load_bits(b.next, v, nx);
+            load_bits(b.seen, v, sn);
+        }
+        uint64_t any = 0, anx = 0;
         uint32_t pc = 0;
 #pragma unroll
         for (uint32_t j = 0; j < WORDS; j++) {
-            uint64_t nw = 0;
-            if (valid) {
-                const uint64_t nx = b.next[(uint64_t)v * WORDS + j];
-                const uint64_t sn = b.seen[(uint64_t)v * WORDS + j];
-                nw = nx & ~sn;
-                if (nx) b.next[(uint64_t)v * WORDS + j] = 0ull;
-                if (nw) b.seen[(uint64_t)v * WORDS + j] = sn | nw;
-                nfront[(uint64_t)v * WORDS + j] = nw;
-            }
-            any |= nw;
-            pc += (uint32_t)__popcll(nw);
-            if (direct) write_levels(b, g.n, v, valid, j, nw, level + 1u);
+            nw[j] = nx[j] & ~sn[j];
+            any |= nw[j];
+            anx |= nx[j];
+            pc += (uint32_t)__popcll(nw[j]);
         }
+        if (valid) {
+            if (anx) {
+                const uint64_t z[WORDS] = {0, 0, 0, 0};
+                store_bits(b.next, v, z);
+            }
+            if (any) {
+                uint64_t ns[WORDS];
+#pragma unroll
+                for (uint32_t j = 0; j < WORDS; j++) ns[j] = sn[j] | nw[j];
+                store_bits(b.seen, v, ns);
+            }
+            store_bits(nfront, v, nw);
+        }
+        if (direct)
+#pragma unroll
+            for (uint32_t j = 0; j < WORDS; j++) write_levels(b, g.n, v, valid, j, nw[j], level + 1u);
         if (valid && any) {
             const uint32_t deg = g.row[v + 1] - g.row[v];
             ls.nf += 1;
@@ -227,7 +257,8 @@ __device__ __forceinline__ void finalize_tile(const gga_dev_graph &g, const gga_
 // slot (slots 0 .. nslots-1; a lane's bit is in at most one, and the slot
 // index is gathered bit-sliced: s0 holds the bits of odd slots, s1 of slots
 // with bit 1 set, ...), inf where the lane never reached the vertex, nothing
-// where it did at a level from nsnap on (written at discovery).
+// where it did at a level from nsnap on (written at discovery).  A vertex
+// no lane reached reads no slot.
 __device__ __forceinline__ void write_rows(const gga_dev_graph &g, const gga_dev_msbfs &b, uint32_t nslots) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t nn = ((uint64_t)g.n + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves loop
@@ -235,25 +266,36 @@ __device__ __forceinline__ void write_rows(const gga_dev_graph &g, const gga_dev
     for (uint64_t vv = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; vv < nn; vv += stride) {
         const bool valid = vv < g.n;
         const uint32_t v = valid ? (uint32_t)vv : 0u;
-        for (uint32_t j = 0; j < WORDS && j * 64u < b.k; j++) {
-            uint64_t found = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, miss = 0;
-            if (valid) {
+        uint64_t sn[WORDS] = {0, 0, 0, 0}, found[WORDS] = {0, 0, 0, 0}, s0[WORDS] = {0, 0, 0, 0},
+                 s1[WORDS] = {0, 0, 0, 0}, s2[WORDS] = {0, 0, 0, 0}, s3[WORDS] = {0, 0, 0, 0};
+        if (valid) {
+            load_bits(b.seen, v, sn);
+            if (sn[0] | sn[1] | sn[2] | sn[3])
                 for (uint32_t s = 0; s < nslots; s++) {
-                    const uint64_t w = b.front[s * slot_words + (uint64_t)v * WORDS + j];
-                    found |= w;
-                    s0 |= (s & 1u) ? w : 0ull;
-                    s1 |= (s & 2u) ? w : 0ull;
-                    s2 |= (s & 4u) ? w : 0ull;
-                    s3 |= (s & 8u) ? w : 0ull;
+                    uint64_t w[WORDS];
+                    load_bits(b.front + s * slot_words, v, w);
+#pragma unroll
+                    for (uint32_t j = 0; j < WORDS; j++) {
+                        found[j] |= w[j];
+                        s0[j] |= (s & 1u) ? w[j] : 0ull;
+                        s1[j] |= (s & 2u) ? w[j] : 0ull;
+                        s2[j] |= (s & 4u) ? w[j] : 0ull;
+                        s3[j] |= (s & 8u) ? w[j] : 0ull;
+                    }
                 }
-                miss = ~b.seen[(uint64_t)v * WORDS + j] & lane_mask(b.k, j);
-            }
-            const uint64_t put = found | miss;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < WORDS; j++) {
+            if (j * 64u >= b.k) break;
+            const uint64_t put = found[j] | (valid ? ~sn[j] & lane_mask(b.k, j) : 0ull);
+            // rows are not read back here: non-temporal stores (2.08 -> 2.06 ms
+            // for 256 BFS on R-MAT s20)
             for (uint32_t t = 0; t < 64u && j * 64u + t < b.k; t++) {
                 if (!((put >> t) & 1ull)) continue;
-                const uint32_t lv = (uint32_t)((s0 >> t) & 1ull) | (uint32_t)((s1 >> t) & 1ull) << 1 |
-                                    (uint32_t)((s2 >> t) & 1ull) << 2 | (uint32_t)((s3 >> t) & 1ull) << 3;
-                b.lev[(uint64_t)(j * 64u + t) * g.n + v] = ((found >> t) & 1ull) ? lv : b.inf;
+                const uint32_t lv = (uint32_t)((s0[j] >> t) & 1ull) | (uint32_t)((s1[j] >> t) & 1ull) << 1 |
+                                    (uint32_t)((s2[j] >> t) & 1ull) << 2 | (uint32_t)((s3[j] >> t) & 1ull) << 3;
+                __builtin_nontemporal_store(((found[j] >> t) & 1ull) ? lv : b.inf,
+                                            &b.lev[(uint64_t)(j * 64u + t) * g.n + v]);
             }
         }
     }
