@@ -23,7 +23,9 @@
 // appear dirtied every line of them again at every wide level (256 BFS on
 // R-MAT s20: about 1.1 ms of 3.3 in the writes).  Levels from nsnap on
 // share one slot and write their values at discovery.
-// One cooperative launch runs all levels; grid barriers separate the phases.
+// One cooperative launch runs all levels; grid barriers separate the phases,
+// and every workgroup sums the level's partials and so takes the same
+// push / pull / end decision without a further barrier.
 #include "../gga_kernels.h"
 #include "compact.hpp"
 #include "sync.hpp"
